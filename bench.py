@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark: TeaLeaf's FP64 linear solve on one MI355X.
+
+Workload (the 2D 4096^2 configuration): the two-rectangle layout on a 10x10
+domain, FP64 PPCG (30 CG presteps for the eigenvalue estimate, 10 inner
+Chebyshev steps), eps 1e-10, dt 0.004.  A step is one timestep, i.e. one
+``tea_leaf()`` solve from its first halo update to the finalised energy
+(the reference's "Tea Solve" time per step).  --warmup W timesteps run
+untimed first, then --steps K are timed; the deck runs W + K timesteps and
+the driver times each solve with a device synchronisation at its end.
+
+Printed: one JSON line with the mean seconds per timestep over the K timed
+steps, the roofline of the dominant kernel (its algorithmic bytes per launch
+over its mean launch time, from HIP events on the chunk's stream during the
+timed steps), and a CPU baseline from the oracle (see cpu_baseline()).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tealeaf-kokkos_amd"))
+
+import decks  # noqa: E402
+import tealeaf_hip  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def cpu_baseline(n: int, threads: int, steps: list[dict], budget_s: float = 20.0) -> dict:
+    """The oracle (C/OpenMP restatement of the reference) timed on a bounded
+    sample of the same workload: on the 4096^2 chunk in its first-timestep
+    state it runs k CG iterations and m PPCG outer iterations (each with the
+    halo updates the reference's driver issues), sized to about `budget_s`
+    seconds.  Seconds per timestep = init + cg_steps * t_cg + outer * t_outer
+    with the iteration counts of the timed GPU steps (the per-step counts the
+    parity tests show the oracle reproduces)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    L = oracle.lib(2)
+    L.tl_oracle_set_reduce_parts(threads)
+    L.tl_oracle_set_threads(threads)
+    ch = tealeaf_hip.ExtChunk(L, 2, n, n)
+    dx = 10.0 / n
+    s = dx / 100.0
+    ch.initialise_chunk(0.0, 0.0, dx, dx)
+    ch.generate_chunk(density=[100.0, 0.1, 0.1], energy=[1e-4, 25.0, 0.1],
+                      xmin=[0, 0 + s, 1 + s], xmax=[0, 1 - s, 6 - s], ymin=[0, 0 + s, 1 + s],
+                      ymax=[0, 2 - s, 6 - s], radius=[0, 0, 0], geometry=[0, 1, 1])
+    ch.update_halo([1, 2, 3], 2)
+    ch.set_field_kernel()
+    t0 = time.perf_counter()
+    ch.update_halo([3, 1], 2)
+    rro = ch.cg_solver_init(1, False, 0.004)
+    ch.update_halo([4, 5], 1)
+    ch.copy_u()
+    t_init = time.perf_counter() - t0
+
+    def cg_iteration(rro):
+        pw = ch.cg_calc_w()
+        rrn = ch.cg_calc_ur(rro / pw)
+        ch.cg_calc_p(rrn / rro)
+        ch.update_halo([4, 5], 1)
+        return rrn
+
+    t0 = time.perf_counter()
+    rro = cg_iteration(rro)
+    t_one = time.perf_counter() - t0
+    k = max(2, min(60, int(0.4 * budget_s / max(t_one, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        rro = cg_iteration(rro)
+    t_cg = (time.perf_counter() - t0) / k
+
+    a, b, theta = oracle.calc_ch_coefs(0.5, 4000.0, 10)
+    ch.ppcg_init(a, b, theta)
+    ch.calculate_residual()
+    ch.update_halo([4, 5], 1)
+
+    def outer(rro):
+        pw = ch.cg_calc_w()
+        ch.cg_calc_ur(rro / pw)
+        ch.ppcg_init_sd()
+        for step in range(1, 11):
+            ch.update_halo([6], 1)
+            ch.ppcg_inner(step)
+        rrn = ch.calculate_2norm(True)
+        ch.cg_calc_p(rrn / rro)
+        ch.update_halo([4, 5], 1)
+        return rrn
+
+    m = max(1, min(20, int(0.5 * budget_s / max(6.0 * t_cg, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(m):
+        rro = outer(rro)
+    t_outer = (time.perf_counter() - t0) / m
+    ch.close()
+
+    per_step = []
+    for st in steps:
+        n_outer = max(st["cheby_steps"] - 1, 0)
+        per_step.append(t_init + st["cg_steps"] * t_cg + n_outer * t_outer)
+    return {
+        "value": statistics.mean(per_step),
+        "unit": "s/timestep",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle on the {n}^2 chunk: {k} CG iterations ({t_cg * 1e3:.1f} ms each) and {m} PPCG "
+                   f"outer iterations ({t_outer * 1e3:.1f} ms each), scaled by the timed steps' iteration counts"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=9, help="timed timesteps")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps before them")
+    ap.add_argument("--n", type=int, default=4096, help="cells per side")
+    ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the solve runs as a single chunk on one MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    total = args.warmup + args.steps
+    deck = decks.two_rect_2d(args.n, args.solver, steps=total, presteps=30, inner=10,
+                             max_iters=10000 if args.solver != "jacobi" else 20000)
+    L = tealeaf_hip.lib(2)
+    L.tl_set_kernel_timing(args.warmup + 1)
+    t0 = time.perf_counter()
+    rep = tealeaf_hip.run_deck(deck, dim=2)
+    wall = time.perf_counter() - t0
+    L.tl_set_kernel_timing(0)
+
+    timed = rep["steps"][args.warmup:]
+    solve = [s["solve_s"] for s in timed]
+    s_per_step = statistics.mean(solve)
+    cells = rep["cells"]
+
+    kern = rep.get("kernels", {})
+    dom_name, dom = max(kern.items(), key=lambda kv: kv[1]["ms"]) if kern else (None, None)
+    roofline = None
+    if dom:
+        avg_s = dom["ms"] / dom["count"] * 1e-3
+        bytes_per_launch = dom["bytes_per_cell"] * cells
+        achieved = bytes_per_launch / avg_s / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                traffic = json.load(f).get(f"{dom_name}@{args.n}")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": dom_name, "avg_launch_us": round(avg_s * 1e6, 2),
+                    "bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.n, args.cpu_threads, timed)
+
+    out = {
+        "metric": "solve s/timestep @eps 1e-10 (FP64 PPCG); HBM GB/s",
+        "value": round(s_per_step, 6),
+        "unit": "s/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * s_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D {args.n}x{args.n} {args.solver.upper()} two-rectangle deck, eps 1e-10, "
+                               f"dt 0.004, presteps 30, inner steps 10",
+                   "cells": cells, "parallelism": "single chunk, one GPU"},
+        "median_s_per_step": round(statistics.median(solve), 6),
+        "iterations": [s["iters"] for s in timed],
+        "temp": [s["temp"] for s in timed],
+        "step1_s": rep["steps"][0]["solve_s"] if args.warmup else None,
+        "wall_s": round(wall, 3),
+        "kernels": {k: {"launches": v["count"], "avg_us": round(1e3 * v["ms"] / v["count"], 2),
+                        "gbs": round(v["bytes_per_cell"] * cells / (v["ms"] / v["count"] * 1e-3) / 1e9, 1)}
+                    for k, v in kern.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,136 @@
+/*
+ * C ABI of libtealeaf2d_hip.so: TeaLeaf's 2D kernel layer (5-point stencil)
+ * on an MI355X (gfx950).
+ *
+ * The ext_*_ entry points keep the names, argument order and conventions of
+ * the reference's extension interface (tealeaf_2d/ext_core.cpp,
+ * tealeaf_2d/ext_chunk.cpp), which the Fortran driver binds through implicit
+ * interfaces:
+ *   - every argument is passed by pointer;
+ *   - a chunk is a 1-based index into a process-global list filled by
+ *     ext_init_ (Chunks[*chunk-1], ext_chunk.cpp:15);
+ *   - double* outputs are overwritten with the chunk's reduction; the value
+ *     is in host memory when the call returns;
+ *   - Fortran LOGICALs arrive as const int* or bool* (low byte read);
+ *   - errors print a message and exit(1) (ext_core.cpp:571-583) unless
+ *     tl_set_abort_on_error(0) was called, in which case the call returns
+ *     and tl_last_error() holds the message.
+ * Device state is resident in HBM; the layout is internal (fields are read
+ * and written in the reference's flat layout through tl_get_field /
+ * tl_set_field).  A stencil input's depth-1 halo across an external face is
+ * taken to be the reflective mirror of the interior, which is what
+ * ext_update_halo_kernel_ establishes and what the reference's driver keeps
+ * in place before every stencil call.
+ */
+#ifndef TEALEAF_EXT2D_H
+#define TEALEAF_EXT2D_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ext_chunk.cpp:18-25.  Creates chunk n+1 of xMax*yMax interior cells
+ * (the Fortran caller passes 13 further arrays, start.f90:77-90, which are
+ * not read). */
+void ext_init_(int* xMax, int* yMax, int* rank);
+/* ext_chunk.cpp:28-33 */
+void ext_finalise_(void);
+
+/* ext_core.cpp:33-52 (vertices, cell centres, volume, areas) */
+void ext_initialise_chunk_(const int* chunk, const double* xMin, const double* yMin, const double* dx,
+                           const double* dy);
+/* ext_core.cpp:57-126 (paints states 2..n over state 1) */
+void ext_generate_chunk_(const int* chunk, const int* numStates, const double* stateDensity,
+                         const double* stateEnergy, const double* stateXMin, const double* stateXMax,
+                         const double* stateYMin, const double* stateYMax, const double* stateRadius,
+                         const int* stateGeometry, const int* rectParam, const int* circParam,
+                         const int* pointParam);
+/* ext_core.cpp:249-256 (energy1 = energy0) */
+void ext_set_field_kernel_(const int* chunk);
+/* ext_core.cpp:130-147 (interior sums of V, V*rho, V*rho*e0, V*rho*u) */
+void ext_field_summary_kernel_(const int* chunk, double* vol, double* mass, double* ie, double* temp);
+/* ext_core.cpp:305-336 (reflective copy on faces whose neighbour is -1;
+ * fields[6] flags density, energy0, energy1, u, p, sd; depth 1 or 2) */
+void ext_update_halo_kernel_(const int* chunk, const int* chunkNeighbours, const int* fields, const int* depth);
+/* ext_core.cpp:151-245.  Multi-chunk face packing: not supported (one chunk
+ * per GPU); calling it is an error. */
+void ext_pack_message_(const int* chunk, const int* fields, const int* offsets, const int* depth, const int* face,
+                       double* buffer, bool* pack);
+
+/* ext_core.cpp:377-413 */
+void ext_cg_solver_init_(const int* chunk, const int* coefficient, const int* preconditioner, double* dt,
+                         double* rro);
+/* ext_core.cpp:417-429 (w = A p, *pw = p.w) */
+void ext_cg_calc_w_(const int* chunk, double* pw);
+/* ext_core.cpp:433-447 (u += alpha p, r -= alpha w, *rrn = r.r or r.z) */
+void ext_cg_calc_ur_(const int* chunk, const double* alpha, double* rrn);
+/* ext_core.cpp:451-464 (p = beta p + r or z) */
+void ext_cg_calc_p_(const int* chunk, const double* beta);
+
+/* ext_core.cpp:468-497 */
+void ext_cheby_solver_init_(const int* chunk, double* alphas, double* betas, const double* theta,
+                            const int* preconditioner, const int* maxChebyIters);
+/* ext_core.cpp:501-515 (chebyCalcStep is 1-based) */
+void ext_cheby_solver_iterate_(const int* chunk, const int* chebyCalcStep);
+
+/* ext_core.cpp:519-539 */
+void ext_ppcg_init_(const int* chunk, double* alphas, double* betas, const double* theta,
+                    const int* maxChebyIters);
+/* ext_core.cpp:543-550 */
+void ext_ppcg_init_sd_(const int* chunk);
+/* ext_core.cpp:554-567 (currentStep is 1-based) */
+void ext_ppcg_inner_(const int* chunk, int* currentStep);
+
+/* ext_core.cpp:340-358 */
+void ext_jacobi_kernel_init_(const int* chunk, const int* coefficient, const double* dt);
+/* ext_core.cpp:362-373 (*error = sum |u_new - u_old|) */
+void ext_jacobi_kernel_solve_(const int* chunk, double* error);
+
+/* ext_core.cpp:260-267 (u0 = u) */
+void ext_solver_copy_u_(const int* chunk);
+/* ext_core.cpp:271-278 (r = u0 - A u) */
+void ext_calculate_residual_(const int* chunk);
+/* ext_core.cpp:282-291 (*norm = sum r^2 if *normArray else sum u0^2) */
+void ext_calculate_2norm_(const int* chunk, double* norm, bool* normArray);
+/* ext_core.cpp:295-302 (energy1 = u / rho) */
+void ext_solver_finalise_(const int* chunk);
+
+/* timer_c.cpp:30 (wall-clock seconds) */
+void timer_c_(double* elapsed);
+
+/* ---- Beside the reference interface --------------------------------- */
+
+/* The solver driver (tea_solve.f90 and its callers restated in C++): runs a
+ * tea.in deck end to end on the GPU and writes a JSON report (per step:
+ * iterations, error, solver diagnostics, field summary, solve seconds) into
+ * json[0..cap).  max_steps > 0 stops after that many timesteps.  Returns 0,
+ * -1 on error (tl_driver_error()), -2 when cap is too small. */
+int tl_run_deck(const char* deck, char* json, int cap, int max_steps);
+const char* tl_driver_error(void);
+/* Per-launch kernel timing with HIP events on the chunk's stream for the
+ * timesteps from `from_step` on (0 = off); tl_run_deck's report then has a
+ * "kernels" table: launches, summed milliseconds, algorithmic bytes/cell. */
+void tl_set_kernel_timing(int from_step);
+
+/* Error handling of the ext_*_ calls. */
+void tl_set_abort_on_error(int on);
+const char* tl_last_error(void);
+void tl_clear_error(void);
+
+/* Test access to chunk state in the reference's flat layout (X = nx+4
+ * fastest, Y = ny+4 rows).  Field ids: 0 density, 1 energy0, 2 energy1, 3 u,
+ * 4 u0, 5 p, 6 r, 7 mi, 8 w, 9 z, 10 kx, 11 ky, 12 kz (3D only), 13 sd,
+ * 14 volume.  dims receives X, Y, Z, nx, ny, nz. */
+int tl_dim(void);
+int tl_num_fields(void);
+int tl_chunk_dims(const int* chunk, int* dims);
+int tl_get_field(const int* chunk, int field, double* out);
+int tl_set_field(const int* chunk, int field, const double* in);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

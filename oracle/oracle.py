@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes
 import json
 import os
+import re
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -29,6 +30,14 @@ def build() -> None:
 
 
 _LIBS: dict[int, ctypes.CDLL] = {}
+
+
+def parse_report(text: str) -> dict:
+    """Load a driver report; C's printf spells non-finite values nan/inf,
+    which become JSON's NaN/Infinity."""
+    text = re.sub(r"(:\s*)-?nan\b", r"\1NaN", text)
+    text = re.sub(r"(:\s*)(-?)inf\b", r"\1\2Infinity", text)
+    return json.loads(text)
 
 
 def lib(dim: int) -> ctypes.CDLL:
@@ -73,7 +82,7 @@ def run_deck(deck: str, dim: int = 2, reduce_parts: int = 1, threads: int = 0,
     rc = L.tl_run_deck(deck.encode(), buf, json_cap, int(max_steps))
     if rc != 0:
         raise RuntimeError(f"oracle run failed ({rc}): {L.tl_last_error().decode()}")
-    return json.loads(buf.value.decode())
+    return parse_report(buf.value.decode())
 
 
 def calc_eigenvalues(alphas, betas):

@@ -1,0 +1,162 @@
+// Host-side view of one TeaLeaf chunk resident in HBM, and the operations
+// the ext_* C-ABI and the solver driver run on it.  Each operation is one
+// (or, for setup, a few) hand-written gfx950 kernel launches on the chunk's
+// stream; operations that reduce return the FP64 total to the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tl_device.hpp"
+
+namespace tl {
+
+// Field ids of the tl_get_field / tl_set_field hooks.
+enum FieldId {
+    F_DENSITY = 0, F_ENERGY0, F_ENERGY1, F_U, F_U0, F_P, F_R, F_MI, F_W, F_Z,
+    F_KX, F_KY, F_KZ, F_SD, F_VOLUME, F_COUNT
+};
+
+// Fortran-side codes (tealeaf_2d/ext_shared.hpp:10-29, data.f90:39-64).
+enum { CHUNK_LEFT = 1, CHUNK_RIGHT = 2, CHUNK_BOTTOM = 3, CHUNK_TOP = 4, CHUNK_BACK = 5, CHUNK_FRONT = 6 };
+enum { FIELD_DENSITY = 1, FIELD_ENERGY0 = 2, FIELD_ENERGY1 = 3, FIELD_U = 4, FIELD_P = 5, FIELD_SD = 6, NUM_FIELDS = 6 };
+enum { CONDUCTIVITY = 1, RECIP_CONDUCTIVITY = 2 };
+enum { G_RECT = 1, G_CIRC = 2, G_POINT = 3 };
+constexpr int EXTERNAL_FACE = -1;
+#if TL_DIM == 3
+constexpr int NUM_FACES = 6;
+#else
+constexpr int NUM_FACES = 4;
+#endif
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define TL_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::tl::Error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Result slots of the host-mapped reduction buffer (4 doubles each).
+enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
+
+// Kernels timed by the optional per-launch event timer.
+enum KernelId {
+    K_CG_INIT = 0, K_CG_W, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_JACOBI,
+    K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_COUNT
+};
+const char* kernel_name(int k);
+// Algorithmic HBM bytes per interior cell of one launch (each array read or
+// written once), 0 where the kernel is not a full sweep.
+double kernel_bytes_per_cell(int k);
+
+// Per-launch timing with HIP events recorded on the chunk's stream; pending
+// pairs are resolved at the next host synchronisation.
+struct KernelTimer {
+    bool on = false;
+    double ms[K_COUNT] = {};
+    long count[K_COUNT] = {};
+    struct Pending {
+        int k;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t take();
+    void begin(int k, hipStream_t s, Pending& p);
+    void end(Pending& p, hipStream_t s);
+    void flush();
+    ~KernelTimer();
+};
+// Timesteps from which the driver turns each chunk's timer on (0 = never).
+void set_kernel_timing_from(int step);
+int kernel_timing_from();
+
+struct Chunk {
+    Geom g{};
+    int rank = 0;
+    double dx = 0, dy = 0, dz = 0;
+    double theta = 0;
+    bool preconditioner = false;
+
+    double* f[F_COUNT] = {};
+    double* u_alt = nullptr;   // ping-pong partner of u (Chebyshev)
+    double* sd_alt = nullptr;  // ping-pong partner of sd (PPCG inner steps)
+
+    // Mesh coordinates (device, reference indexing).
+    double *vertexX = nullptr, *vertexY = nullptr, *vertexZ = nullptr;
+    double *cellX = nullptr, *cellY = nullptr, *cellZ = nullptr;
+
+    // Chebyshev / PPCG coefficients, copied from the caller at init.
+    std::vector<double> alphas, betas;
+
+    // Reduction scratch.
+    double* partials = nullptr;
+    unsigned* ticket = nullptr;
+    double* d_result = nullptr;           // [S_COUNT * 4] device
+    volatile double* h_result = nullptr;  // [S_COUNT * 4] host-mapped
+    double* h_result_dev = nullptr;       // device alias of h_result
+
+    hipStream_t stream = nullptr;
+    mutable KernelTimer timer;
+
+    Chunk(int nx, int ny, int nz, int rank);
+    ~Chunk();
+    Chunk(const Chunk&) = delete;
+    Chunk& operator=(const Chunk&) = delete;
+
+    Reduce reducer(Slot s) const;
+    double result(Slot s, int k = 0) const;  // waits for the stream
+    void sync() const;
+
+    // Field access in the reference's flat layout (X*Y*Z doubles).
+    void get_field(int id, double* host) const;
+    void set_field(int id, const double* host);
+};
+
+// ---- operations (tl_kernels.hip) ----------------------------------------
+void op_initialise_chunk(Chunk& c, double xmin, double ymin, double zmin, double dx, double dy, double dz);
+void op_generate_chunk(Chunk& c, int nstates, const double* density, const double* energy,
+                       const double* xmin, const double* xmax, const double* ymin, const double* ymax,
+                       const double* zmin, const double* zmax, const double* radius, const int* geometry,
+                       int rect, int circ, int point);
+void op_set_field(Chunk& c);
+void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* temp);
+// fields[6] flags in FIELD_* order, neighbours[NUM_FACES] (EXTERNAL_FACE marks a reflective face)
+void op_update_halo(Chunk& c, const int* neighbours, const int* fields, int depth);
+
+double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt);
+double op_cg_calc_w(Chunk& c);
+double op_cg_calc_ur(Chunk& c, double alpha);
+void op_cg_calc_p(Chunk& c, double beta);
+
+// Chebyshev, PPCG and Jacobi come in two forms.  The plain form keeps the
+// reference's memory effects exactly (scratch w written, updates in place).
+// The fused form computes the same values in one sweep, skips the write of
+// the never-read scratch w, and ping-pongs u (Chebyshev, Jacobi) or sd
+// (PPCG) between two buffers, so a cell's stencil inputs are never
+// overwritten while its neighbours still need them.
+void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, double theta,
+                   bool preconditioner, bool fused);
+void op_cheby_iterate(Chunk& c, int step1, bool fused);  // 1-based step
+
+void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, double theta);
+void op_ppcg_init_sd(Chunk& c);
+void op_ppcg_inner(Chunk& c, int step1, bool fused);  // 1-based step
+
+void op_jacobi_init(Chunk& c, int coefficient, double dt);
+double op_jacobi_solve(Chunk& c, bool fused);
+
+void op_copy_u(Chunk& c);
+void op_calc_residual(Chunk& c);
+double op_calc_2norm(Chunk& c, bool norm_r);
+void op_finalise(Chunk& c);
+
+}  // namespace tl

@@ -1,0 +1,195 @@
+// Device-side building blocks shared by every TeaLeaf kernel on gfx950:
+// the padded HBM layout of a chunk, the launch geometry (64-lane column
+// strips, XCD-aware tile order), clamped stencil loads, and a deterministic
+// single-pass grid reduction that hands its result to the host through
+// mapped pinned memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#ifndef TL_DIM
+#error "compile with -DTL_DIM=2 or -DTL_DIM=3"
+#endif
+
+namespace tl {
+
+constexpr int HALO = 2;
+// Interior column kk = 2 of every row starts on a 256-byte boundary: a row
+// begins XOFF doubles before the first halo column.
+constexpr int XOFF = 30;
+constexpr int ROW_ALIGN = 32;  // doubles (256 B)
+
+constexpr int WAVE = 64;
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int BLOCK = WAVE * WAVES_PER_BLOCK;
+constexpr int NUM_XCD = 8;
+
+// Geometry of one chunk.  Reference coordinates (kk, jj, ii) run over the
+// padded extents X = nx + 4, Y = ny + 4, Z = nz + 4 (Z = 1 in 2D), interior
+// cells are [2, n + 2) on each axis (tealeaf_2d/ext_shared.hpp:47-49).
+// Device element of (kk, jj, ii) is base + ii * sy + jj * sx + kk.
+struct Geom {
+    int nx, ny, nz;
+    int X, Y, Z;
+    long sx, sy;
+    long base;
+    long alloc;  // doubles per field allocation
+    // Launch tiling: 64 columns x (4 waves) per block, each lane walks a
+    // strip of `strip` cells along the slowest interior axis.
+    int strip;
+    int tiles_x, tiles_y, tiles_s;
+    long nblocks;
+};
+
+__host__ __device__ inline long gidx(const Geom& g, int kk, int jj, int ii)
+{
+    return g.base + (long)ii * g.sy + (long)jj * g.sx + kk;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (w -> XCD w % 8).
+// Remap so each XCD owns one contiguous run of tiles, which keeps vertical
+// stencil neighbours inside that XCD's L2.
+__device__ inline long xcd_tile(long w, long n)
+{
+    const long x = w % NUM_XCD, m = w / NUM_XCD;
+    const long q = n / NUM_XCD, r = n % NUM_XCD;
+    return x * q + (x < r ? x : r) + m;
+}
+
+// Strip owned by the current lane.  In 2D the strip runs along y (rows);
+// in 3D along z (planes) with each wave owning one row of the y tile.
+struct Strip {
+    long i;      // device index of the first cell
+    long step;   // stride between consecutive cells of the strip
+    int kk, jj, ii;
+    int count;   // cells in the strip (0 when the lane is outside the mesh)
+};
+
+__device__ inline Strip make_strip(const Geom& g)
+{
+    const long t = xcd_tile(blockIdx.x, g.nblocks);
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int tx = (int)(t % g.tiles_x);
+    const long rest = t / g.tiles_x;
+    Strip s;
+    s.kk = HALO + tx * WAVE + lane;
+#if TL_DIM == 3
+    const int ty = (int)(rest % g.tiles_y);
+    const int ts = (int)(rest / g.tiles_y);
+    s.jj = HALO + ty * WAVES_PER_BLOCK + wave;
+    s.ii = HALO + ts * g.strip;
+    int c = g.nz + HALO - s.ii;
+    s.step = g.sy;
+    const bool ok = s.kk < g.nx + HALO && s.jj < g.ny + HALO;
+#else
+    const int ts = (int)rest;
+    s.jj = HALO + (ts * WAVES_PER_BLOCK + wave) * g.strip;
+    s.ii = 0;
+    int c = g.ny + HALO - s.jj;
+    s.step = g.sx;
+    const bool ok = s.kk < g.nx + HALO;
+#endif
+    if (c > g.strip) c = g.strip;
+    if (c < 0 || !ok) c = 0;
+    s.count = c;
+    s.i = gidx(g, s.kk, s.jj, s.ii);
+    return s;
+}
+
+__device__ inline void strip_advance(Strip& s)
+{
+    s.i += s.step;
+#if TL_DIM == 3
+    s.ii++;
+#else
+    s.jj++;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic single-pass grid reduction of NV doubles.
+//
+// Every lane sums its own cells in strip order; the wave folds with a fixed
+// xor butterfly; lane 0 of each wave stores to LDS; wave partials are added
+// in wave order; the block partial goes to `partials`.  The last block to
+// arrive (agent-scope atomic ticket) adds all block partials in block order
+// with the same pattern, writes the totals to device memory and to the
+// host-mapped result slot, and re-arms the ticket.  The order depends only on
+// the mesh and the launch geometry, so results are bitwise reproducible.
+struct Reduce {
+    double* partials;   // [nblocks * NV]
+    unsigned* ticket;   // armed at 0
+    double* dout;       // device copy of the totals [NV]
+    double* hout;       // host-mapped copy of the totals [NV]
+};
+
+template <int NV>
+__device__ inline void wave_fold(double (&v)[NV])
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], off, WAVE);
+    }
+}
+
+template <int NV>
+__device__ inline void block_fold(double (&v)[NV], double (*lds)[WAVES_PER_BLOCK])
+{
+    wave_fold<NV>(v);
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = lds[k][0];
+#pragma unroll
+            for (int w = 1; w < WAVES_PER_BLOCK; ++w) s += lds[k][w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <int NV>
+__device__ inline void grid_reduce(double (&v)[NV], const Reduce& r, long nblocks)
+{
+    __shared__ double lds[NV][WAVES_PER_BLOCK];
+    __shared__ int last;
+    block_fold<NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) r.partials[(long)blockIdx.x * NV + k] = v[k];
+        __threadfence();
+        const unsigned t = atomicAdd(r.ticket, 1u);
+        last = (t == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (long b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += r.partials[b * NV + k];
+    }
+    block_fold<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            r.dout[k] = acc[k];
+            r.hout[k] = acc[k];
+        }
+        *r.ticket = 0u;
+        __threadfence_system();
+    }
+}
+
+}  // namespace tl

@@ -1,0 +1,611 @@
+// Host driver: TeaLeaf's Fortran caller of the ext_* layer, restated in C++
+// so that a deck runs end to end on the GPU with no Fortran.  It follows
+//   read_input.f90 / parse.f90       deck keywords, defaults, state shrink
+//   start.f90, diffuse.f90           set-up order and the timestep loop
+//   tea_solve.f90                    CG, Chebyshev, PPCG and Jacobi control
+//   tea_leaf_cheby.f90:233-360       tqli, eigenvalues, Chebyshev coefficients
+//   field_summary.f90                per-step totals
+// and calls the chunk operations directly, choosing the fused kernels.
+// Per-iteration halo updates of u, p and sd are not issued: every stencil
+// kernel already reads the reflective mirror across external faces, which is
+// what those updates would have put in memory.
+#include "tl_chunk.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace tl {
+namespace {
+
+struct State {
+    bool defined = false;
+    double density = 0, energy = 0, xmin = 0, xmax = 0, ymin = 0, ymax = 0, zmin = 0, zmax = 0, radius = 0;
+    int geometry = 0;
+};
+
+struct Deck {
+    // Defaults of read_input.f90:39-80.  Unsuffixed Fortran literals are
+    // single precision, widened to double exactly as the reference does.
+    double xmin = 0.0, ymin = 0.0, zmin = 0.0, xmax = 100.0, ymax = 100.0, zmax = 100.0;
+    int x_cells = 10, y_cells = 10, z_cells = 10;
+    double dtinit = (double)0.1f;
+    int max_iters = 1000;
+    double eps = (double)1.0e-10f;
+    double end_time = 10.0;
+    int end_step = 640000;  // g_ibig (data.f90:28)
+    int summary_frequency = 10;
+    int coefficient = CONDUCTIVITY;
+    bool errswitch = false;
+    int presteps = 30;
+    double epslim = (double)1e-5f;
+    bool check_result = false;
+    int ppcg_inner_steps = 10;
+    bool preconditioner = false;
+    bool use_cheby = false, use_cg = false, use_ppcg = false, use_jacobi = true;
+    int test_problem = 0;
+    std::vector<State> states;
+};
+
+// ---- deck reader (parse.f90:130-300, read_input.f90:83-297) --------------
+
+// One content line of the *tea block, already cleaned.
+std::vector<std::string> deck_lines(const std::string& text)
+{
+    std::vector<std::string> out;
+    std::string here;
+    std::istringstream in(text);
+    std::string raw;
+    while (std::getline(in, raw)) {
+        std::string l = raw.substr(0, std::min<size_t>(raw.size(), 100));  // READ (a100)
+        for (char& ch : l)
+            if ((unsigned char)ch < 32 || (unsigned char)ch > 128) ch = ' ';
+        auto trim = [](std::string& s) {
+            size_t a = s.find_first_not_of(' ');
+            if (a == std::string::npos) { s.clear(); return; }
+            size_t b = s.find_last_not_of(' ');
+            s = s.substr(a, b - a + 1);
+        };
+        trim(l);
+        for (char cut : {'!', ';'}) {
+            size_t k = l.find(cut);
+            if (k != std::string::npos) { l.resize(k); trim(l); }
+        }
+        for (char& ch : l) {
+            if (ch >= 'A' && ch <= 'Z') ch = (char)(ch + 32);
+            if (ch == '=' || ch == ',') ch = ' ';
+        }
+        if (!l.empty() && l[0] == '*') {
+            std::string nugget = l.substr(0, l.find(' '));
+            if (nugget.compare(1, 3, "end") == 0) {
+                std::string want = "*" + nugget.substr(4);
+                size_t last = here.rfind('*');
+                if (last == std::string::npos || here.substr(last) != want) throw Error("Unmatched */*end pair.");
+                if (here == "*tea") return out;
+                here.resize(last);
+            } else {
+                here += nugget;
+            }
+            continue;
+        }
+        if (here == "*tea" && !l.empty()) out.push_back(l);
+    }
+    return out;
+}
+
+struct Words {
+    const std::vector<std::string>& lines;
+    size_t line = 0, pos = 0;
+    bool started = false;
+
+    bool next_line()
+    {
+        if (started) ++line;
+        started = true;
+        pos = 0;
+        return line < lines.size();
+    }
+    // parse_getword: next blank-separated word; with `wrap` an empty line
+    // end continues on the next line.
+    std::string word(bool wrap)
+    {
+        for (;;) {
+            if (line >= lines.size()) return "";
+            const std::string& l = lines[line];
+            while (pos < l.size() && l[pos] == ' ') ++pos;
+            if (pos < l.size()) {
+                size_t e = l.find(' ', pos);
+                if (e == std::string::npos) e = l.size();
+                std::string w = l.substr(pos, e - pos);
+                pos = e;
+                return w;
+            }
+            if (!wrap) return "";
+            ++line;
+            pos = 0;
+        }
+    }
+};
+
+int parse_int(const std::string& w)  // READ (I7)
+{
+    std::string s = w.substr(0, std::min<size_t>(w.size(), 7));
+    char* end = nullptr;
+    long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str() || *end) throw Error("Error attempting to convert to integer:" + w);
+    return (int)v;
+}
+
+double parse_real(const std::string& w)  // '.0' appended when no '.', then READ (E27.20)
+{
+    std::string s = w.find('.') == std::string::npos ? w + ".0" : w;
+    char* end = nullptr;
+    double v = std::strtod(s.c_str(), &end);
+    if (end == s.c_str() || *end) throw Error("Error attempting to convert to real:" + w);
+    return v;
+}
+
+Deck parse_deck(const std::string& text)
+{
+    Deck d;
+    const std::vector<std::string> lines = deck_lines(text);
+    int state_max = 0;
+    {
+        Words w{lines};
+        while (w.next_line()) {
+            for (std::string word = w.word(false); !word.empty(); word = w.word(false)) {
+                if (word == "state") {
+                    state_max = std::max(state_max, parse_int(w.word(true)));
+                    break;
+                }
+            }
+        }
+    }
+    if (state_max < 1) throw Error("No states defined.");
+    d.states.resize(state_max);
+    Words w{lines};
+    while (w.next_line()) {
+        for (std::string k = w.word(false); !k.empty(); k = w.word(false)) {
+            auto R = [&] { return parse_real(w.word(true)); };
+            auto I = [&] { return parse_int(w.word(true)); };
+            if (k == "initial_timestep") d.dtinit = R();
+            else if (k == "end_time") d.end_time = R();
+            else if (k == "end_step") d.end_step = I();
+            else if (k == "xmin") d.xmin = R();
+            else if (k == "xmax") d.xmax = R();
+            else if (k == "ymin") d.ymin = R();
+            else if (k == "ymax") d.ymax = R();
+            else if (k == "x_cells") d.x_cells = I();
+            else if (k == "y_cells") d.y_cells = I();
+#if TL_DIM == 3
+            else if (k == "zmin") d.zmin = R();
+            else if (k == "zmax") d.zmax = R();
+            else if (k == "z_cells") d.z_cells = I();
+#endif
+            else if (k == "visit_frequency") (void)I();
+            else if (k == "summary_frequency") d.summary_frequency = I();
+            else if (k == "tl_ch_cg_presteps") d.presteps = I();
+            else if (k == "tl_ppcg_inner_steps") d.ppcg_inner_steps = I();
+            else if (k == "tl_ch_cg_epslim") d.epslim = R();
+            else if (k == "tl_check_result") d.check_result = true;
+            else if (k == "tl_ch_cg_errswitch") d.errswitch = true;
+            else if (k == "tl_preconditioner_on") d.preconditioner = true;
+            else if (k == "tl_use_jacobi") { d.use_cheby = d.use_cg = d.use_ppcg = false; d.use_jacobi = true; }
+            else if (k == "tl_use_cg") { d.use_cheby = d.use_ppcg = d.use_jacobi = false; d.use_cg = true; }
+            else if (k == "tl_use_ppcg") { d.use_cheby = d.use_cg = d.use_jacobi = false; d.use_ppcg = true; }
+            else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = false; d.use_cheby = true; }
+            else if (k == "tl_max_iters") d.max_iters = I();
+            else if (k == "tl_eps") d.eps = R();
+            else if (k == "tl_coefficient_density") d.coefficient = CONDUCTIVITY;
+            else if (k == "tl_coefficient_inverrse_density") d.coefficient = RECIP_CONDUCTIVITY;
+            else if (k == "test_problem") d.test_problem = I();
+            else if (k == "state") {
+                const int s = I();
+                if (s < 1 || s > state_max) throw Error("state " + std::to_string(s) + " out of range");
+                State& st = d.states[s - 1];
+                if (st.defined) throw Error("State defined twice.");
+                st.defined = true;
+                for (std::string q = w.word(false); !q.empty(); q = w.word(false)) {
+                    if (q == "xmin") st.xmin = R();
+                    else if (q == "ymin") st.ymin = R();
+                    else if (q == "xmax") st.xmax = R();
+                    else if (q == "ymax") st.ymax = R();
+#if TL_DIM == 3
+                    else if (q == "zmin") st.zmin = R();
+                    else if (q == "zmax") st.zmax = R();
+#endif
+                    else if (q == "radius") st.radius = R();
+                    else if (q == "density") st.density = R();
+                    else if (q == "energy") st.energy = R();
+                    else if (q == "geometry") {
+                        const std::string gname = w.word(true);
+#if TL_DIM == 3
+                        if (gname == "cuboid") st.geometry = G_RECT;
+#else
+                        if (gname == "rectangle") st.geometry = G_RECT;
+#endif
+                        else if (gname == "circle") st.geometry = G_CIRC;
+                        else if (gname == "point") st.geometry = G_POINT;
+                    }
+                }
+            }
+        }
+    }
+    // State extents pulled a hundredth of a cell inwards (read_input.f90:284-297).
+    const double dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
+    const double dy = (d.ymax - d.ymin) / (double)(float)d.y_cells;
+    const double dz = (d.zmax - d.zmin) / (double)(float)d.z_cells;
+    for (size_t n = 1; n < d.states.size(); ++n) {
+        State& s = d.states[n];
+        s.xmin = s.xmin + (dx / 100.0);
+        s.ymin = s.ymin + (dy / 100.0);
+        s.xmax = s.xmax - (dx / 100.0);
+        s.ymax = s.ymax - (dy / 100.0);
+        if (TL_DIM == 3) {
+            s.zmin = s.zmin + (dz / 100.0);
+            s.zmax = s.zmax - (dz / 100.0);
+        }
+    }
+    return d;
+}
+
+// ---- eigenvalue estimate and Chebyshev coefficients -----------------------
+
+// tqli (tea_leaf_cheby.f90:233-283): implicit-shift QL on the symmetric
+// tridiagonal (d, e), 1-based, e(2..n) the sub-diagonal before the shift.
+int tqli(std::vector<double>& d, std::vector<double>& e, int n)
+{
+    for (int i = 1; i < n; ++i) e[i] = e[i + 1];
+    if (n >= 1) e[n] = 0.0;
+    for (int l = 1; l <= n; ++l) {
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m <= n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) + dd == dd) break;
+            }
+            if (m == l) break;
+            if (iter == 30) return 1;
+            ++iter;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::sqrt(g * g + 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(std::fabs(r), g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool restart = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = s * e[i];
+                const double b = c * e[i];
+                r = std::sqrt(f * f + g * g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = d[i + 1] - p;
+                    e[m] = 0.0;
+                    restart = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+            }
+            if (restart) continue;
+            d[l] = d[l] - p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return 0;
+}
+
+// tea_calc_eigenvalues (tea_leaf_cheby.f90:285-332), CG coefficients 1..m.
+int calc_eigenvalues(const std::vector<double>& a, const std::vector<double>& b, int m, double& emin, double& emax)
+{
+    std::vector<double> diag(m + 2, 0.0), off(m + 2, 0.0);
+    for (int n = 1; n <= m; ++n) {
+        diag[n] = 1.0 / a[n];
+        if (n > 1) diag[n] = diag[n] + b[n - 1] / a[n - 1];
+        if (n < m) off[n + 1] = std::sqrt(b[n]) / a[n];
+    }
+    if (tqli(diag, off, m)) return 1;
+    std::vector<double> v(diag.begin() + 1, diag.begin() + 1 + m);
+    std::sort(v.begin(), v.end());
+    emin = v.front();
+    emax = v.back();
+    return (emin < 0.0 || emax < 0.0) ? 1 : 0;
+}
+
+// tea_calc_ch_coefs (tea_leaf_cheby.f90:334-360), coefficients 1..n.
+void calc_ch_coefs(std::vector<double>& ca, std::vector<double>& cb, double emin, double emax, double& theta, int n)
+{
+    theta = (emax + emin) / 2;
+    const double delta = (emax - emin) / 2;
+    const double sigma = theta / delta;
+    double rho_old = 1.0 / sigma;
+    for (int k = 1; k <= n; ++k) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+        ca[k] = rho_new * rho_old;
+        cb[k] = 2.0 * rho_new / delta;
+        rho_old = rho_new;
+    }
+}
+
+// ---- solve ----------------------------------------------------------------
+
+struct StepInfo {
+    int iters = 0, switch_step = 0, est_itc = 0, cg_steps = 0, cheby_steps = 0;
+    double error = 0, exact_error = 0, eigmin = 0, eigmax = 0, solve_s = 0;
+};
+
+const int kAllExternal[6] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
+
+void halo(Chunk& c, std::initializer_list<int> which, int depth)
+{
+    int fields[NUM_FIELDS] = {0, 0, 0, 0, 0, 0};
+    for (int f : which) fields[f - 1] = 1;
+    op_update_halo(c, kAllExternal, fields, depth);
+}
+
+// tea_leaf (tea_solve.f90:36-605) for one chunk.
+StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
+{
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    StepInfo si;
+    const int max_iters = d.max_iters;
+    double error = 1e10, exact_error = 0.0;
+    int cheby_calc_steps = 0, cg_calc_steps = 0;
+    double rro = 0.0, pw = 0.0, rrn = 0.0;
+    double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
+    int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
+    std::vector<double> cg_alphas(max_iters + 3, 0.0), cg_betas(max_iters + 3, 0.0);
+    std::vector<double> ch_alphas(max_iters + 3, 0.0), ch_betas(max_iters + 3, 0.0);
+
+    halo(c, {FIELD_ENERGY1, FIELD_DENSITY}, 2);
+    if (d.use_cg || d.use_cheby || d.use_ppcg) {
+        rro = op_cg_init(c, d.coefficient, d.preconditioner, dt);
+    } else if (d.use_jacobi) {
+        op_jacobi_init(c, d.coefficient, dt);
+    }
+    op_copy_u(c);
+
+    int n;
+    for (n = 1; n <= max_iters; ++n) {
+        bool ch_switch_check;
+        if (d.errswitch)
+            ch_switch_check = (cheby_calc_steps > 0) || ((error <= d.epslim) && (n >= 20));
+        else
+            ch_switch_check = (n >= d.presteps) && (error <= 1.0);
+
+        if ((d.use_cheby || d.use_ppcg) && ch_switch_check) {
+            if (cheby_calc_steps == 0) {
+                max_cheby_iters = max_iters - n + 2;
+                rro = error;
+                if (calc_eigenvalues(cg_alphas, cg_betas, n - 1, eigmin, eigmax))
+                    throw Error("tea_leaf: Error in calculating eigenvalues");
+                eigmin = eigmin * (double)0.95f;
+                eigmax = eigmax * (double)1.05f;
+                calc_ch_coefs(ch_alphas, ch_betas, eigmin, eigmax, theta,
+                              d.use_cheby ? max_cheby_iters : d.ppcg_inner_steps);
+                cn = eigmax / eigmin;
+            }
+            if (d.use_cheby) {
+                if (cheby_calc_steps == 0) {
+                    // tea_leaf_cheby_first_step (tea_solve.f90:658-750)
+                    const double bb = op_calc_2norm(c, false);
+                    op_cheby_init(c, &ch_alphas[1], &ch_betas[1], max_cheby_iters, theta, d.preconditioner, true);
+                    op_cheby_iterate(c, max_cheby_iters, true);
+                    error = op_calc_2norm(c, true);
+                    const double it_alpha = DBL_EPSILON * bb / (4.0 * error);
+                    const double gamm = (std::sqrt(cn) - 1.0) / (std::sqrt(cn) + 1.0);
+                    est_itc = (int)std::lround(std::log(it_alpha) / (2.0 * std::log(gamm)));
+                    cheby_calc_steps = 1;
+                    switch_step = n;
+                } else {
+                    op_cheby_iterate(c, cheby_calc_steps, true);
+                    if ((n >= est_itc) && (n % 10 == 0)) error = op_calc_2norm(c, true);
+                }
+            } else {
+                if (cheby_calc_steps == 0) {
+                    cheby_calc_steps = 1;
+                    op_ppcg_init(c, &ch_alphas[1], &ch_betas[1], max_cheby_iters, theta);
+                    op_calc_residual(c);
+                    switch_step = n;
+                }
+                pw = op_cg_calc_w(c);
+                const double alpha = rro / pw;
+                (void)op_cg_calc_ur(c, alpha);
+                // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
+                op_ppcg_init_sd(c);
+                for (int step = 1; step <= d.ppcg_inner_steps; ++step) op_ppcg_inner(c, step, true);
+                rrn = op_calc_2norm(c, true);
+                const double beta = rrn / rro;
+                op_cg_calc_p(c, beta);
+                error = rrn;
+                rro = rrn;
+            }
+            ++cheby_calc_steps;
+        } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
+            ++cg_calc_steps;
+            pw = op_cg_calc_w(c);
+            const double alpha = rro / pw;
+            if (d.use_cheby || d.use_ppcg) cg_alphas[n] = alpha;
+            rrn = op_cg_calc_ur(c, alpha);
+            const double beta = rrn / rro;
+            if (d.use_cheby || d.use_ppcg) cg_betas[n] = beta;
+            op_cg_calc_p(c, beta);
+            error = rrn;
+            rro = rrn;
+        } else if (d.use_jacobi) {
+            error = op_jacobi_solve(c, true);
+            if (n % 50 == 0) {
+                op_calc_residual(c);
+                error = op_calc_2norm(c, true);
+            }
+        }
+        if (std::fabs(error) < d.eps) break;
+    }
+    if (d.check_result) {
+        op_calc_residual(c);
+        exact_error = op_calc_2norm(c, true);
+    }
+    si.iters = n - 1;
+    si.error = error;
+    si.exact_error = exact_error;
+    si.switch_step = switch_step;
+    si.est_itc = est_itc;
+    si.eigmin = eigmin;
+    si.eigmax = eigmax;
+    si.cg_steps = cg_calc_steps;
+    si.cheby_steps = cheby_calc_steps;
+    op_finalise(c);
+    halo(c, {FIELD_ENERGY1}, 1);
+    c.sync();
+    si.solve_s = std::chrono::duration<double>(clock::now() - t0).count();
+    return si;
+}
+
+struct Json {
+    std::string s;
+    void f(const char* fmt, ...)
+    {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        s += buf;
+    }
+};
+
+void summary(Json& o, Chunk& c, int step, double time)
+{
+    double vol, mass, ie, temp;
+    op_field_summary(c, &vol, &mass, &ie, &temp);
+    o.f("\"step\": %d, \"time\": %.17g, \"vol\": %.17g, \"mass\": %.17g, \"ie\": %.17g, \"temp\": %.17g", step, time,
+        vol, mass, ie, temp);
+}
+
+// initialise -> start -> diffuse (start.f90:24-162, diffuse.f90:23-160).
+std::string run_deck(const std::string& text, int max_steps)
+{
+    const Deck d = parse_deck(text);
+    const int nz = TL_DIM == 3 ? d.z_cells : 1;
+    auto c = std::make_unique<Chunk>(d.x_cells, d.y_cells, nz, 0);
+    const double dx = (d.xmax - d.xmin) / (double)(float)d.x_cells;
+    const double dy = (d.ymax - d.ymin) / (double)(float)d.y_cells;
+    const double dz = (d.zmax - d.zmin) / (double)(float)d.z_cells;
+    op_initialise_chunk(*c, d.xmin, d.ymin, d.zmin, dx, dy, dz);
+    const int ns = (int)d.states.size();
+    std::vector<double> sd(ns), se(ns), x0(ns), x1(ns), y0(ns), y1(ns), z0(ns), z1(ns), sr(ns);
+    std::vector<int> sg(ns);
+    for (int k = 0; k < ns; ++k) {
+        const State& s = d.states[k];
+        sd[k] = s.density; se[k] = s.energy;
+        x0[k] = s.xmin; x1[k] = s.xmax; y0[k] = s.ymin; y1[k] = s.ymax; z0[k] = s.zmin; z1[k] = s.zmax;
+        sr[k] = s.radius; sg[k] = s.geometry;
+    }
+    op_generate_chunk(*c, ns, sd.data(), se.data(), x0.data(), x1.data(), y0.data(), y1.data(), z0.data(), z1.data(),
+                      sr.data(), sg.data(), G_RECT, G_CIRC, G_POINT);
+    halo(*c, {FIELD_DENSITY, FIELD_ENERGY0, FIELD_ENERGY1}, 2);
+
+    Json o;
+    o.f("{\"dim\": %d, \"x_cells\": %d, \"y_cells\": %d, \"z_cells\": %d, \"device\": \"hip\", ", TL_DIM, d.x_cells,
+        d.y_cells, nz);
+    o.f("\"solver\": \"%s\", \"eps\": %.17g, \"dtinit\": %.17g, \"max_iters\": %d, ",
+        d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? "ppcg" : "jacobi", d.eps, d.dtinit, d.max_iters);
+    o.f("\"initial\": {");
+    summary(o, *c, 0, 0.0);
+    o.f("}, \"steps\": [");
+
+    op_set_field(*c);
+    double time = 0.0, dt = d.dtinit, total = 0.0;
+    const double g_small = (double)1.0e-16f;
+    int step = 0;
+    for (;;) {
+        ++step;
+        c->timer.on = kernel_timing_from() > 0 && step >= kernel_timing_from();
+        if (d.dtinit <= dt) dt = d.dtinit;  // timestep.f90 / calc_dt.f90:35
+        const StepInfo si = tea_leaf(*c, d, dt);
+        total += si.solve_s;
+        time = time + dt;
+        o.f("%s{", step > 1 ? ", " : "");
+        o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
+            "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
+            "\"dt\": %.17g, ",
+            si.iters, si.error, si.exact_error, si.switch_step, si.est_itc, si.eigmin, si.eigmax, si.cg_steps,
+            si.cheby_steps, si.solve_s, dt);
+        if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
+            summary(o, *c, step, time);
+        else
+            o.f("\"step\": %d, \"time\": %.17g", step, time);
+        o.f("}");
+        if (time + g_small > d.end_time || step >= d.end_step || (max_steps > 0 && step >= max_steps)) break;
+    }
+    o.f("], \"final\": {");
+    summary(o, *c, step, time);
+    o.f("}, \"test_problem\": %d, \"total_solve_s\": %.9g, \"cells\": %ld", d.test_problem, total,
+        (long)d.x_cells * d.y_cells * nz);
+    if (kernel_timing_from() > 0) {
+        c->sync();
+        o.f(", \"kernels\": {");
+        bool first = true;
+        for (int k = 0; k < K_COUNT; ++k) {
+            if (!c->timer.count[k]) continue;
+            o.f("%s\"%s\": {\"count\": %ld, \"ms\": %.6f, \"bytes_per_cell\": %.1f}", first ? "" : ", ",
+                kernel_name(k), c->timer.count[k], c->timer.ms[k], kernel_bytes_per_cell(k));
+            first = false;
+        }
+        o.f("}");
+    }
+    o.f("}");
+    return o.s;
+}
+
+std::string g_driver_error;
+
+}  // namespace
+}  // namespace tl
+
+extern "C" {
+
+// Runs a tea.in deck end to end on the GPU and writes a JSON report (per-step
+// iteration counts, solver diagnostics, field summaries, solve seconds) into
+// json[0..cap).  max_steps > 0 stops after that many timesteps.  Returns 0,
+// -1 on an error (message in tl_driver_error()), -2 if cap is too small.
+int tl_run_deck(const char* deck, char* json, int cap, int max_steps)
+{
+    try {
+        tl::g_driver_error.clear();
+        const std::string out = tl::run_deck(deck ? deck : "", max_steps);
+        if ((int)out.size() + 1 > cap) {
+            tl::g_driver_error = "JSON buffer too small: need " + std::to_string(out.size() + 1);
+            return -2;
+        }
+        std::memcpy(json, out.c_str(), out.size() + 1);
+        return 0;
+    } catch (const std::exception& e) {
+        tl::g_driver_error = e.what();
+        return -1;
+    }
+}
+
+const char* tl_driver_error(void) { return tl::g_driver_error.c_str(); }
+
+// Per-launch kernel timing with HIP events on the chunk's stream, for the
+// timesteps from `from_step` on (0 turns it off); tl_run_deck then adds a
+// "kernels" table (launch count, summed milliseconds, algorithmic bytes per
+// cell) to its report.
+void tl_set_kernel_timing(int from_step) { tl::set_kernel_timing_from(from_step); }
+
+}  // extern "C"

@@ -1,0 +1,265 @@
+// The ext_* C-ABI of TeaLeaf's kernel layer, re-exported on top of the
+// gfx950 chunk operations.  Names, argument order and meaning follow
+// tealeaf_2d/ext_core.cpp and ext_chunk.cpp (tealeaf_3d/ for TL_DIM == 3):
+// every argument is a pointer (Fortran implicit interfaces), a chunk is a
+// 1-based index into a process-global list, double* outputs are overwritten
+// with the chunk's reduction, and a LOGICAL passed as bool* is read through
+// its low byte.  Errors print a message and exit(1) as the reference's
+// abort() does (ext_core.cpp:571-583), unless a caller has switched that off
+// with tl_set_abort_on_error(0), in which case the call returns and the
+// message is kept for tl_last_error().
+#include "tl_chunk.hpp"
+
+#include <sys/time.h>
+
+#include <cstdarg>
+#include <cstdlib>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace tl;
+
+namespace {
+
+std::vector<std::unique_ptr<Chunk>> g_chunks;
+std::string g_error;
+bool g_abort_on_error = true;
+
+void fail(const std::string& msg)
+{
+    g_error = msg;
+    if (g_abort_on_error) {
+        std::fprintf(stderr, "\nError in the TeaLeaf HIP library: %s\n", msg.c_str());
+        std::exit(1);
+    }
+}
+
+Chunk* chunk_at(const int* id)
+{
+    if (!id || *id < 1 || *id > (int)g_chunks.size() || !g_chunks[*id - 1])
+        throw Error("invalid chunk " + std::to_string(id ? *id : 0));
+    return g_chunks[*id - 1].get();
+}
+
+template <class F>
+void guarded(F&& f)
+{
+    try {
+        f();
+    } catch (const std::exception& e) {
+        fail(e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- test and driver hooks (no counterpart in the reference) --------------
+
+const char* tl_last_error(void) { return g_error.c_str(); }
+void tl_clear_error(void) { g_error.clear(); }
+void tl_set_abort_on_error(int on) { g_abort_on_error = on != 0; }
+int tl_dim(void) { return TL_DIM; }
+int tl_num_fields(void) { return F_COUNT; }
+
+int tl_chunk_dims(const int* chunk, int* dims)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = chunk_at(chunk);
+        dims[0] = c->g.X; dims[1] = c->g.Y; dims[2] = c->g.Z;
+        dims[3] = c->g.nx; dims[4] = c->g.ny; dims[5] = TL_DIM == 3 ? c->g.nz : 1;
+        rc = 0;
+    });
+    return rc;
+}
+
+int tl_get_field(const int* chunk, int field, double* out)
+{
+    int rc = -1;
+    guarded([&] { chunk_at(chunk)->get_field(field, out); rc = 0; });
+    return rc;
+}
+
+int tl_set_field(const int* chunk, int field, const double* in)
+{
+    int rc = -1;
+    guarded([&] { chunk_at(chunk)->set_field(field, in); rc = 0; });
+    return rc;
+}
+
+// ---- ext_chunk.cpp:18-33 --------------------------------------------------
+
+#if TL_DIM == 3
+void ext_init_(int* xMax, int* yMax, int* zMax, int* rank)
+{
+    guarded([&] { g_chunks.emplace_back(new Chunk(*xMax, *yMax, *zMax, *rank)); });
+}
+#else
+void ext_init_(int* xMax, int* yMax, int* rank)
+{
+    guarded([&] { g_chunks.emplace_back(new Chunk(*xMax, *yMax, 1, *rank)); });
+}
+#endif
+
+// The reference deletes its chunks but keeps the list; here the list is
+// emptied too, so an ext_init_ after ext_finalise_ starts again at chunk 1.
+void ext_finalise_(void) { g_chunks.clear(); }
+
+// ---- ext_core.cpp ----------------------------------------------------------
+
+#if TL_DIM == 3
+void ext_initialise_chunk_(const int* chunk, const double* xMin, const double* yMin, const double* zMin,
+                           const double* dx, const double* dy, const double* dz)
+{
+    guarded([&] { op_initialise_chunk(*chunk_at(chunk), *xMin, *yMin, *zMin, *dx, *dy, *dz); });
+}
+
+void ext_generate_chunk_(const int* chunk, const int* numStates, const double* density, const double* energy,
+                         const double* xMin, const double* xMax, const double* yMin, const double* yMax,
+                         const double* zMin, const double* zMax, const double* radius, const int* geometry,
+                         const int* rectParam, const int* circParam, const int* pointParam)
+{
+    guarded([&] {
+        op_generate_chunk(*chunk_at(chunk), *numStates, density, energy, xMin, xMax, yMin, yMax, zMin, zMax,
+                          radius, geometry, *rectParam, *circParam, *pointParam);
+    });
+}
+#else
+void ext_initialise_chunk_(const int* chunk, const double* xMin, const double* yMin, const double* dx,
+                           const double* dy)
+{
+    guarded([&] { op_initialise_chunk(*chunk_at(chunk), *xMin, *yMin, 0.0, *dx, *dy, 0.0); });
+}
+
+void ext_generate_chunk_(const int* chunk, const int* numStates, const double* density, const double* energy,
+                         const double* xMin, const double* xMax, const double* yMin, const double* yMax,
+                         const double* radius, const int* geometry, const int* rectParam, const int* circParam,
+                         const int* pointParam)
+{
+    guarded([&] {
+        op_generate_chunk(*chunk_at(chunk), *numStates, density, energy, xMin, xMax, yMin, yMax, nullptr, nullptr,
+                          radius, geometry, *rectParam, *circParam, *pointParam);
+    });
+}
+#endif
+
+void ext_field_summary_kernel_(const int* chunk, double* vol, double* mass, double* ie, double* temp)
+{
+    guarded([&] { op_field_summary(*chunk_at(chunk), vol, mass, ie, temp); });
+}
+
+// Multi-chunk face packing (ext_core.cpp:151-245) serves neighbour chunks
+// over MPI; a single chunk on one GPU never has one.
+#if TL_DIM == 3
+void ext_pack_message_(const int*, const int*, const int*, const int*, const int*, const int*, double*, bool*)
+#else
+void ext_pack_message_(const int*, const int*, const int*, const int*, const int*, double*, bool*)
+#endif
+{
+    fail("ext_pack_message_: multi-chunk halo exchange is not supported by this library");
+}
+
+void ext_set_field_kernel_(const int* chunk)
+{
+    guarded([&] { op_set_field(*chunk_at(chunk)); });
+}
+
+void ext_solver_copy_u_(const int* chunk)
+{
+    guarded([&] { op_copy_u(*chunk_at(chunk)); });
+}
+
+void ext_calculate_residual_(const int* chunk)
+{
+    guarded([&] { op_calc_residual(*chunk_at(chunk)); });
+}
+
+void ext_calculate_2norm_(const int* chunk, double* norm, bool* normArray)
+{
+    guarded([&] {
+        const bool use_r = *reinterpret_cast<const unsigned char*>(normArray) != 0;
+        *norm = op_calc_2norm(*chunk_at(chunk), use_r);
+    });
+}
+
+void ext_solver_finalise_(const int* chunk)
+{
+    guarded([&] { op_finalise(*chunk_at(chunk)); });
+}
+
+void ext_update_halo_kernel_(const int* chunk, const int* chunkNeighbours, const int* fields, const int* depth)
+{
+    guarded([&] { op_update_halo(*chunk_at(chunk), chunkNeighbours, fields, *depth); });
+}
+
+void ext_jacobi_kernel_init_(const int* chunk, const int* coefficient, const double* dt)
+{
+    guarded([&] { op_jacobi_init(*chunk_at(chunk), *coefficient, *dt); });
+}
+
+void ext_jacobi_kernel_solve_(const int* chunk, double* error)
+{
+    guarded([&] { *error = op_jacobi_solve(*chunk_at(chunk), false); });
+}
+
+void ext_cg_solver_init_(const int* chunk, const int* coefficient, const int* preconditioner, double* dt,
+                         double* rro)
+{
+    guarded([&] { *rro = op_cg_init(*chunk_at(chunk), *coefficient, *preconditioner != 0, *dt); });
+}
+
+void ext_cg_calc_w_(const int* chunk, double* pw)
+{
+    guarded([&] { *pw = op_cg_calc_w(*chunk_at(chunk)); });
+}
+
+void ext_cg_calc_ur_(const int* chunk, const double* alpha, double* rrn)
+{
+    guarded([&] { *rrn = op_cg_calc_ur(*chunk_at(chunk), *alpha); });
+}
+
+void ext_cg_calc_p_(const int* chunk, const double* beta)
+{
+    guarded([&] { op_cg_calc_p(*chunk_at(chunk), *beta); });
+}
+
+void ext_cheby_solver_init_(const int* chunk, double* alphas, double* betas, const double* theta,
+                            const int* preconditioner, const int* maxChebyIters)
+{
+    guarded([&] {
+        op_cheby_init(*chunk_at(chunk), alphas, betas, *maxChebyIters, *theta, *preconditioner != 0, false);
+    });
+}
+
+void ext_cheby_solver_iterate_(const int* chunk, const int* chebyCalcStep)
+{
+    guarded([&] { op_cheby_iterate(*chunk_at(chunk), *chebyCalcStep, false); });
+}
+
+void ext_ppcg_init_(const int* chunk, double* alphas, double* betas, const double* theta, const int* maxChebyIters)
+{
+    guarded([&] { op_ppcg_init(*chunk_at(chunk), alphas, betas, *maxChebyIters, *theta); });
+}
+
+void ext_ppcg_init_sd_(const int* chunk)
+{
+    guarded([&] { op_ppcg_init_sd(*chunk_at(chunk)); });
+}
+
+void ext_ppcg_inner_(const int* chunk, int* currentStep)
+{
+    guarded([&] { op_ppcg_inner(*chunk_at(chunk), *currentStep, false); });
+}
+
+// timer_c.cpp:30 (wall-clock seconds for the Fortran timer)
+void timer_c_(double* elapsed)
+{
+    struct timeval t;
+    gettimeofday(&t, nullptr);
+    *elapsed = (double)t.tv_sec + 1.0e-6 * (double)t.tv_usec;
+}
+
+}  // extern "C"

@@ -1,0 +1,1124 @@
+// gfx950 kernels for TeaLeaf's linear-solve path and the chunk operations
+// that launch them.  Every per-cell expression keeps the operand order of
+// the reference functor it replaces (cited at each kernel) and the library
+// is built with -ffp-contract=off, so cell values are bitwise those of the
+// reference's IEEE evaluation; only the order of the FP64 sums differs.
+//
+// Boundary handling: a stencil input's neighbour across an external face is
+// read as the mirrored interior value.  That is exactly what the reference
+// reads after its depth-1 reflective halo update (ext_update_halo.hpp), so
+// the solver loop needs no halo launches and the arithmetic is unchanged.
+#include "tl_chunk.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tl {
+
+namespace {
+
+constexpr int FLAT_BLOCK = 256;
+
+struct KPtr {
+    const double* __restrict__ kx;
+    const double* __restrict__ ky;
+    const double* __restrict__ kz;
+};
+
+// SMVP: tealeaf_2d/ext_shared.hpp:37-41, tealeaf_3d/ext_shared.hpp:39-45.
+__device__ __forceinline__ double smvp(const Geom& g, const double* __restrict__ a, const KPtr& k,
+                                       long i, int kk, int jj, int ii)
+{
+    const double ac = a[i];
+    const double aw = a[kk > HALO ? i - 1 : i];
+    const double ae = a[kk < g.nx + 1 ? i + 1 : i];
+    const double as = a[jj > HALO ? i - g.sx : i];
+    const double an = a[jj < g.ny + 1 ? i + g.sx : i];
+    const double kxc = k.kx[i], kxe = k.kx[i + 1];
+    const double kyc = k.ky[i], kyn = k.ky[i + g.sx];
+#if TL_DIM == 3
+    const double ab = a[ii > HALO ? i - g.sy : i];
+    const double af = a[ii < g.nz + 1 ? i + g.sy : i];
+    const double kzc = k.kz[i], kzf = k.kz[i + g.sy];
+    return (1.0 + (kxe + kxc) + (kyn + kyc) + (kzf + kzc)) * ac
+         - (kxe * ae + kxc * aw)
+         - (kyn * an + kyc * as)
+         - (kzf * af + kzc * ab);
+#else
+    (void)ii;
+    return (1.0 + (kxe + kxc) + (kyn + kyc)) * ac
+         - (kxe * ae + kxc * aw)
+         - (kyn * an + kyc * as);
+#endif
+}
+
+// Diagonal of A (the Jacobi preconditioner's reciprocal), ext_cg_kernel.hpp:119-121.
+__device__ __forceinline__ double diag(const Geom& g, const KPtr& k, long i)
+{
+#if TL_DIM == 3
+    return (1.0 + (k.kx[i + 1] + k.kx[i]) + (k.ky[i + g.sx] + k.ky[i]) + (k.kz[i + g.sy] + k.kz[i]));
+#else
+    return (1.0 + (k.kx[i + 1] + k.kx[i]) + (k.ky[i + g.sx] + k.ky[i]));
+#endif
+}
+
+// Flat walk over the padded reference domain (setup kernels only).
+struct Flat {
+    long i;
+    int kk, jj, ii;
+};
+
+__device__ __forceinline__ bool flat_cell(const Geom& g, long q, Flat& c)
+{
+    const long n = (long)g.X * g.Y * g.Z;
+    if (q >= n) return false;
+    c.kk = (int)(q % g.X);
+    const long r = q / g.X;
+    c.jj = (int)(r % g.Y);
+    c.ii = (int)(r / g.Y);
+    c.i = gidx(g, c.kk, c.jj, c.ii);
+    return true;
+}
+
+__device__ __forceinline__ bool in_one(const Geom& g, const Flat& c)
+{
+#if TL_DIM == 3
+    if (!(c.ii > 0 && c.ii < g.Z - 1)) return false;
+#endif
+    return c.jj > 0 && c.jj < g.Y - 1 && c.kk > 0 && c.kk < g.X - 1;
+}
+
+__device__ __forceinline__ bool in_skew(const Geom& g, const Flat& c)
+{
+#if TL_DIM == 3
+    if (!(c.ii >= HALO && c.ii < g.Z - 1)) return false;
+#endif
+    return c.jj >= HALO && c.jj < g.Y - 1 && c.kk >= HALO && c.kk < g.X - 1;
+}
+
+// ----------------------------------------------------------------------------
+// Setup kernels
+
+// ext_initialise_chunk.hpp:80-88 (volume = dx*dy[*dz] over the full domain)
+__global__ void k_fill(Geom g, double* __restrict__ a, double v)
+{
+    Flat c;
+    if (flat_cell(g, (long)blockIdx.x * FLAT_BLOCK + threadIdx.x, c)) a[c.i] = v;
+}
+
+struct StateArgs {
+    int n, rect, circ, point;
+    const double *density, *energy, *xmin, *xmax, *ymin, *ymax, *zmin, *zmax, *radius;
+    const int* geometry;
+};
+
+// ext_generate_chunk.hpp:38-86
+__global__ void k_generate(Geom g, StateArgs s, const double* __restrict__ vX, const double* __restrict__ vY,
+                           const double* __restrict__ vZ, const double* __restrict__ cX,
+                           const double* __restrict__ cY, const double* __restrict__ cZ,
+                           double* __restrict__ energy0, double* __restrict__ density, double* __restrict__ u)
+{
+    Flat c;
+    if (!flat_cell(g, (long)blockIdx.x * FLAT_BLOCK + threadIdx.x, c)) return;
+    const int kk = c.kk, jj = c.jj;
+    double e0 = s.energy[0], d0 = s.density[0];
+    for (int ss = 1; ss < s.n; ++ss) {
+        bool apply = false;
+        if (s.geometry[ss] == s.rect) {
+            apply = vX[kk + 1] >= s.xmin[ss] && vX[kk] < s.xmax[ss] && vY[jj + 1] >= s.ymin[ss] && vY[jj] < s.ymax[ss];
+#if TL_DIM == 3
+            apply = apply && vZ[c.ii + 1] >= s.zmin[ss] && vZ[c.ii] < s.zmax[ss];
+#endif
+        } else if (s.geometry[ss] == s.circ) {
+#if TL_DIM == 3
+            const double radius = sqrt((cX[kk] - s.xmin[ss]) * (cX[kk] - s.xmin[ss]) +
+                                       (cY[jj] - s.ymin[ss]) * (cY[jj] - s.ymin[ss]) +
+                                       (cZ[c.ii] - s.zmin[ss]) * (cZ[c.ii] - s.zmin[ss]));
+#else
+            const double radius = sqrt((cX[kk] - s.xmin[ss]) * (cX[kk] - s.xmin[ss]) +
+                                       (cY[jj] - s.ymin[ss]) * (cY[jj] - s.ymin[ss]));
+#endif
+            apply = radius <= s.radius[ss];
+        } else if (s.geometry[ss] == s.point) {
+            apply = vX[kk] == s.xmin[ss] && vY[jj] == s.ymin[ss];
+#if TL_DIM == 3
+            apply = apply && vZ[c.ii] == s.zmin[ss];
+#endif
+        }
+        if (apply) {
+            e0 = s.energy[ss];
+            d0 = s.density[ss];
+        }
+    }
+    energy0[c.i] = e0;
+    density[c.i] = d0;
+    if (in_one(g, c)) u[c.i] = e0 * d0;
+    (void)vZ; (void)cZ;
+}
+
+// ext_field_summary.hpp:20-31: vol, mass, ie, temp over the interior.
+__global__ __launch_bounds__(BLOCK) void k_field_summary(Geom g, const double* __restrict__ volume,
+                                                         const double* __restrict__ density,
+                                                         const double* __restrict__ energy0,
+                                                         const double* __restrict__ u, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const double cellVol = volume[s.i];
+        const double cellMass = cellVol * density[s.i];
+        v[0] += cellVol;
+        v[1] += cellMass;
+        v[2] += cellMass * energy0[s.i];
+        v[3] += cellMass * u[s.i];
+    }
+    grid_reduce<4>(v, red, g.nblocks);
+}
+
+// ----------------------------------------------------------------------------
+// Reflective halo (ext_core.cpp:305-336, ext_update_halo.hpp:11-76 in 2D;
+// tealeaf_3d/ext_update_halo.hpp:11-136 in 3D) as one in-place gather.
+//
+// 2D: the reference updates left/right over every row (halo rows included)
+// and then top/bottom over every column, so a cell in the top/bottom band of
+// depth <= d takes the value of (mirrored column if its column is in the
+// left/right band of depth <= d, else its own column; mirrored row); a cell
+// only in the left/right band takes (mirrored column, own row).  3D updates
+// face interiors only.  No source cell is written by the same launch, so all
+// threads read pre-launch values.
+struct HaloFields {
+    double* f[NUM_FIELDS];
+    int count;
+};
+
+__device__ __forceinline__ int hdepth(int k, int n)  // 0 inside, 1 or 2 in the halo
+{
+    return k < HALO ? HALO - k : (k >= n + HALO ? k - (n + HALO - 1) : 0);
+}
+
+__device__ __forceinline__ int mirror(int k, int n)
+{
+    return k < HALO ? 2 * HALO - 1 - k : 2 * (n + HALO) - 1 - k;
+}
+
+__global__ void k_update_halo(Geom g, HaloFields hf, int depth, int mask)
+{
+    const long t = (long)blockIdx.x * FLAT_BLOCK + threadIdx.x;
+    int kk, jj;
+#if TL_DIM == 2
+    // Ring of width 2: rows 0,1,Y-2,Y-1 over the full width, then columns
+    // 0,1,X-2,X-1 over the interior rows.
+    const long nrow = 4L * g.X, ncol = 4L * g.ny;
+    if (t < nrow) {
+        const int r = (int)(t / g.X);
+        kk = (int)(t % g.X);
+        jj = r < 2 ? r : g.Y - 4 + r;
+    } else if (t < nrow + ncol) {
+        const long q = t - nrow;
+        const int cidx = (int)(q / g.ny);
+        jj = HALO + (int)(q % g.ny);
+        kk = cidx < 2 ? cidx : g.X - 4 + cidx;
+    } else {
+        return;
+    }
+    const int kd = hdepth(kk, g.nx), jd = hdepth(jj, g.ny);
+    const bool kin = kd >= 1 && kd <= depth && (mask & (kk < HALO ? 1 : 2));
+    const bool jin = jd >= 1 && jd <= depth && (mask & (jj < HALO ? 4 : 8));
+    int sk, sj;
+    if (jin) {
+        sk = kin ? mirror(kk, g.nx) : kk;
+        sj = mirror(jj, g.ny);
+    } else if (kin) {
+        sk = mirror(kk, g.nx);
+        sj = jj;
+    } else {
+        return;
+    }
+    const long dst = gidx(g, kk, jj, 0), src = gidx(g, sk, sj, 0);
+#else
+    // Six face slabs of two layers over the face interiors.
+    int ii;
+    const long nlr = 4L * g.ny * g.nz, ntb = 4L * g.nx * g.nz, nfb = 4L * g.nx * g.ny;
+    int sk, sj, si, axis, layer;
+    if (t < nlr) {
+        axis = 0;
+        layer = (int)(t / ((long)g.ny * g.nz));
+        const long q = t % ((long)g.ny * g.nz);
+        jj = HALO + (int)(q % g.ny);
+        ii = HALO + (int)(q / g.ny);
+        kk = layer < 2 ? layer : g.X - 4 + layer;
+    } else if (t < nlr + ntb) {
+        axis = 1;
+        const long q0 = t - nlr;
+        layer = (int)(q0 / ((long)g.nx * g.nz));
+        const long q = q0 % ((long)g.nx * g.nz);
+        kk = HALO + (int)(q % g.nx);
+        ii = HALO + (int)(q / g.nx);
+        jj = layer < 2 ? layer : g.Y - 4 + layer;
+    } else if (t < nlr + ntb + nfb) {
+        axis = 2;
+        const long q0 = t - nlr - ntb;
+        layer = (int)(q0 / ((long)g.nx * g.ny));
+        const long q = q0 % ((long)g.nx * g.ny);
+        kk = HALO + (int)(q % g.nx);
+        jj = HALO + (int)(q / g.nx);
+        ii = layer < 2 ? layer : g.Z - 4 + layer;
+    } else {
+        return;
+    }
+    sk = kk; sj = jj; si = ii;
+    if (axis == 0) {
+        if (hdepth(kk, g.nx) > depth || !(mask & (kk < HALO ? 1 : 2))) return;
+        sk = mirror(kk, g.nx);
+    } else if (axis == 1) {
+        if (hdepth(jj, g.ny) > depth || !(mask & (jj < HALO ? 4 : 8))) return;
+        sj = mirror(jj, g.ny);
+    } else {
+        if (hdepth(ii, g.nz) > depth || !(mask & (ii < HALO ? 16 : 32))) return;
+        si = mirror(ii, g.nz);
+    }
+    const long dst = gidx(g, kk, jj, ii), src = gidx(g, sk, sj, si);
+#endif
+    for (int q = 0; q < hf.count; ++q) hf.f[q][dst] = hf.f[q][src];
+}
+
+// ----------------------------------------------------------------------------
+// CG (ext_cg_kernel.hpp)
+
+// CGInitU (:22-27) over the full domain, CGInitW (:47-55) on the one-halo
+// domain and CGInitK (:74-83) on the skew domain, in one pass.  kx/ky/kz
+// read w = rho or 1/rho of the left/lower neighbour, which the skew domain
+// keeps inside the one-halo domain, so w is recomputed from density.
+__global__ void k_cg_init_k(Geom g, int coef, double rx, double ry, double rz, const double* __restrict__ density,
+                            const double* __restrict__ energy, double* __restrict__ u, double* __restrict__ p,
+                            double* __restrict__ r, double* __restrict__ w, double* __restrict__ kx,
+                            double* __restrict__ ky, double* __restrict__ kz)
+{
+    Flat c;
+    if (!flat_cell(g, (long)blockIdx.x * FLAT_BLOCK + threadIdx.x, c)) return;
+    const long i = c.i;
+    p[i] = 0.0;
+    r[i] = 0.0;
+    u[i] = energy[i] * density[i];
+    if (in_one(g, c)) w[i] = (coef == CONDUCTIVITY) ? density[i] : 1.0 / density[i];
+    if (in_skew(g, c)) {
+        const double wc = (coef == CONDUCTIVITY) ? density[i] : 1.0 / density[i];
+        const double wl = (coef == CONDUCTIVITY) ? density[i - 1] : 1.0 / density[i - 1];
+        const double wd = (coef == CONDUCTIVITY) ? density[i - g.sx] : 1.0 / density[i - g.sx];
+        kx[i] = rx * (wl + wc) / (2.0 * wl * wc);
+        ky[i] = ry * (wd + wc) / (2.0 * wd * wc);
+#if TL_DIM == 3
+        const double wb = (coef == CONDUCTIVITY) ? density[i - g.sy] : 1.0 / density[i - g.sy];
+        kz[i] = rz * (wb + wc) / (2.0 * wb * wc);
+#endif
+    }
+    (void)rz; (void)kz;
+}
+
+// CGCalcRRO (:107-132)
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_rro(Geom g, KPtr k, int pre, const double* __restrict__ u,
+                                                       double* __restrict__ w, double* __restrict__ r,
+                                                       double* __restrict__ p, double* __restrict__ mi,
+                                                       double* __restrict__ z, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const double sm = smvp(g, u, k, i, s.kk, s.jj, s.ii);
+        w[i] = sm;
+        const double rv = u[i] - sm;
+        r[i] = rv;
+        double pv;
+        if (pre) {
+            const double m = 1.0 / diag(g, k, i);
+            mi[i] = m;
+            pv = m * rv;
+            z[i] = pv;
+        } else {
+            pv = rv;
+        }
+        p[i] = pv;
+        v[0] += rv * pv;
+    }
+    grid_reduce<1>(v, red, g.nblocks);
+}
+
+// CGCalcW (:178-189)
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const double* __restrict__ p,
+                                                     double* __restrict__ w, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const double sm = smvp(g, p, k, s.i, s.kk, s.jj, s.ii);
+        w[s.i] = sm;
+        v[0] += sm * p[s.i];
+    }
+    grid_reduce<1>(v, red, g.nblocks);
+}
+
+// CGCalcUr (:244-264)
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha, int pre, double* __restrict__ u,
+                                                      double* __restrict__ r, const double* __restrict__ p,
+                                                      const double* __restrict__ w, const double* __restrict__ mi,
+                                                      double* __restrict__ z, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        u[i] = u[i] + alpha * p[i];
+        const double rv = r[i] - alpha * w[i];
+        r[i] = rv;
+        if (pre) {
+            const double zv = mi[i] * rv;
+            z[i] = zv;
+            v[0] += rv * zv;
+        } else {
+            v[0] += rv * rv;
+        }
+    }
+    grid_reduce<1>(v, red, g.nblocks);
+}
+
+// CGCalcP (:301-310)
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta, int pre, double* __restrict__ p,
+                                                     const double* __restrict__ r, const double* __restrict__ z)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        p[i] = beta * p[i] + (pre ? z[i] : r[i]);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Chebyshev (ext_cheby_kernel.hpp)
+
+// ChebyInit (:44-54) when first, ChebyIterate (:78-90) otherwise.  The plain
+// form writes w and updates p in place; ChebyCalcU (:50-58) follows as its
+// own launch.  The fused form also does u += p, writing u_out (!= u).
+template <bool FIRST, bool FUSED>
+__global__ __launch_bounds__(BLOCK) void k_cheby(Geom g, KPtr k, int pre, double theta, double a, double b,
+                                                 const double* __restrict__ u, const double* __restrict__ u0,
+                                                 const double* __restrict__ mi, double* __restrict__ w,
+                                                 double* __restrict__ r, double* __restrict__ p,
+                                                 double* __restrict__ u_out)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const double sm = smvp(g, u, k, i, s.kk, s.jj, s.ii);
+        if (!FUSED) w[i] = sm;
+        const double rv = u0[i] - sm;
+        r[i] = rv;
+        const double zr = pre ? mi[i] * rv : rv;
+        const double pv = FIRST ? zr / theta : a * p[i] + b * zr;
+        p[i] = pv;
+        if (FUSED) u_out[i] = u[i] + pv;
+    }
+}
+
+// ChebyCalcU (:50-58)
+__global__ __launch_bounds__(BLOCK) void k_add_p(Geom g, double* __restrict__ u, const double* __restrict__ p)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) u[s.i] = u[s.i] + p[s.i];
+}
+
+// ----------------------------------------------------------------------------
+// PPCG (ext_ppcg_kernel.hpp)
+
+// PPCGInitSd (:18-26)
+__global__ __launch_bounds__(BLOCK) void k_ppcg_init_sd(Geom g, int pre, double theta, const double* __restrict__ r,
+                                                        const double* __restrict__ mi, double* __restrict__ sd)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        sd[i] = (pre ? mi[i] * r[i] : r[i]) / theta;
+    }
+}
+
+// PPCGCalcU (:47-57): r -= A sd; u += sd
+__global__ __launch_bounds__(BLOCK) void k_ppcg_calc_u(Geom g, KPtr k, const double* __restrict__ sd,
+                                                       double* __restrict__ r, double* __restrict__ u)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const double sm = smvp(g, sd, k, i, s.kk, s.jj, s.ii);
+        r[i] = r[i] - sm;
+        u[i] = u[i] + sd[i];
+    }
+}
+
+// PPCGCalcSd (:80-89)
+__global__ __launch_bounds__(BLOCK) void k_ppcg_calc_sd(Geom g, int pre, double a, double b,
+                                                        const double* __restrict__ r, const double* __restrict__ mi,
+                                                        double* __restrict__ sd)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        sd[i] = a * sd[i] + b * (pre ? mi[i] * r[i] : r[i]);
+    }
+}
+
+// PPCGCalcU followed by PPCGCalcSd in one sweep; sd_out != sd.
+__global__ __launch_bounds__(BLOCK) void k_ppcg_inner_fused(Geom g, KPtr k, int pre, double a, double b,
+                                                            const double* __restrict__ sd,
+                                                            double* __restrict__ r, double* __restrict__ u,
+                                                            const double* __restrict__ mi,
+                                                            double* __restrict__ sd_out)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const double sdc = sd[i];
+        const double sm = smvp(g, sd, k, i, s.kk, s.jj, s.ii);
+        const double rv = r[i] - sm;
+        r[i] = rv;
+        u[i] = u[i] + sdc;
+        sd_out[i] = a * sdc + b * (pre ? mi[i] * rv : rv);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Jacobi (ext_jacobi_kernel.hpp)
+
+// JacobiInit (:126-150)
+__global__ void k_jacobi_init(Geom g, int coef, double rx, double ry, double rz, const double* __restrict__ density,
+                              const double* __restrict__ energy, double* __restrict__ u0, double* __restrict__ u,
+                              double* __restrict__ kx, double* __restrict__ ky, double* __restrict__ kz)
+{
+    Flat c;
+    if (!flat_cell(g, (long)blockIdx.x * FLAT_BLOCK + threadIdx.x, c)) return;
+    const long i = c.i;
+    if (in_one(g, c)) {
+        const double v = energy[i] * density[i];
+        u0[i] = v;
+        u[i] = v;
+    }
+    if (in_skew(g, c)) {
+        const double dC = (coef == CONDUCTIVITY) ? density[i] : 1.0 / density[i];
+        const double dL = (coef == CONDUCTIVITY) ? density[i - 1] : 1.0 / density[i - 1];
+        const double dD = (coef == CONDUCTIVITY) ? density[i - g.sx] : 1.0 / density[i - g.sx];
+        kx[i] = rx * (dL + dC) / (2.0 * dL * dC);
+        ky[i] = ry * (dD + dC) / (2.0 * dD * dC);
+#if TL_DIM == 3
+        const double dB = (coef == CONDUCTIVITY) ? density[i - g.sy] : 1.0 / density[i - g.sy];
+        kz[i] = rz * (dB + dC) / (2.0 * dB * dC);
+#endif
+    }
+    (void)rz; (void)kz;
+}
+
+// JacobiSolve (:200-209): un = (u0 + sum k*r_nbr) / (1 + sum k); error += |un - r|.
+// `prev` is the previous iterate (the reference's copy r = u); the result
+// goes to `next`, which may be the buffer u lived in only when prev != next.
+__global__ __launch_bounds__(BLOCK) void k_jacobi_solve(Geom g, KPtr k, const double* __restrict__ u0,
+                                                        const double* __restrict__ prev, double* __restrict__ next,
+                                                        Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const double rc = prev[i];
+        const double rw = prev[s.kk > HALO ? i - 1 : i];
+        const double re = prev[s.kk < g.nx + 1 ? i + 1 : i];
+        const double rs = prev[s.jj > HALO ? i - g.sx : i];
+        const double rn = prev[s.jj < g.ny + 1 ? i + g.sx : i];
+        const double kxc = k.kx[i], kxe = k.kx[i + 1], kyc = k.ky[i], kyn = k.ky[i + g.sx];
+#if TL_DIM == 3
+        const double rb = prev[s.ii > HALO ? i - g.sy : i];
+        const double rf = prev[s.ii < g.nz + 1 ? i + g.sy : i];
+        const double kzc = k.kz[i], kzf = k.kz[i + g.sy];
+        const double un = (u0[i] + (kxe * re + kxc * rw) + (kyn * rn + kyc * rs) + (kzf * rf + kzc * rb))
+                        / (1.0 + (kxc + kxe) + (kyc + kyn) + (kzc + kzf));
+#else
+        const double un = (u0[i] + (kxe * re + kxc * rw) + (kyn * rn + kyc * rs))
+                        / (1.0 + (kxc + kxe) + (kyc + kyn));
+#endif
+        next[i] = un;
+        v[0] += fabs(un - rc);
+    }
+    grid_reduce<1>(v, red, g.nblocks);
+}
+
+// ----------------------------------------------------------------------------
+// Shared solver methods (ext_solver_methods.hpp)
+
+// CopyU (:19-25)
+__global__ __launch_bounds__(BLOCK) void k_copy_u(Geom g, const double* __restrict__ u, double* __restrict__ u0)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) u0[s.i] = u[s.i];
+}
+
+// CalculateResidual (:48-58)
+__global__ __launch_bounds__(BLOCK) void k_residual(Geom g, KPtr k, const double* __restrict__ u,
+                                                    const double* __restrict__ u0, double* __restrict__ r)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const double sm = smvp(g, u, k, s.i, s.kk, s.jj, s.ii);
+        r[s.i] = u0[s.i] - sm;
+    }
+}
+
+// Calculate2Norm (:80-89)
+__global__ __launch_bounds__(BLOCK) void k_2norm(Geom g, const double* __restrict__ a, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) v[0] += a[s.i] * a[s.i];
+    grid_reduce<1>(v, red, g.nblocks);
+}
+
+// Finalise (:104-111)
+__global__ __launch_bounds__(BLOCK) void k_finalise(Geom g, const double* __restrict__ u,
+                                                    const double* __restrict__ density, double* __restrict__ energy)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) energy[s.i] = u[s.i] / density[s.i];
+}
+
+inline unsigned flat_blocks(const Geom& g)
+{
+    const long n = (long)g.X * g.Y * g.Z;
+    return (unsigned)((n + FLAT_BLOCK - 1) / FLAT_BLOCK);
+}
+
+inline KPtr kptr(const Chunk& c) { return KPtr{c.f[F_KX], c.f[F_KY], c.f[F_KZ]}; }
+
+inline void check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+#define STRIP_LAUNCH(kernel, c, ...)                                                          \
+    do {                                                                                      \
+        hipLaunchKernelGGL(kernel, dim3((unsigned)(c).g.nblocks), dim3(BLOCK), 0, (c).stream, \
+                           (c).g, __VA_ARGS__);                                               \
+        check_launch();                                                                       \
+    } while (0)
+
+// Runs STMT between two timer events when the chunk's timer is on.
+#define TIMED(c, kid, STMT)                                  \
+    do {                                                     \
+        if ((c).timer.on) {                                  \
+            KernelTimer::Pending p_;                         \
+            (c).timer.begin((kid), (c).stream, p_);          \
+            STMT;                                            \
+            (c).timer.end(p_, (c).stream);                   \
+        } else {                                             \
+            STMT;                                            \
+        }                                                    \
+    } while (0)
+
+#define FLAT_LAUNCH(kernel, c, ...)                                                               \
+    do {                                                                                          \
+        hipLaunchKernelGGL(kernel, dim3(flat_blocks((c).g)), dim3(FLAT_BLOCK), 0, (c).stream, (c).g, \
+                           __VA_ARGS__);                                                          \
+        check_launch();                                                                           \
+    } while (0)
+
+}  // namespace
+
+// ============================================================================
+// Kernel timer
+
+static int g_timing_from = 0;
+void set_kernel_timing_from(int step) { g_timing_from = step < 0 ? 0 : step; }
+int kernel_timing_from() { return g_timing_from; }
+
+const char* kernel_name(int k)
+{
+    static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_ur", "cg_calc_p", "cheby_iterate",
+                                         "ppcg_init_sd", "ppcg_inner", "jacobi_solve", "calc_residual",
+                                         "calc_2norm", "copy_u", "finalise", "field_summary", "update_halo"};
+    return (k >= 0 && k < K_COUNT) ? names[k] : "?";
+}
+
+double kernel_bytes_per_cell(int k)
+{
+    const double kb = 8.0 * TL_DIM;  // the kx, ky[, kz] coefficient arrays
+    switch (k) {
+    case K_CG_W: return 16.0 + kb;         // p, coefficients -> w
+    case K_CG_UR: return 48.0;             // u, p, r, w -> u, r
+    case K_CG_P: return 24.0;              // p, r -> p
+    case K_CHEBY: return 48.0 + kb;        // u, u0, p, coefficients -> r, p, u'
+    case K_PPCG_SD: return 16.0;           // r -> sd
+    case K_PPCG_INNER: return 48.0 + kb;   // sd, r, u, coefficients -> r, u, sd'
+    case K_JACOBI: return 24.0 + kb;       // u, u0, coefficients -> u'
+    case K_RESIDUAL: return 24.0 + kb;     // u, u0, coefficients -> r
+    case K_2NORM: return 8.0;
+    case K_COPY_U: return 16.0;
+    case K_FINALISE: return 24.0;
+    case K_SUMMARY: return 32.0;
+    default: return 0.0;
+    }
+}
+
+hipEvent_t KernelTimer::take()
+{
+    if (pool.empty()) {
+        hipEvent_t e;
+        TL_HIP(hipEventCreate(&e));
+        return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+}
+
+void KernelTimer::begin(int k, hipStream_t s, Pending& p)
+{
+    if (pending.size() >= 512) flush();
+    p.k = k;
+    p.a = take();
+    p.b = take();
+    TL_HIP(hipEventRecord(p.a, s));
+}
+
+void KernelTimer::end(Pending& p, hipStream_t s)
+{
+    TL_HIP(hipEventRecord(p.b, s));
+    pending.push_back(p);
+}
+
+void KernelTimer::flush()
+{
+    if (pending.empty()) return;
+    TL_HIP(hipEventSynchronize(pending.back().b));
+    for (Pending& p : pending) {
+        float t = 0.0f;
+        TL_HIP(hipEventElapsedTime(&t, p.a, p.b));
+        ms[p.k] += t;
+        count[p.k] += 1;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+}
+
+KernelTimer::~KernelTimer()
+{
+    for (Pending& p : pending) {
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+// ============================================================================
+// Chunk
+
+static Geom make_geom(int nx, int ny, int nz)
+{
+    Geom g{};
+    g.nx = nx;
+    g.ny = ny;
+    g.X = nx + 2 * HALO;
+    g.Y = ny + 2 * HALO;
+#if TL_DIM == 3
+    g.nz = nz;
+    g.Z = nz + 2 * HALO;
+#else
+    (void)nz;
+    g.nz = 1;
+    g.Z = 1;
+#endif
+    g.sx = ((XOFF + g.X + ROW_ALIGN - 1) / ROW_ALIGN) * ROW_ALIGN;
+    g.sy = g.sx * g.Y;
+    g.base = XOFF;
+    g.alloc = g.base + g.sy * g.Z + ROW_ALIGN;
+    g.tiles_x = (nx + WAVE - 1) / WAVE;
+#if TL_DIM == 3
+    g.tiles_y = (ny + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    const long cols = (long)g.tiles_x * g.tiles_y;
+    const int len = nz;
+    const int per = 1;
+#else
+    g.tiles_y = 1;
+    const long cols = g.tiles_x;
+    const int len = ny;
+    const int per = WAVES_PER_BLOCK;
+#endif
+    // Longest strip (<= 16) that still gives >= 2048 workgroups.
+    int strip = 16;
+    while (strip > 1 && cols * ((len + (long)per * strip - 1) / ((long)per * strip)) < 2048) strip /= 2;
+    g.strip = strip;
+    g.tiles_s = (int)((len + (long)per * strip - 1) / ((long)per * strip));
+    g.nblocks = cols * g.tiles_s;
+    return g;
+}
+
+Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
+{
+#if TL_DIM == 3
+    if (nx < 2 || ny < 2 || nz < 2) throw Error("a chunk needs at least 2 interior cells per axis");
+#else
+    if (nx < 2 || ny < 2) throw Error("a chunk needs at least 2 interior cells per axis");
+#endif
+    g = make_geom(nx, ny, nz);
+    TL_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const size_t bytes = sizeof(double) * (size_t)g.alloc;
+    for (int k = 0; k < F_COUNT; ++k) {
+#if TL_DIM == 2
+        if (k == F_KZ) continue;
+#endif
+        TL_HIP(hipMalloc(&f[k], bytes));
+        TL_HIP(hipMemsetAsync(f[k], 0, bytes, stream));
+    }
+    TL_HIP(hipMalloc(&u_alt, bytes));
+    TL_HIP(hipMemsetAsync(u_alt, 0, bytes, stream));
+    TL_HIP(hipMalloc(&sd_alt, bytes));
+    TL_HIP(hipMemsetAsync(sd_alt, 0, bytes, stream));
+    TL_HIP(hipMalloc(&vertexX, sizeof(double) * (g.X + 1)));
+    TL_HIP(hipMalloc(&vertexY, sizeof(double) * (g.Y + 1)));
+    TL_HIP(hipMalloc(&vertexZ, sizeof(double) * (g.Z + 1)));
+    TL_HIP(hipMalloc(&cellX, sizeof(double) * g.X));
+    TL_HIP(hipMalloc(&cellY, sizeof(double) * g.Y));
+    TL_HIP(hipMalloc(&cellZ, sizeof(double) * g.Z));
+    TL_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)g.nblocks));
+    TL_HIP(hipMalloc(&ticket, sizeof(unsigned)));
+    TL_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), stream));
+    TL_HIP(hipMalloc(&d_result, sizeof(double) * 4 * S_COUNT));
+    double* h = nullptr;
+    TL_HIP(hipHostMalloc(&h, sizeof(double) * 4 * S_COUNT, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(h, 0, sizeof(double) * 4 * S_COUNT);
+    h_result = h;
+    TL_HIP(hipHostGetDevicePointer((void**)&h_result_dev, h, 0));
+    TL_HIP(hipStreamSynchronize(stream));
+}
+
+Chunk::~Chunk()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (double* p : f) (void)hipFree(p);
+    (void)hipFree(u_alt);
+    (void)hipFree(sd_alt);
+    (void)hipFree(vertexX);
+    (void)hipFree(vertexY);
+    (void)hipFree(vertexZ);
+    (void)hipFree(cellX);
+    (void)hipFree(cellY);
+    (void)hipFree(cellZ);
+    (void)hipFree(partials);
+    (void)hipFree(ticket);
+    (void)hipFree(d_result);
+    if (h_result) (void)hipHostFree((void*)h_result);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+Reduce Chunk::reducer(Slot s) const
+{
+    return Reduce{partials, ticket, d_result + 4 * s, h_result_dev + 4 * s};
+}
+
+void Chunk::sync() const
+{
+    TL_HIP(hipStreamSynchronize(stream));
+    if (timer.on) timer.flush();
+}
+
+double Chunk::result(Slot s, int k) const
+{
+    sync();
+    return h_result[4 * s + k];
+}
+
+void Chunk::get_field(int id, double* host) const
+{
+    if (id < 0 || id >= F_COUNT || !f[id]) throw Error("invalid field id " + std::to_string(id));
+    TL_HIP(hipMemcpy2DAsync(host, sizeof(double) * g.X, f[id] + g.base, sizeof(double) * g.sx,
+                            sizeof(double) * g.X, (size_t)g.Y * g.Z, hipMemcpyDeviceToHost, stream));
+    sync();
+}
+
+void Chunk::set_field(int id, const double* host)
+{
+    if (id < 0 || id >= F_COUNT || !f[id]) throw Error("invalid field id " + std::to_string(id));
+    TL_HIP(hipMemcpy2DAsync(f[id] + g.base, sizeof(double) * g.sx, host, sizeof(double) * g.X,
+                            sizeof(double) * g.X, (size_t)g.Y * g.Z, hipMemcpyHostToDevice, stream));
+    sync();
+}
+
+// ============================================================================
+// Operations
+
+// ext_core.cpp:33-52 (vertices and cell centres are a few thousand values,
+// computed on the host with the reference's expressions and copied once).
+void op_initialise_chunk(Chunk& c, double xmin, double ymin, double zmin, double dx, double dy, double dz)
+{
+    c.dx = dx;
+    c.dy = dy;
+    const Geom& g = c.g;
+    std::vector<double> vx(g.X + 1), vy(g.Y + 1), vz(g.Z + 1), cx(g.X), cy(g.Y), cz(g.Z);
+    for (int i = 0; i < g.X + 1; ++i) vx[i] = xmin + dx * (i - HALO);
+    for (int i = 0; i < g.Y + 1; ++i) vy[i] = ymin + dy * (i - HALO);
+    for (int i = 0; i < g.X; ++i) cx[i] = 0.5 * (vx[i] + vx[i + 1]);
+    for (int i = 0; i < g.Y; ++i) cy[i] = 0.5 * (vy[i] + vy[i + 1]);
+#if TL_DIM == 3
+    c.dz = dz;
+    for (int i = 0; i < g.Z + 1; ++i) vz[i] = zmin + dz * (i - HALO);
+    for (int i = 0; i < g.Z; ++i) cz[i] = 0.5 * (vz[i] + vz[i + 1]);
+    const double vol = dx * dy * dz;
+#else
+    (void)zmin;
+    (void)dz;
+    const double vol = dx * dy;
+#endif
+    TL_HIP(hipMemcpyAsync(c.vertexX, vx.data(), sizeof(double) * vx.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(c.vertexY, vy.data(), sizeof(double) * vy.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(c.vertexZ, vz.data(), sizeof(double) * vz.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(c.cellX, cx.data(), sizeof(double) * cx.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(c.cellY, cy.data(), sizeof(double) * cy.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(c.cellZ, cz.data(), sizeof(double) * cz.size(), hipMemcpyHostToDevice, c.stream));
+    FLAT_LAUNCH(k_fill, c, c.f[F_VOLUME], vol);
+    c.sync();  // the host vectors die here
+}
+
+// ext_core.cpp:57-126
+void op_generate_chunk(Chunk& c, int n, const double* density, const double* energy, const double* xmin,
+                       const double* xmax, const double* ymin, const double* ymax, const double* zmin,
+                       const double* zmax, const double* radius, const int* geometry, int rect, int circ,
+                       int point)
+{
+    if (n < 1) throw Error("generate_chunk needs at least one state");
+    std::vector<double> hs(9 * (size_t)n, 0.0);
+    const double* src[9] = {density, energy, xmin, xmax, ymin, ymax, zmin, zmax, radius};
+    for (int a = 0; a < 9; ++a)
+        if (src[a]) std::memcpy(&hs[(size_t)a * n], src[a], sizeof(double) * n);
+    double* ds = nullptr;
+    int* dg = nullptr;
+    TL_HIP(hipMalloc(&ds, sizeof(double) * hs.size()));
+    TL_HIP(hipMalloc(&dg, sizeof(int) * n));
+    TL_HIP(hipMemcpyAsync(ds, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, c.stream));
+    TL_HIP(hipMemcpyAsync(dg, geometry, sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+    StateArgs s{n, rect, circ, point, ds, ds + n, ds + 2 * n, ds + 3 * n, ds + 4 * n, ds + 5 * n,
+                ds + 6 * n, ds + 7 * n, ds + 8 * n, dg};
+    FLAT_LAUNCH(k_generate, c, s, c.vertexX, c.vertexY, c.vertexZ, c.cellX, c.cellY, c.cellZ, c.f[F_ENERGY0],
+                c.f[F_DENSITY], c.f[F_U]);
+    c.sync();
+    (void)hipFree(ds);
+    (void)hipFree(dg);
+}
+
+// ext_core.cpp:249-256 (energy1 = energy0 over the full domain)
+void op_set_field(Chunk& c)
+{
+    TL_HIP(hipMemcpyAsync(c.f[F_ENERGY1], c.f[F_ENERGY0], sizeof(double) * c.g.alloc, hipMemcpyDeviceToDevice,
+                          c.stream));
+}
+
+void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* temp)
+{
+    TIMED(c, K_SUMMARY, STRIP_LAUNCH(k_field_summary, c, c.f[F_VOLUME], c.f[F_DENSITY], c.f[F_ENERGY0], c.f[F_U],
+                                     c.reducer(S_SUMMARY)));
+    c.sync();
+    *vol = c.h_result[4 * S_SUMMARY + 0];
+    *mass = c.h_result[4 * S_SUMMARY + 1];
+    *ie = c.h_result[4 * S_SUMMARY + 2];
+    *temp = c.h_result[4 * S_SUMMARY + 3];
+}
+
+void op_update_halo(Chunk& c, const int* neighbours, const int* fields, int depth)
+{
+    if (depth < 1 || depth > HALO) throw Error("halo depth must be 1 or 2");
+    int mask = 0;
+    for (int f = 0; f < NUM_FACES; ++f) {
+        if (neighbours[f] != EXTERNAL_FACE) continue;
+        switch (f + 1) {
+        case CHUNK_LEFT: mask |= 1; break;
+        case CHUNK_RIGHT: mask |= 2; break;
+        case CHUNK_BOTTOM: mask |= 4; break;
+        case CHUNK_TOP: mask |= 8; break;
+        case CHUNK_BACK: mask |= 16; break;
+        case CHUNK_FRONT: mask |= 32; break;
+        }
+    }
+    HaloFields hf{};
+    static const int field_of[NUM_FIELDS] = {F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD};
+    for (int q = 0; q < NUM_FIELDS; ++q)
+        if (fields[q]) hf.f[hf.count++] = c.f[field_of[q]];
+    if (hf.count == 0 || mask == 0) return;
+    const Geom& g = c.g;
+#if TL_DIM == 3
+    const long n = 4L * g.ny * g.nz + 4L * g.nx * g.nz + 4L * g.nx * g.ny;
+#else
+    const long n = 4L * g.X + 4L * g.ny;
+#endif
+    TIMED(c, K_HALO, hipLaunchKernelGGL(k_update_halo, dim3((unsigned)((n + FLAT_BLOCK - 1) / FLAT_BLOCK)),
+                                        dim3(FLAT_BLOCK), 0, c.stream, g, hf, depth, mask));
+    check_launch();
+}
+
+// ext_core.cpp:377-413
+double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt)
+{
+    if (coefficient < CONDUCTIVITY && coefficient < RECIP_CONDUCTIVITY)
+        throw Error("Coefficient " + std::to_string(coefficient) + " is not valid.");
+    c.preconditioner = preconditioner;
+    const double rx = dt / (c.dx * c.dx);
+    const double ry = dt / (c.dy * c.dy);
+    const double rz = TL_DIM == 3 ? dt / (c.dz * c.dz) : 0.0;
+    TIMED(c, K_CG_INIT, {
+        FLAT_LAUNCH(k_cg_init_k, c, coefficient, rx, ry, rz, c.f[F_DENSITY], c.f[F_ENERGY1], c.f[F_U], c.f[F_P],
+                    c.f[F_R], c.f[F_W], c.f[F_KX], c.f[F_KY], c.f[F_KZ]);
+        STRIP_LAUNCH(k_cg_calc_rro, c, kptr(c), (int)preconditioner, c.f[F_U], c.f[F_W], c.f[F_R], c.f[F_P],
+                     c.f[F_MI], c.f[F_Z], c.reducer(S_RRO));
+    });
+    return c.result(S_RRO);
+}
+
+// ext_core.cpp:417-429
+double op_cg_calc_w(Chunk& c)
+{
+    TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w, c, kptr(c), c.f[F_P], c.f[F_W], c.reducer(S_PW)));
+    return c.result(S_PW);
+}
+
+// ext_core.cpp:433-447
+double op_cg_calc_ur(Chunk& c, double alpha)
+{
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, alpha, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
+                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN)));
+    return c.result(S_RRN);
+}
+
+// ext_core.cpp:451-464
+void op_cg_calc_p(Chunk& c, double beta)
+{
+    TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, beta, (int)c.preconditioner, c.f[F_P], c.f[F_R], c.f[F_Z]));
+}
+
+static void store_coefs(Chunk& c, const double* alphas, const double* betas, int n)
+{
+    if (n < 0) throw Error("negative coefficient count");
+    c.alphas.assign(alphas, alphas + n);
+    c.betas.assign(betas, betas + n);
+}
+
+// ext_core.cpp:468-497
+void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, double theta, bool preconditioner,
+                   bool fused)
+{
+    c.preconditioner = preconditioner;
+    store_coefs(c, alphas, betas, n);
+    const int pre = preconditioner;
+    if (fused) {
+        STRIP_LAUNCH((k_cheby<true, true>), c, kptr(c), pre, theta, 0.0, 0.0, c.f[F_U], c.f[F_U0], c.f[F_MI],
+                     c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt);
+        std::swap(c.f[F_U], c.u_alt);
+    } else {
+        STRIP_LAUNCH((k_cheby<true, false>), c, kptr(c), pre, theta, 0.0, 0.0, c.f[F_U], c.f[F_U0], c.f[F_MI],
+                     c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt);
+        STRIP_LAUNCH(k_add_p, c, c.f[F_U], c.f[F_P]);
+    }
+}
+
+// ext_core.cpp:501-515
+void op_cheby_iterate(Chunk& c, int step1, bool fused)
+{
+    const int s = step1 - 1;
+    if (s < 0 || s >= (int)c.alphas.size())
+        throw Error("Chebyshev step " + std::to_string(step1) + " outside the stored coefficients");
+    const double a = c.alphas[s], b = c.betas[s];
+    const int pre = c.preconditioner;
+    if (fused) {
+        TIMED(c, K_CHEBY, STRIP_LAUNCH((k_cheby<false, true>), c, kptr(c), pre, 0.0, a, b, c.f[F_U], c.f[F_U0],
+                                       c.f[F_MI], c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt));
+        std::swap(c.f[F_U], c.u_alt);
+    } else {
+        STRIP_LAUNCH((k_cheby<false, false>), c, kptr(c), pre, 0.0, a, b, c.f[F_U], c.f[F_U0], c.f[F_MI], c.f[F_W],
+                     c.f[F_R], c.f[F_P], c.u_alt);
+        STRIP_LAUNCH(k_add_p, c, c.f[F_U], c.f[F_P]);
+    }
+}
+
+// ext_core.cpp:519-539
+void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, double theta)
+{
+    c.theta = theta;
+    store_coefs(c, alphas, betas, n);
+}
+
+// ext_core.cpp:543-550
+void op_ppcg_init_sd(Chunk& c)
+{
+    TIMED(c, K_PPCG_SD, STRIP_LAUNCH(k_ppcg_init_sd, c, (int)c.preconditioner, c.theta, c.f[F_R], c.f[F_MI],
+                                     c.f[F_SD]));
+}
+
+// ext_core.cpp:554-567
+void op_ppcg_inner(Chunk& c, int step1, bool fused)
+{
+    const int s = step1 - 1;
+    if (s < 0 || s >= (int)c.alphas.size())
+        throw Error("PPCG step " + std::to_string(step1) + " outside the stored coefficients");
+    const double a = c.alphas[s], b = c.betas[s];
+    const int pre = c.preconditioner;
+    if (fused) {
+        TIMED(c, K_PPCG_INNER, STRIP_LAUNCH(k_ppcg_inner_fused, c, kptr(c), pre, a, b, c.f[F_SD], c.f[F_R],
+                                            c.f[F_U], c.f[F_MI], c.sd_alt));
+        std::swap(c.f[F_SD], c.sd_alt);
+    } else {
+        STRIP_LAUNCH(k_ppcg_calc_u, c, kptr(c), c.f[F_SD], c.f[F_R], c.f[F_U]);
+        STRIP_LAUNCH(k_ppcg_calc_sd, c, pre, a, b, c.f[F_R], c.f[F_MI], c.f[F_SD]);
+    }
+}
+
+// ext_core.cpp:340-358
+void op_jacobi_init(Chunk& c, int coefficient, double dt)
+{
+    if (coefficient < CONDUCTIVITY && coefficient < RECIP_CONDUCTIVITY)
+        throw Error("Coefficient " + std::to_string(coefficient) + " is not valid.");
+    const double rx = dt / (c.dx * c.dx);
+    const double ry = dt / (c.dy * c.dy);
+    const double rz = TL_DIM == 3 ? dt / (c.dz * c.dz) : 0.0;
+    FLAT_LAUNCH(k_jacobi_init, c, coefficient, rx, ry, rz, c.f[F_DENSITY], c.f[F_ENERGY1], c.f[F_U0], c.f[F_U],
+                c.f[F_KX], c.f[F_KY], c.f[F_KZ]);
+}
+
+// ext_core.cpp:362-373.  Plain form: r = u over the whole allocation, then
+// u from r in place.  Fused form: the new iterate goes into r's buffer and
+// the two swap, leaving r holding the previous iterate as the reference's
+// copy does.
+double op_jacobi_solve(Chunk& c, bool fused)
+{
+    if (fused) {
+        TIMED(c, K_JACOBI, STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_U], c.f[F_R], c.reducer(S_ERR)));
+        std::swap(c.f[F_U], c.f[F_R]);
+    } else {
+        TL_HIP(hipMemcpyAsync(c.f[F_R], c.f[F_U], sizeof(double) * c.g.alloc, hipMemcpyDeviceToDevice, c.stream));
+        STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_R], c.f[F_U], c.reducer(S_ERR));
+    }
+    return c.result(S_ERR);
+}
+
+void op_copy_u(Chunk& c) { TIMED(c, K_COPY_U, STRIP_LAUNCH(k_copy_u, c, c.f[F_U], c.f[F_U0])); }
+
+void op_calc_residual(Chunk& c)
+{
+    TIMED(c, K_RESIDUAL, STRIP_LAUNCH(k_residual, c, kptr(c), c.f[F_U], c.f[F_U0], c.f[F_R]));
+}
+
+double op_calc_2norm(Chunk& c, bool norm_r)
+{
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, norm_r ? c.f[F_R] : c.f[F_U0], c.reducer(S_NORM)));
+    return c.result(S_NORM);
+}
+
+void op_finalise(Chunk& c)
+{
+    TIMED(c, K_FINALISE, STRIP_LAUNCH(k_finalise, c, c.f[F_U], c.f[F_DENSITY], c.f[F_ENERGY1]));
+}
+
+}  // namespace tl

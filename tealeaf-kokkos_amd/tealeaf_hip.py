@@ -1,0 +1,264 @@
+"""Python access to the gfx950 TeaLeaf libraries through their C ABI.
+
+The libraries (``libtealeaf2d_hip.so`` and ``libtealeaf3d_hip.so``, built
+beside this file by its Makefile) export the reference's ``ext_*_`` kernel
+entry points plus a C++ driver entry point ``tl_run_deck``.  This module
+loads them with ctypes; nothing here computes anything on the CPU, and every
+call fails loudly when a library is missing.
+
+``ExtChunk`` wraps one chunk of any library exporting the ``ext_*_`` ABI
+with the reference's calling conventions (every argument by pointer), so the
+same test code can drive the HIP library and, for comparison, the oracle.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+FIELDS = ["density", "energy0", "energy1", "u", "u0", "p", "r", "mi", "w", "z",
+          "kx", "ky", "kz", "sd", "volume"]
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+
+# Fortran-side codes (tealeaf_2d/ext_shared.hpp:10-29).
+FIELD_DENSITY, FIELD_ENERGY0, FIELD_ENERGY1, FIELD_U, FIELD_P, FIELD_SD = 1, 2, 3, 4, 5, 6
+CONDUCTIVITY, RECIP_CONDUCTIVITY = 1, 2
+G_RECT, G_CIRC, G_POINT = 1, 2, 3
+EXTERNAL_FACE = -1
+
+_P_INT = ctypes.POINTER(ctypes.c_int)
+_P_DBL = ctypes.POINTER(ctypes.c_double)
+
+
+def build(jobs: int = 8) -> None:
+    """Compile both libraries for gfx950 with the package Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}"], check=True)
+
+
+def library_path(dim: int) -> str:
+    return os.path.join(HERE, f"libtealeaf{dim}d_hip.so")
+
+
+_LIBS: dict[int, ctypes.CDLL] = {}
+
+
+def parse_report(text: str) -> dict:
+    """Load a driver report; C's printf spells non-finite values nan/inf,
+    which become JSON's NaN/Infinity."""
+    text = re.sub(r"(:\s*)-?nan\b", r"\1NaN", text)
+    text = re.sub(r"(:\s*)(-?)inf\b", r"\1\2Infinity", text)
+    return json.loads(text)
+
+
+def lib(dim: int) -> ctypes.CDLL:
+    """Load the HIP library for ``dim`` (2 or 3); raises when it is not built."""
+    if dim not in (2, 3):
+        raise ValueError("dim must be 2 or 3")
+    if dim not in _LIBS:
+        path = library_path(dim)
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run tealeaf_hip.build()")
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        L.tl_run_deck.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.tl_run_deck.restype = ctypes.c_int
+        L.tl_driver_error.restype = ctypes.c_char_p
+        L.tl_last_error.restype = ctypes.c_char_p
+        L.tl_set_abort_on_error(0)
+        _LIBS[dim] = L
+    return _LIBS[dim]
+
+
+def run_deck(deck: str, dim: int = 2, max_steps: int = 0, json_cap: int = 1 << 22) -> dict:
+    """Run a ``tea.in`` deck on the GPU through the C++ driver; returns its JSON report."""
+    L = lib(dim)
+    buf = ctypes.create_string_buffer(json_cap)
+    rc = L.tl_run_deck(deck.encode(), buf, json_cap, int(max_steps))
+    if rc != 0:
+        raise RuntimeError(f"tl_run_deck failed ({rc}): {L.tl_driver_error().decode()}")
+    return parse_report(buf.value.decode())
+
+
+def _i(v: int):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _d(v: float):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+class ExtChunk:
+    """One chunk driven through the ``ext_*_`` C ABI of ``libhandle``.
+
+    Arrays are exchanged in the reference's flat layout: shape (Z, Y, X) with
+    X = nx + 4 fastest (Z = 1 in 2D).
+    """
+
+    def __init__(self, libhandle: ctypes.CDLL, dim: int, nx: int, ny: int, nz: int = 1):
+        self.L = libhandle
+        self.dim = dim
+        self.L.ext_finalise_()
+        if dim == 3:
+            self.L.ext_init_(_i(nx), _i(ny), _i(nz), _i(0))
+        else:
+            self.L.ext_init_(_i(nx), _i(ny), _i(0))
+        self._check()
+        self.chunk = ctypes.c_int(1)
+        dims = (ctypes.c_int * 6)()
+        if self.L.tl_chunk_dims(ctypes.byref(self.chunk), dims) != 0:
+            raise RuntimeError("chunk was not created")
+        self.X, self.Y, self.Z = dims[0], dims[1], dims[2]
+        self.nx, self.ny, self.nz = dims[3], dims[4], dims[5]
+        self.shape = (self.Z, self.Y, self.X)
+
+    def _check(self):
+        msg = self.L.tl_last_error()
+        if msg:
+            self.L.tl_clear_error()
+            raise RuntimeError(msg.decode())
+
+    @property
+    def c(self):
+        return ctypes.byref(self.chunk)
+
+    def close(self):
+        self.L.ext_finalise_()
+
+    # ---- fields --------------------------------------------------------
+    def get(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        rc = self.L.tl_get_field(self.c, FIELD_ID[name], out.ctypes.data_as(_P_DBL))
+        if rc != 0:
+            self._check()
+            raise RuntimeError(f"tl_get_field({name}) failed")
+        return out
+
+    def set(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
+        rc = self.L.tl_set_field(self.c, FIELD_ID[name], a.ctypes.data_as(_P_DBL))
+        if rc != 0:
+            self._check()
+            raise RuntimeError(f"tl_set_field({name}) failed")
+
+    # ---- ext_* entry points --------------------------------------------
+    def initialise_chunk(self, xmin, ymin, dx, dy, zmin=0.0, dz=0.0):
+        if self.dim == 3:
+            self.L.ext_initialise_chunk_(self.c, _d(xmin), _d(ymin), _d(zmin), _d(dx), _d(dy), _d(dz))
+        else:
+            self.L.ext_initialise_chunk_(self.c, _d(xmin), _d(ymin), _d(dx), _d(dy))
+        self._check()
+
+    def generate_chunk(self, density, energy, xmin, xmax, ymin, ymax, radius, geometry,
+                       zmin=None, zmax=None):
+        n = len(density)
+        arr = lambda v: (ctypes.c_double * n)(*[float(x) for x in v])
+        geo = (ctypes.c_int * n)(*[int(x) for x in geometry])
+        if self.dim == 3:
+            self.L.ext_generate_chunk_(self.c, _i(n), arr(density), arr(energy), arr(xmin), arr(xmax),
+                                       arr(ymin), arr(ymax), arr(zmin), arr(zmax), arr(radius), geo,
+                                       _i(G_RECT), _i(G_CIRC), _i(G_POINT))
+        else:
+            self.L.ext_generate_chunk_(self.c, _i(n), arr(density), arr(energy), arr(xmin), arr(xmax),
+                                       arr(ymin), arr(ymax), arr(radius), geo,
+                                       _i(G_RECT), _i(G_CIRC), _i(G_POINT))
+        self._check()
+
+    def set_field_kernel(self):
+        self.L.ext_set_field_kernel_(self.c)
+        self._check()
+
+    def field_summary(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        self.L.ext_field_summary_kernel_(self.c, *[ctypes.byref(x) for x in v])
+        self._check()
+        return tuple(x.value for x in v)
+
+    def update_halo(self, fields, depth, neighbours=None):
+        nf = 6 if self.dim == 3 else 4
+        nb = (ctypes.c_int * nf)(*(neighbours or [EXTERNAL_FACE] * nf))
+        fl = (ctypes.c_int * 6)(*[1 if (k + 1) in fields else 0 for k in range(6)])
+        self.L.ext_update_halo_kernel_(self.c, nb, fl, _i(depth))
+        self._check()
+
+    def cg_solver_init(self, coefficient, preconditioner, dt):
+        rro = ctypes.c_double()
+        self.L.ext_cg_solver_init_(self.c, _i(coefficient), _i(1 if preconditioner else 0), _d(dt),
+                                   ctypes.byref(rro))
+        self._check()
+        return rro.value
+
+    def cg_calc_w(self):
+        pw = ctypes.c_double()
+        self.L.ext_cg_calc_w_(self.c, ctypes.byref(pw))
+        self._check()
+        return pw.value
+
+    def cg_calc_ur(self, alpha):
+        rrn = ctypes.c_double()
+        self.L.ext_cg_calc_ur_(self.c, _d(alpha), ctypes.byref(rrn))
+        self._check()
+        return rrn.value
+
+    def cg_calc_p(self, beta):
+        self.L.ext_cg_calc_p_(self.c, _d(beta))
+        self._check()
+
+    def cheby_solver_init(self, alphas, betas, theta, preconditioner):
+        n = len(alphas)
+        a = (ctypes.c_double * n)(*alphas)
+        b = (ctypes.c_double * n)(*betas)
+        self.L.ext_cheby_solver_init_(self.c, a, b, _d(theta), _i(1 if preconditioner else 0), _i(n))
+        self._check()
+
+    def cheby_solver_iterate(self, step):
+        self.L.ext_cheby_solver_iterate_(self.c, _i(step))
+        self._check()
+
+    def ppcg_init(self, alphas, betas, theta):
+        n = len(alphas)
+        a = (ctypes.c_double * n)(*alphas)
+        b = (ctypes.c_double * n)(*betas)
+        self.L.ext_ppcg_init_(self.c, a, b, _d(theta), _i(n))
+        self._check()
+
+    def ppcg_init_sd(self):
+        self.L.ext_ppcg_init_sd_(self.c)
+        self._check()
+
+    def ppcg_inner(self, step):
+        self.L.ext_ppcg_inner_(self.c, _i(step))
+        self._check()
+
+    def jacobi_init(self, coefficient, dt):
+        self.L.ext_jacobi_kernel_init_(self.c, _i(coefficient), _d(dt))
+        self._check()
+
+    def jacobi_solve(self):
+        err = ctypes.c_double()
+        self.L.ext_jacobi_kernel_solve_(self.c, ctypes.byref(err))
+        self._check()
+        return err.value
+
+    def copy_u(self):
+        self.L.ext_solver_copy_u_(self.c)
+        self._check()
+
+    def calculate_residual(self):
+        self.L.ext_calculate_residual_(self.c)
+        self._check()
+
+    def calculate_2norm(self, norm_r: bool):
+        v = ctypes.c_double()
+        flag = ctypes.c_bool(bool(norm_r))
+        self.L.ext_calculate_2norm_(self.c, ctypes.byref(v), ctypes.byref(flag))
+        self._check()
+        return v.value
+
+    def finalise(self):
+        self.L.ext_solver_finalise_(self.c)
+        self._check()

@@ -1,0 +1,281 @@
+"""Pins of the CPU oracle (no GPU).
+
+The oracle is the checker of the HIP library, so it is itself checked here
+against what the reference ships and what the mathematics fixes:
+  - the reference's QA temperatures for its shipped decks
+    (tealeaf_2d/field_summary.f90:107, tealeaf_3d/field_summary.f90:115);
+  - the stencil operator against an independently assembled matrix
+    I + sum over interior faces of k (e_i - e_j)(e_i - e_j)^T, which also
+    pins the reflective (zero-flux) boundary;
+  - converged solves against a sparse direct solve of the same system;
+  - the eigenvalue routine against LAPACK (numpy.linalg.eigvalsh) and the
+    Chebyshev coefficients against their closed form in Chebyshev
+    polynomials;
+  - conservation: A has unit column sums, so sum(u) = sum(u0) - sum(r);
+  - the deck reader's single-precision defaults.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import read_deck
+
+FIELD_DENSITY, FIELD_ENERGY0, FIELD_ENERGY1, FIELD_U, FIELD_P, FIELD_SD = 1, 2, 3, 4, 5, 6
+
+
+def test_qa_tp11_2d(oracle):
+    r = oracle.run_deck(read_deck("tp11_2d.in"), dim=2, reduce_parts=8, threads=8)
+    t = r["final"]["temp"]
+    qa = abs(100.0 * (t / 103.46970921076068) - 100.0)
+    assert qa < 0.001  # the reference's own pass bar
+    assert abs(t / 103.46970921076068 - 1.0) < 1e-8
+    assert r["initial"]["vol"] == pytest.approx(100.0, rel=1e-13)
+
+
+def test_qa_tp11_3d(oracle):
+    r = oracle.run_deck(read_deck("tp11_3d.in"), dim=3, reduce_parts=8, threads=8)
+    t = r["final"]["temp"]
+    assert abs(t / 58.442896702081427 - 1.0) < 1e-13
+    # the deck leaves initial_timestep at its single-precision default 0.1
+    assert r["dtinit"] == float(np.float32(0.1))
+
+
+def test_default_eps_is_single_precision(oracle):
+    deck = read_deck("tp11_2d.in").replace("tl_eps=1.0e-15\n", "").replace("end_step=20", "end_step=1")
+    r = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
+    assert r["eps"] == float(np.float32(1.0e-10))
+
+
+def test_reduction_order_sensitivity(oracle):
+    """Kokkos' sum order depends on the thread-pool size; the per-step
+    totals move far less than the 1e-11 parity bar."""
+    import decks
+    deck = decks.two_rect_2d(96, "ppcg", steps=2, presteps=20)
+    a = oracle.run_deck(deck, dim=2, reduce_parts=1, threads=8)
+    b = oracle.run_deck(deck, dim=2, reduce_parts=7, threads=8)
+    for sa, sb in zip(a["steps"], b["steps"]):
+        assert abs(sa["iters"] - sb["iters"]) <= 1
+        assert abs(sa["temp"] - sb["temp"]) <= 1e-12 * abs(sb["temp"])
+
+
+# ---------------------------------------------------------------------------
+# operator and solves
+
+def _chunk(hip, oracle, dim, n):
+    return hip.ExtChunk(oracle.lib(dim), dim, *n)
+
+
+def _interior_index(ch):
+    if ch.dim == 3:
+        sl = (slice(2, -2), slice(2, -2), slice(2, -2))
+    else:
+        sl = (slice(None), slice(2, -2), slice(2, -2))
+    idx = np.arange(int(np.prod(ch.shape))).reshape(ch.shape)[sl]
+    return sl, idx
+
+
+def assemble(ch, kx, ky, kz=None):
+    """I + sum over interior faces of k (e_i - e_j)(e_i - e_j)^T on the
+    interior cells (faces on the mesh boundary carry no flux)."""
+    sl, idx = _interior_index(ch)
+    cells = idx.ravel()
+    pos = {c: i for i, c in enumerate(cells)}
+    n = len(cells)
+    rows, cols, vals = list(range(n)), list(range(n)), [1.0] * n
+    X = ch.X
+    P = ch.X * ch.Y
+    flat = {"x": (1, kx), "y": (X, ky)}
+    if kz is not None:
+        flat["z"] = (P, kz)
+    kflat = {a: k.ravel() for a, (s, k) in flat.items()}
+    for c in cells:
+        for a, (stride, _) in flat.items():
+            nb = c - stride  # face between nb and c carries k[c]
+            if nb in pos:
+                k = kflat[a][c]
+                i, j = pos[c], pos[nb]
+                rows += [i, j, i, j]
+                cols += [i, j, j, i]
+                vals += [k, k, -k, -k]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)), sl
+
+
+@pytest.mark.parametrize("dim,n", [(2, (6, 5, 1)), (3, (4, 3, 5))])
+def test_stencil_matches_assembled_operator(hip, oracle, dim, n):
+    rng = np.random.default_rng(dim)
+    ch = _chunk(hip, oracle, dim, n)
+    ks = {name: rng.uniform(0.2, 3.0, size=ch.shape) for name in ["kx", "ky"] + (["kz"] if dim == 3 else [])}
+    for name, a in ks.items():
+        ch.set(name, a)
+    A, sl = assemble(ch, ks["kx"], ks["ky"], ks.get("kz"))
+    m = A.shape[0]
+    cols = []
+    for j in range(m):
+        p = np.zeros(ch.shape)
+        p[sl].flat[j] = 1.0
+        ch.set("p", p)
+        ch.update_halo([FIELD_P], 1)
+        ch.cg_calc_w()
+        cols.append(ch.get("w")[sl].ravel().copy())
+    M = np.array(cols).T
+    D = A.toarray()
+    assert np.allclose(M, D, rtol=1e-14, atol=1e-14)
+    assert np.allclose(M, M.T, rtol=1e-14, atol=1e-14)
+    assert np.allclose(M.sum(axis=0), 1.0, rtol=0, atol=1e-13)
+    ch.close()
+
+
+def _solve_reference(ch, dim):
+    """Dense/sparse solution of A u = u0 with the oracle's own coefficients."""
+    kx, ky = ch.get("kx"), ch.get("ky")
+    kz = ch.get("kz") if dim == 3 else None
+    A, sl = assemble(ch, kx, ky, kz)
+    u0 = ch.get("u0")[sl].ravel()
+    return spla.spsolve(A.tocsc(), u0), sl
+
+
+@pytest.mark.parametrize("solver", ["cg", "jacobi", "chebyshev", "ppcg"])
+def test_solvers_converge_to_direct_solve(hip, oracle, solver):
+    """One timestep to a tight eps: the final energy equals the sparse direct
+    solve of the same system (u = A^-1 u0, energy = u / density)."""
+    import decks
+    n = 40
+    deck = decks.two_rect_2d(n, solver, steps=1, eps=1e-22, presteps=20, max_iters=40000,
+                             extra=("tl_check_result",))
+    rep = oracle.run_deck(deck, dim=2, reduce_parts=4, threads=4)
+    # rebuild the same system through the ABI
+    ch = _chunk(hip, oracle, 2, (n, n, 1))
+    dx = 10.0 / n
+    s = dx / 100.0
+    ch.initialise_chunk(0.0, 0.0, dx, dx)
+    ch.generate_chunk(density=[100.0, 0.1, 0.1], energy=[1e-4, 25.0, 0.1], xmin=[0, s, 1 + s],
+                      xmax=[0, 1 - s, 6 - s], ymin=[0, s, 1 + s], ymax=[0, 2 - s, 6 - s], radius=[0, 0, 0],
+                      geometry=[0, 1, 1])
+    ch.update_halo([FIELD_DENSITY, FIELD_ENERGY0, FIELD_ENERGY1], 2)
+    ch.set_field_kernel()
+    ch.update_halo([FIELD_ENERGY1, FIELD_DENSITY], 2)
+    ch.cg_solver_init(1, False, 0.004)
+    ch.copy_u()
+    u, sl = _solve_reference(ch, 2)
+    vol = ch.get("volume")[sl].ravel()
+    rho = ch.get("density")[sl].ravel()
+    temp = float(np.sum(vol * rho * u))
+    ch.close()
+    step = rep["steps"][0]
+    assert step["exact_error"] < 1e-18
+    assert abs(step["temp"] - temp) <= 1e-11 * abs(temp)
+
+
+def test_conservation_identity(hip, oracle):
+    """sum(A u) = sum(u) (unit column sums), so after any number of CG
+    iterations sum(u) + sum(r) = sum(u0) up to rounding."""
+    ch = _chunk(hip, oracle, 2, (30, 26, 1))
+    dx = 10.0 / 30
+    ch.initialise_chunk(0.0, 0.0, dx, 10.0 / 26)
+    ch.generate_chunk(density=[100.0, 0.1], energy=[1e-4, 25.0], xmin=[0, 0.01], xmax=[0, 3.0],
+                      ymin=[0, 0.01], ymax=[0, 4.0], radius=[0, 0], geometry=[0, 1])
+    ch.update_halo([FIELD_DENSITY, FIELD_ENERGY0, FIELD_ENERGY1], 2)
+    ch.set_field_kernel()
+    rro = ch.cg_solver_init(1, False, 0.004)
+    ch.update_halo([FIELD_U, FIELD_P], 1)
+    ch.copy_u()
+    sl = (slice(None), slice(2, -2), slice(2, -2))
+    s0 = ch.get("u0")[sl].sum()
+    for _ in range(7):
+        pw = ch.cg_calc_w()
+        rrn = ch.cg_calc_ur(rro / pw)
+        ch.cg_calc_p(rrn / rro)
+        ch.update_halo([FIELD_U, FIELD_P], 1)
+        rro = rrn
+        su = ch.get("u")[sl].sum()
+        sr = ch.get("r")[sl].sum()
+        assert abs(su + sr - s0) <= 1e-12 * abs(s0)
+    ch.close()
+
+
+# ---------------------------------------------------------------------------
+# eigenvalues and Chebyshev coefficients
+
+def _lanczos_tridiagonal(alphas, betas):
+    m = len(alphas)
+    T = np.zeros((m, m))
+    for k in range(m):
+        T[k, k] = 1.0 / alphas[k] + (betas[k - 1] / alphas[k - 1] if k > 0 else 0.0)
+        if k + 1 < m:
+            T[k, k + 1] = T[k + 1, k] = math.sqrt(betas[k]) / alphas[k]
+    return T
+
+
+@pytest.mark.parametrize("m", [2, 5, 30, 97])
+def test_eigenvalues_match_lapack(oracle, m):
+    rng = np.random.default_rng(m)
+    alphas = list(rng.uniform(0.01, 2.0, m))
+    betas = list(rng.uniform(0.05, 0.95, m))
+    info, emin, emax = oracle.calc_eigenvalues(alphas, betas)
+    ev = np.linalg.eigvalsh(_lanczos_tridiagonal(alphas, betas))
+    assert info == 0
+    assert emin == pytest.approx(ev[0], rel=1e-11, abs=1e-13)
+    assert emax == pytest.approx(ev[-1], rel=1e-12)
+
+
+def test_eigenvalues_of_cg_bracket_operator_spectrum(hip, oracle):
+    """CG on a small SPD system: the Lanczos Ritz values from its alphas and
+    betas lie inside the operator's spectrum, and the extreme ones converge
+    to its ends."""
+    rng = np.random.default_rng(5)
+    ch = _chunk(hip, oracle, 2, (9, 8, 1))
+    for name in ("kx", "ky"):
+        ch.set(name, rng.uniform(0.5, 4.0, size=ch.shape))
+    A, sl = assemble(ch, ch.get("kx"), ch.get("ky"))
+    b = rng.uniform(0.5, 1.5, A.shape[0])
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    rro = r @ r
+    alphas, betas = [], []
+    D = A.toarray()
+    for _ in range(A.shape[0]):
+        w = D @ p
+        a = rro / (p @ w)
+        x += a * p
+        r -= a * w
+        rrn = r @ r
+        bt = rrn / rro
+        alphas.append(a)
+        betas.append(bt)
+        p = r + bt * p
+        rro = rrn
+        if rrn < 1e-28:
+            break
+    info, emin, emax = oracle.calc_eigenvalues(alphas, betas)
+    ev = np.linalg.eigvalsh(D)
+    assert info == 0
+    assert ev[0] - 1e-9 <= emin and emax <= ev[-1] + 1e-9
+    assert emax == pytest.approx(ev[-1], rel=1e-6)
+    ch.close()
+
+
+def test_negative_eigenvalue_is_an_error(oracle):
+    info, _, _ = oracle.calc_eigenvalues([-1.0, 0.5, 0.7], [0.3, 0.2, 0.1])
+    assert info == 1
+
+
+@pytest.mark.parametrize("eigmin,eigmax,n", [(0.5, 40.0, 25), (1.0, 3.0, 10), (0.01, 9000.0, 60)])
+def test_chebyshev_coefficients_closed_form(oracle, eigmin, eigmax, n):
+    """rho_k = 1/(2 sigma - rho_{k-1}), rho_0 = 1/sigma has the closed form
+    rho_k = T_k(sigma) / T_{k+1}(sigma), so alpha_k = T_{k-1}/T_{k+1} and
+    beta_k = 2 T_k / (delta T_{k+1})."""
+    a, b, theta = oracle.calc_ch_coefs(eigmin, eigmax, n)
+    theta_x = (eigmax + eigmin) / 2
+    delta = (eigmax - eigmin) / 2
+    sigma = theta_x / delta
+    T = [1.0, sigma]
+    for _ in range(n + 1):
+        T.append(2 * sigma * T[-1] - T[-2])
+    assert theta == theta_x
+    for k in range(1, n + 1):
+        assert a[k - 1] == pytest.approx(T[k - 1] / T[k + 1], rel=1e-10)
+        assert b[k - 1] == pytest.approx(2 * T[k] / (delta * T[k + 1]), rel=1e-10)
