@@ -114,6 +114,13 @@ const char* tl_driver_error(void);
  * "kernels" table: launches, summed milliseconds, algorithmic bytes/cell. */
 void tl_set_kernel_timing(int from_step);
 
+/* The fused PPCG outer iteration the driver uses (2D, no preconditioner),
+ * on a chunk made by ext_init_: with the given alpha, CGCalcUr + PPCGInitSd
+ * + nsteps inner steps (alphas/betas[0..nsteps), theta) + ||r||^2 in one
+ * temporal-blocked sweep.  Returns 0, or -1 with tl_last_error() set. */
+int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
+                        int nsteps, double* rrn);
+
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);
 const char* tl_last_error(void);

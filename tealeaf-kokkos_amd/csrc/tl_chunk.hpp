@@ -49,8 +49,8 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 
 // Kernels timed by the optional per-launch event timer.
 enum KernelId {
-    K_CG_INIT = 0, K_CG_W, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_JACOBI,
-    K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_COUNT
+    K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
+    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -79,6 +79,19 @@ struct KernelTimer {
 void set_kernel_timing_from(int step);
 int kernel_timing_from();
 
+// Runs STMT between two timer events when the chunk's timer is on.
+#define TL_TIMED(c, kid, STMT)                               \
+    do {                                                     \
+        if ((c).timer.on) {                                  \
+            ::tl::KernelTimer::Pending p_;                   \
+            (c).timer.begin((kid), (c).stream, p_);          \
+            STMT;                                            \
+            (c).timer.end(p_, (c).stream);                   \
+        } else {                                             \
+            STMT;                                            \
+        }                                                    \
+    } while (0)
+
 struct Chunk {
     Geom g{};
     int rank = 0;
@@ -89,17 +102,26 @@ struct Chunk {
     double* f[F_COUNT] = {};
     double* u_alt = nullptr;   // ping-pong partner of u (Chebyshev)
     double* sd_alt = nullptr;  // ping-pong partner of sd (PPCG inner steps)
+    double* p_alt = nullptr;   // ping-pong partner of p (CGCalcP folded into CGCalcW)
+    double* r_alt = nullptr;   // ping-pong partner of r (temporal-blocked PPCG)
 
     // Mesh coordinates (device, reference indexing).
     double *vertexX = nullptr, *vertexY = nullptr, *vertexZ = nullptr;
     double *cellX = nullptr, *cellY = nullptr, *cellZ = nullptr;
 
-    // Chebyshev / PPCG coefficients, copied from the caller at init.
+    // Chebyshev / PPCG coefficients, copied from the caller at init; the
+    // device copy holds, per block of up to 12 steps starting at step k,
+    // a[k..k+n) followed by b[k..k+n) at offset 2k.
     std::vector<double> alphas, betas;
+    double* d_coef = nullptr;
+    int d_coef_cap = 0;
 
-    // Reduction scratch.
+    // Reduction scratch and device-resident solver scalars.
     double* partials = nullptr;
-    unsigned* ticket = nullptr;
+    DevScalars* ds = nullptr;
+    double* hist_a = nullptr;  // CG alphas / betas by iteration (device)
+    double* hist_b = nullptr;
+    int hist_cap = 0;
     double* d_result = nullptr;           // [S_COUNT * 4] device
     volatile double* h_result = nullptr;  // [S_COUNT * 4] host-mapped
     double* h_result_dev = nullptr;       // device alias of h_result
@@ -112,8 +134,9 @@ struct Chunk {
     Chunk(const Chunk&) = delete;
     Chunk& operator=(const Chunk&) = delete;
 
-    Reduce reducer(Slot s) const;
+    Reduce reducer() const { return Reduce{partials}; }
     double result(Slot s, int k = 0) const;  // waits for the stream
+    void ensure_history(int n);
     void sync() const;
 
     // Field access in the reference's flat layout (X*Y*Z doubles).
@@ -136,6 +159,26 @@ double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt);
 double op_cg_calc_w(Chunk& c);
 double op_cg_calc_ur(Chunk& c, double alpha);
 void op_cg_calc_p(Chunk& c, double beta);
+
+// Device-resident forms for the driver: the scalars (rro, pw, alpha, rrn,
+// beta, error) stay in Chunk::ds and are updated by the reductions' second
+// phase exactly as tea_solve.f90 updates them; nothing waits for the host.
+// calc_w with `fold` first applies the pending p = beta p + r (or z).
+// calc_ur records alpha/beta of CG iteration `n` (1-based) when n > 0.
+void op_cg_calc_w_dev(Chunk& c, bool fold);
+void op_cg_calc_ur_dev(Chunk& c, int n);
+void op_cg_calc_p_dev(Chunk& c);
+void op_ppcg_calc_ur_dev(Chunk& c);  // u += alpha p, r -= alpha w; its r.r is not used
+void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in the PPCG loop
+double op_error(Chunk& c);        // waits and returns ds->error
+void op_finish_rrn(Chunk& c, long nblocks);  // second phase of an rrn reduction, PPCG recurrences
+
+// Temporal-blocked PPCG outer iteration (tl_ppcg_tb.hip): after CGCalcW,
+// CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep.
+// Returns the number of rrn partials; op_finish_rrn closes the reduction.
+bool ppcg_tb_supported(const Chunk& c, int inner_steps);
+long op_ppcg_outer_tb(Chunk& c, int inner_steps);
+void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b);  // 1..n
 
 // Chebyshev, PPCG and Jacobi come in two forms.  The plain form keeps the
 // reference's memory effects exactly (scratch w written, updates in place).

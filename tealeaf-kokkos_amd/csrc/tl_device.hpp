@@ -109,20 +109,25 @@ __device__ inline void strip_advance(Strip& s)
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic single-pass grid reduction of NV doubles.
+// Deterministic two-phase reductions.
 //
-// Every lane sums its own cells in strip order; the wave folds with a fixed
-// xor butterfly; lane 0 of each wave stores to LDS; wave partials are added
-// in wave order; the block partial goes to `partials`.  The last block to
-// arrive (agent-scope atomic ticket) adds all block partials in block order
-// with the same pattern, writes the totals to device memory and to the
-// host-mapped result slot, and re-arms the ticket.  The order depends only on
-// the mesh and the launch geometry, so results are bitwise reproducible.
+// Phase 1 (inside each sweep): every lane sums its own cells in strip order,
+// the wave folds with a fixed xor butterfly, lane 0 of each wave stores to
+// LDS, the wave partials are added in wave order and the block's total goes
+// to partials[block].  Phase 2 (k_finish, one workgroup) adds the block
+// partials in block order with the same pattern and applies the scalar
+// epilogue.  No fences or atomics are needed inside the sweep: the kernel
+// boundary orders the two phases.  The summation order depends only on the
+// mesh and the launch geometry, so results are bitwise reproducible.
 struct Reduce {
-    double* partials;   // [nblocks * NV]
-    unsigned* ticket;   // armed at 0
-    double* dout;       // device copy of the totals [NV]
-    double* hout;       // host-mapped copy of the totals [NV]
+    double* partials;  // [nblocks * NV]
+};
+
+// Solver scalars kept on the device so that consecutive kernels can use the
+// recurrences without a host round trip (same IEEE divisions as the host).
+struct DevScalars {
+    double rro, pw, alpha, rrn, beta, error;
+    double pad[2];
 };
 
 template <int NV>
@@ -135,10 +140,13 @@ __device__ inline void wave_fold(double (&v)[NV])
     }
 }
 
-template <int NV>
-__device__ inline void block_fold(double (&v)[NV], double (*lds)[WAVES_PER_BLOCK])
+// Folds the block's lanes into thread 0 (v valid on thread 0 only).
+template <int NV, int NWAVES = WAVES_PER_BLOCK>
+__device__ inline void block_fold(double (&v)[NV])
 {
+    __shared__ double lds[NV][NWAVES];
     wave_fold<NV>(v);
+    if (NWAVES == 1) return;
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
     if (lane == 0) {
 #pragma unroll
@@ -150,45 +158,19 @@ __device__ inline void block_fold(double (&v)[NV], double (*lds)[WAVES_PER_BLOCK
         for (int k = 0; k < NV; ++k) {
             double s = lds[k][0];
 #pragma unroll
-            for (int w = 1; w < WAVES_PER_BLOCK; ++w) s += lds[k][w];
+            for (int w = 1; w < NWAVES; ++w) s += lds[k][w];
             v[k] = s;
         }
     }
-    __syncthreads();
 }
 
-template <int NV>
-__device__ inline void grid_reduce(double (&v)[NV], const Reduce& r, long nblocks)
+template <int NV, int NWAVES = WAVES_PER_BLOCK>
+__device__ inline void block_partial(double (&v)[NV], const Reduce& r)
 {
-    __shared__ double lds[NV][WAVES_PER_BLOCK];
-    __shared__ int last;
-    block_fold<NV>(v, lds);
+    block_fold<NV, NWAVES>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) r.partials[(long)blockIdx.x * NV + k] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(r.ticket, 1u);
-        last = (t == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (long b = threadIdx.x; b < nblocks; b += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += r.partials[b * NV + k];
-    }
-    block_fold<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            r.dout[k] = acc[k];
-            r.hout[k] = acc[k];
-        }
-        *r.ticket = 0u;
-        __threadfence_system();
     }
 }
 

@@ -356,7 +356,12 @@ void halo(Chunk& c, std::initializer_list<int> which, int depth)
     op_update_halo(c, kAllExternal, fields, depth);
 }
 
-// tea_leaf (tea_solve.f90:36-605) for one chunk.
+// tea_leaf (tea_solve.f90:36-605) for one chunk.  The CG and PPCG scalars
+// (rro, pw, alpha, rrn, beta, error) live on the device and follow the
+// Fortran recurrences there; the host reads `error` once per iteration for
+// the convergence and switch tests.  CGCalcP is deferred into the next
+// CGCalcW (`p_pending`) and applied on its own only where r changes first
+// (the residual recomputed at the PPCG switch).
 StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
 {
     using clock = std::chrono::steady_clock;
@@ -365,15 +370,17 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     const int max_iters = d.max_iters;
     double error = 1e10, exact_error = 0.0;
     int cheby_calc_steps = 0, cg_calc_steps = 0;
-    double rro = 0.0, pw = 0.0, rrn = 0.0;
     double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
     int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
-    std::vector<double> cg_alphas(max_iters + 3, 0.0), cg_betas(max_iters + 3, 0.0);
+    bool p_pending = false;
+    std::vector<double> cg_alphas, cg_betas;
     std::vector<double> ch_alphas(max_iters + 3, 0.0), ch_betas(max_iters + 3, 0.0);
+    const bool want_history = d.use_cheby || d.use_ppcg;
 
     halo(c, {FIELD_ENERGY1, FIELD_DENSITY}, 2);
     if (d.use_cg || d.use_cheby || d.use_ppcg) {
-        rro = op_cg_init(c, d.coefficient, d.preconditioner, dt);
+        (void)op_cg_init(c, d.coefficient, d.preconditioner, dt);
+        if (want_history) c.ensure_history(max_iters + 1);
     } else if (d.use_jacobi) {
         op_jacobi_init(c, d.coefficient, dt);
     }
@@ -390,7 +397,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
         if ((d.use_cheby || d.use_ppcg) && ch_switch_check) {
             if (cheby_calc_steps == 0) {
                 max_cheby_iters = max_iters - n + 2;
-                rro = error;
+                op_read_history(c, n - 1, cg_alphas, cg_betas);
                 if (calc_eigenvalues(cg_alphas, cg_betas, n - 1, eigmin, eigmax))
                     throw Error("tea_leaf: Error in calculating eigenvalues");
                 eigmin = eigmin * (double)0.95f;
@@ -401,7 +408,9 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             }
             if (d.use_cheby) {
                 if (cheby_calc_steps == 0) {
-                    // tea_leaf_cheby_first_step (tea_solve.f90:658-750)
+                    // tea_leaf_cheby_first_step (tea_solve.f90:658-750); the
+                    // pending p update is dead: ChebyInit overwrites p.
+                    p_pending = false;
                     const double bb = op_calc_2norm(c, false);
                     op_cheby_init(c, &ch_alphas[1], &ch_betas[1], max_cheby_iters, theta, d.preconditioner, true);
                     op_cheby_iterate(c, max_cheby_iters, true);
@@ -419,33 +428,35 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 if (cheby_calc_steps == 0) {
                     cheby_calc_steps = 1;
                     op_ppcg_init(c, &ch_alphas[1], &ch_betas[1], max_cheby_iters, theta);
+                    if (p_pending) {
+                        op_cg_calc_p_dev(c);
+                        p_pending = false;
+                    }
                     op_calc_residual(c);
                     switch_step = n;
                 }
-                pw = op_cg_calc_w(c);
-                const double alpha = rro / pw;
-                (void)op_cg_calc_ur(c, alpha);
-                // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
-                op_ppcg_init_sd(c);
-                for (int step = 1; step <= d.ppcg_inner_steps; ++step) op_ppcg_inner(c, step, true);
-                rrn = op_calc_2norm(c, true);
-                const double beta = rrn / rro;
-                op_cg_calc_p(c, beta);
-                error = rrn;
-                rro = rrn;
+                // rro on the device is the last rrn (= error), as at tea_solve.f90:187,304
+                op_cg_calc_w_dev(c, p_pending);
+                if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
+                    // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
+                    TL_TIMED(c, K_PPCG_OUTER, op_finish_rrn(c, op_ppcg_outer_tb(c, d.ppcg_inner_steps)));
+                } else {
+                    op_ppcg_calc_ur_dev(c);
+                    // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
+                    op_ppcg_init_sd(c);
+                    for (int step = 1; step <= d.ppcg_inner_steps; ++step) op_ppcg_inner(c, step, true);
+                    op_2norm_rrn_dev(c);
+                }
+                p_pending = true;
+                error = op_error(c);
             }
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
             ++cg_calc_steps;
-            pw = op_cg_calc_w(c);
-            const double alpha = rro / pw;
-            if (d.use_cheby || d.use_ppcg) cg_alphas[n] = alpha;
-            rrn = op_cg_calc_ur(c, alpha);
-            const double beta = rrn / rro;
-            if (d.use_cheby || d.use_ppcg) cg_betas[n] = beta;
-            op_cg_calc_p(c, beta);
-            error = rrn;
-            rro = rrn;
+            op_cg_calc_w_dev(c, p_pending);
+            op_cg_calc_ur_dev(c, want_history ? n : 0);
+            p_pending = true;
+            error = op_error(c);
         } else if (d.use_jacobi) {
             error = op_jacobi_solve(c, true);
             if (n % 50 == 0) {

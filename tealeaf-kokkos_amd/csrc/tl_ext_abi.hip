@@ -90,6 +90,29 @@ int tl_set_field(const int* chunk, int field, const double* in)
     return rc;
 }
 
+// Fused forms used by the C++ driver, exposed for parity tests on a chunk
+// created with ext_init_.  One PPCG outer iteration after CGCalcW with the
+// given alpha: CGCalcUr + PPCGInitSd + nsteps inner steps (coefficients
+// alphas/betas[0..nsteps), theta) + ||r||^2, as one temporal-blocked sweep.
+// Returns 0, or -1 (tl_last_error) when the fused form does not apply.
+int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
+                        int nsteps, double* rrn)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = chunk_at(chunk);
+        DevScalars s{};
+        s.rro = 1.0;
+        s.alpha = alpha;
+        TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+        op_ppcg_init(*c, alphas, betas, nsteps, theta);
+        op_finish_rrn(*c, op_ppcg_outer_tb(*c, nsteps));
+        *rrn = op_error(*c);
+        rc = 0;
+    });
+    return rc;
+}
+
 // ---- ext_chunk.cpp:18-33 --------------------------------------------------
 
 #if TL_DIM == 3

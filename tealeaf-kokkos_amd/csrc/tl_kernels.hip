@@ -173,7 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_field_summary(Geom g, const double* _
         v[2] += cellMass * energy0[s.i];
         v[3] += cellMass * u[s.i];
     }
-    grid_reduce<4>(v, red, g.nblocks);
+    block_partial<4>(v, red);
 }
 
 // ----------------------------------------------------------------------------
@@ -342,31 +342,61 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_rro(Geom g, KPtr k, int pre, 
         p[i] = pv;
         v[0] += rv * pv;
     }
-    grid_reduce<1>(v, red, g.nblocks);
+    block_partial<1>(v, red);
 }
 
-// CGCalcW (:178-189)
+// CGCalcW (:178-189): w = A p, partial p.w.  With FOLD the pending
+// CGCalcP of the previous iteration (:301-310, p = beta p + r or z) is
+// applied on the fly: every stencil point of p is rebuilt from the old p and
+// r (z) with the same expression, the new p goes to p_out (a second buffer,
+// since neighbours still read the old one) and A is applied to it.
+template <bool FOLD>
 __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const double* __restrict__ p,
-                                                     double* __restrict__ w, Reduce red)
+                                                     const double* __restrict__ q, const DevScalars* __restrict__ ds,
+                                                     double* __restrict__ p_out, double* __restrict__ w, Reduce red)
 {
     Strip s = make_strip(g);
     double v[1] = {0.0};
+    const double beta = FOLD ? ds->beta : 0.0;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) {
-        const double sm = smvp(g, p, k, s.i, s.kk, s.jj, s.ii);
-        w[s.i] = sm;
-        v[0] += sm * p[s.i];
+        const long i = s.i;
+        double sm, pc;
+        if (FOLD) {
+            const long iw = s.kk > HALO ? i - 1 : i, ie = s.kk < g.nx + 1 ? i + 1 : i;
+            const long is = s.jj > HALO ? i - g.sx : i, in = s.jj < g.ny + 1 ? i + g.sx : i;
+            pc = beta * p[i] + q[i];
+            const double pw_ = beta * p[iw] + q[iw], pe = beta * p[ie] + q[ie];
+            const double ps = beta * p[is] + q[is], pn = beta * p[in] + q[in];
+            const double kxc = k.kx[i], kxe = k.kx[i + 1], kyc = k.ky[i], kyn = k.ky[i + g.sx];
+#if TL_DIM == 3
+            const long ib = s.ii > HALO ? i - g.sy : i, iff = s.ii < g.nz + 1 ? i + g.sy : i;
+            const double pb = beta * p[ib] + q[ib], pf = beta * p[iff] + q[iff];
+            const double kzc = k.kz[i], kzf = k.kz[i + g.sy];
+            sm = (1.0 + (kxe + kxc) + (kyn + kyc) + (kzf + kzc)) * pc - (kxe * pe + kxc * pw_) - (kyn * pn + kyc * ps)
+               - (kzf * pf + kzc * pb);
+#else
+            sm = (1.0 + (kxe + kxc) + (kyn + kyc)) * pc - (kxe * pe + kxc * pw_) - (kyn * pn + kyc * ps);
+#endif
+            p_out[i] = pc;
+        } else {
+            pc = p[i];
+            sm = smvp(g, p, k, i, s.kk, s.jj, s.ii);
+        }
+        w[i] = sm;
+        v[0] += sm * pc;
     }
-    grid_reduce<1>(v, red, g.nblocks);
+    block_partial<1>(v, red);
 }
 
-// CGCalcUr (:244-264)
-__global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha, int pre, double* __restrict__ u,
-                                                      double* __restrict__ r, const double* __restrict__ p,
-                                                      const double* __restrict__ w, const double* __restrict__ mi,
-                                                      double* __restrict__ z, Reduce red)
+// CGCalcUr (:244-264); alpha from the device scalars when ds is given.
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, const DevScalars* __restrict__ ds,
+                                                      int pre, double* __restrict__ u, double* __restrict__ r,
+                                                      const double* __restrict__ p, const double* __restrict__ w,
+                                                      const double* __restrict__ mi, double* __restrict__ z, Reduce red)
 {
     Strip s = make_strip(g);
     double v[1] = {0.0};
+    const double alpha = ds ? ds->alpha : alpha_v;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) {
         const long i = s.i;
         u[i] = u[i] + alpha * p[i];
@@ -380,17 +410,75 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha, int 
             v[0] += rv * rv;
         }
     }
-    grid_reduce<1>(v, red, g.nblocks);
+    block_partial<1>(v, red);
 }
 
-// CGCalcP (:301-310)
-__global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta, int pre, double* __restrict__ p,
-                                                     const double* __restrict__ r, const double* __restrict__ z)
+// CGCalcP (:301-310); beta from the device scalars when ds is given.
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, const DevScalars* __restrict__ ds,
+                                                     double* __restrict__ p, const double* __restrict__ q)
 {
     Strip s = make_strip(g);
+    const double beta = ds ? ds->beta : beta_v;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) {
         const long i = s.i;
-        p[i] = beta * p[i] + (pre ? z[i] : r[i]);
+        p[i] = beta * p[i] + q[i];
+    }
+}
+
+// Second phase of every reduction: one workgroup adds the block partials in
+// block order and applies the scalar recurrence of tea_solve.f90 that
+// consumes the total, so the next kernel can read it from `ds`.
+enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN };
+
+template <int NV>
+__global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
+                                                  DevScalars* __restrict__ ds, double* __restrict__ dout,
+                                                  double* __restrict__ hout, double* __restrict__ hist_a,
+                                                  double* __restrict__ hist_b, int hist_n)
+{
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
+    for (long b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += partials[b * NV + k];
+    }
+    block_fold<NV>(v);
+    if (threadIdx.x != 0) return;
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = v[k];
+    switch (ep) {
+    case EP_RRO:  // rro of ext_cg_solver_init
+        ds->rro = v[0];
+        break;
+    case EP_PW: {  // alpha = rro / pw (tea_solve.f90:322,397)
+        const double alpha = ds->rro / v[0];
+        ds->pw = v[0];
+        ds->alpha = alpha;
+        out[1] = alpha;
+        break;
+    }
+    case EP_RRN: {  // beta = rrn / rro; error = rrn; rro = rrn (:358,:373-374,:417,:433-434)
+        const double beta = v[0] / ds->rro;
+        ds->rrn = v[0];
+        ds->beta = beta;
+        if (hist_n >= 0) {
+            hist_a[hist_n] = ds->alpha;
+            hist_b[hist_n] = beta;
+        }
+        ds->rro = v[0];
+        ds->error = v[0];
+        out[1] = beta;
+        break;
+    }
+    default:
+        break;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        dout[k] = out[k];
+        hout[k] = out[k];
     }
 }
 
@@ -546,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_solve(Geom g, KPtr k, const do
         next[i] = un;
         v[0] += fabs(un - rc);
     }
-    grid_reduce<1>(v, red, g.nblocks);
+    block_partial<1>(v, red);
 }
 
 // ----------------------------------------------------------------------------
@@ -576,7 +664,7 @@ __global__ __launch_bounds__(BLOCK) void k_2norm(Geom g, const double* __restric
     Strip s = make_strip(g);
     double v[1] = {0.0};
     for (int n = 0; n < s.count; ++n, strip_advance(s)) v[0] += a[s.i] * a[s.i];
-    grid_reduce<1>(v, red, g.nblocks);
+    block_partial<1>(v, red);
 }
 
 // Finalise (:104-111)
@@ -608,18 +696,7 @@ inline void check_launch()
         check_launch();                                                                       \
     } while (0)
 
-// Runs STMT between two timer events when the chunk's timer is on.
-#define TIMED(c, kid, STMT)                                  \
-    do {                                                     \
-        if ((c).timer.on) {                                  \
-            KernelTimer::Pending p_;                         \
-            (c).timer.begin((kid), (c).stream, p_);          \
-            STMT;                                            \
-            (c).timer.end(p_, (c).stream);                   \
-        } else {                                             \
-            STMT;                                            \
-        }                                                    \
-    } while (0)
+#define TIMED TL_TIMED
 
 #define FLAT_LAUNCH(kernel, c, ...)                                                               \
     do {                                                                                          \
@@ -627,6 +704,15 @@ inline void check_launch()
                            __VA_ARGS__);                                                          \
         check_launch();                                                                           \
     } while (0)
+
+// Second phase of a reduction whose first phase wrote `nblocks` partials.
+template <int NV>
+void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
+{
+    hipLaunchKernelGGL(k_finish<NV>, dim3(1), dim3(BLOCK), 0, c.stream, c.partials, nblocks, ep, c.ds,
+                       c.d_result + 4 * slot, c.h_result_dev + 4 * slot, c.hist_a, c.hist_b, hist_n);
+    check_launch();
+}
 
 }  // namespace
 
@@ -639,9 +725,10 @@ int kernel_timing_from() { return g_timing_from; }
 
 const char* kernel_name(int k)
 {
-    static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_ur", "cg_calc_p", "cheby_iterate",
-                                         "ppcg_init_sd", "ppcg_inner", "jacobi_solve", "calc_residual",
-                                         "calc_2norm", "copy_u", "finalise", "field_summary", "update_halo"};
+    static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_p_w", "cg_calc_ur", "cg_calc_p",
+                                         "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
+                                         "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
+                                         "field_summary", "update_halo"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -650,6 +737,8 @@ double kernel_bytes_per_cell(int k)
     const double kb = 8.0 * TL_DIM;  // the kx, ky[, kz] coefficient arrays
     switch (k) {
     case K_CG_W: return 16.0 + kb;         // p, coefficients -> w
+    case K_CG_WP: return 32.0 + kb;        // p, r, coefficients -> p', w
+    case K_PPCG_OUTER: return 48.0 + kb;   // u, p, r, w, coefficients -> u, r
     case K_CG_UR: return 48.0;             // u, p, r, w -> u, r
     case K_CG_P: return 24.0;              // p, r -> p
     case K_CHEBY: return 48.0 + kb;        // u, u0, p, coefficients -> r, p, u'
@@ -780,15 +869,19 @@ Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
     TL_HIP(hipMemsetAsync(u_alt, 0, bytes, stream));
     TL_HIP(hipMalloc(&sd_alt, bytes));
     TL_HIP(hipMemsetAsync(sd_alt, 0, bytes, stream));
+    TL_HIP(hipMalloc(&p_alt, bytes));
+    TL_HIP(hipMemsetAsync(p_alt, 0, bytes, stream));
+    TL_HIP(hipMalloc(&r_alt, bytes));
+    TL_HIP(hipMemsetAsync(r_alt, 0, bytes, stream));
     TL_HIP(hipMalloc(&vertexX, sizeof(double) * (g.X + 1)));
     TL_HIP(hipMalloc(&vertexY, sizeof(double) * (g.Y + 1)));
     TL_HIP(hipMalloc(&vertexZ, sizeof(double) * (g.Z + 1)));
     TL_HIP(hipMalloc(&cellX, sizeof(double) * g.X));
     TL_HIP(hipMalloc(&cellY, sizeof(double) * g.Y));
     TL_HIP(hipMalloc(&cellZ, sizeof(double) * g.Z));
-    TL_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)g.nblocks));
-    TL_HIP(hipMalloc(&ticket, sizeof(unsigned)));
-    TL_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), stream));
+    TL_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)std::max<long>(g.nblocks, 65536)));
+    TL_HIP(hipMalloc(&ds, sizeof(DevScalars)));
+    TL_HIP(hipMemsetAsync(ds, 0, sizeof(DevScalars), stream));
     TL_HIP(hipMalloc(&d_result, sizeof(double) * 4 * S_COUNT));
     double* h = nullptr;
     TL_HIP(hipHostMalloc(&h, sizeof(double) * 4 * S_COUNT, hipHostMallocMapped | hipHostMallocCoherent));
@@ -804,6 +897,12 @@ Chunk::~Chunk()
     for (double* p : f) (void)hipFree(p);
     (void)hipFree(u_alt);
     (void)hipFree(sd_alt);
+    (void)hipFree(p_alt);
+    (void)hipFree(r_alt);
+    (void)hipFree(d_coef);
+    (void)hipFree(ds);
+    (void)hipFree(hist_a);
+    (void)hipFree(hist_b);
     (void)hipFree(vertexX);
     (void)hipFree(vertexY);
     (void)hipFree(vertexZ);
@@ -811,15 +910,23 @@ Chunk::~Chunk()
     (void)hipFree(cellY);
     (void)hipFree(cellZ);
     (void)hipFree(partials);
-    (void)hipFree(ticket);
     (void)hipFree(d_result);
     if (h_result) (void)hipHostFree((void*)h_result);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-Reduce Chunk::reducer(Slot s) const
+void Chunk::ensure_history(int n)
 {
-    return Reduce{partials, ticket, d_result + 4 * s, h_result_dev + 4 * s};
+    if (n <= hist_cap) return;
+    sync();
+    (void)hipFree(hist_a);
+    (void)hipFree(hist_b);
+    hist_a = hist_b = nullptr;
+    TL_HIP(hipMalloc(&hist_a, sizeof(double) * (size_t)(n + 1)));
+    TL_HIP(hipMalloc(&hist_b, sizeof(double) * (size_t)(n + 1)));
+    TL_HIP(hipMemsetAsync(hist_a, 0, sizeof(double) * (size_t)(n + 1), stream));
+    TL_HIP(hipMemsetAsync(hist_b, 0, sizeof(double) * (size_t)(n + 1), stream));
+    hist_cap = n;
 }
 
 void Chunk::sync() const
@@ -921,7 +1028,8 @@ void op_set_field(Chunk& c)
 void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* temp)
 {
     TIMED(c, K_SUMMARY, STRIP_LAUNCH(k_field_summary, c, c.f[F_VOLUME], c.f[F_DENSITY], c.f[F_ENERGY0], c.f[F_U],
-                                     c.reducer(S_SUMMARY)));
+                                     c.reducer()));
+    finish<4>(c, c.g.nblocks, S_SUMMARY, EP_STORE);
     c.sync();
     *vol = c.h_result[4 * S_SUMMARY + 0];
     *mass = c.h_result[4 * S_SUMMARY + 1];
@@ -973,30 +1081,88 @@ double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt)
         FLAT_LAUNCH(k_cg_init_k, c, coefficient, rx, ry, rz, c.f[F_DENSITY], c.f[F_ENERGY1], c.f[F_U], c.f[F_P],
                     c.f[F_R], c.f[F_W], c.f[F_KX], c.f[F_KY], c.f[F_KZ]);
         STRIP_LAUNCH(k_cg_calc_rro, c, kptr(c), (int)preconditioner, c.f[F_U], c.f[F_W], c.f[F_R], c.f[F_P],
-                     c.f[F_MI], c.f[F_Z], c.reducer(S_RRO));
+                     c.f[F_MI], c.f[F_Z], c.reducer());
     });
+    finish<1>(c, c.g.nblocks, S_RRO, EP_RRO);
     return c.result(S_RRO);
 }
 
 // ext_core.cpp:417-429
 double op_cg_calc_w(Chunk& c)
 {
-    TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w, c, kptr(c), c.f[F_P], c.f[F_W], c.reducer(S_PW)));
+    TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
+                                  c.reducer()));
+    finish<1>(c, c.g.nblocks, S_PW, EP_STORE);
     return c.result(S_PW);
 }
 
 // ext_core.cpp:433-447
 double op_cg_calc_ur(Chunk& c, double alpha)
 {
-    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, alpha, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
-                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN)));
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, alpha, nullptr, (int)c.preconditioner, c.f[F_U], c.f[F_R],
+                                   c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_STORE);
     return c.result(S_RRN);
 }
 
 // ext_core.cpp:451-464
 void op_cg_calc_p(Chunk& c, double beta)
 {
-    TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, beta, (int)c.preconditioner, c.f[F_P], c.f[F_R], c.f[F_Z]));
+    TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, beta, nullptr, c.f[F_P], c.preconditioner ? c.f[F_Z] : c.f[F_R]));
+}
+
+void op_cg_calc_w_dev(Chunk& c, bool fold)
+{
+    const double* q = c.preconditioner ? c.f[F_Z] : c.f[F_R];
+    if (fold) {
+        TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_calc_w<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
+                                      c.reducer()));
+        std::swap(c.f[F_P], c.p_alt);
+    } else {
+        TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
+                                      c.reducer()));
+    }
+    finish<1>(c, c.g.nblocks, S_PW, EP_PW);
+}
+
+void op_cg_calc_ur_dev(Chunk& c, int n)
+{
+    if (n > 0) c.ensure_history(n);
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
+                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN, n > 0 ? n : -1);
+}
+
+void op_ppcg_calc_ur_dev(Chunk& c)
+{
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
+                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+}
+
+void op_cg_calc_p_dev(Chunk& c)
+{
+    TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, 0.0, c.ds, c.f[F_P], c.preconditioner ? c.f[F_Z] : c.f[F_R]));
+}
+
+void op_2norm_rrn_dev(Chunk& c)
+{
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN);
+}
+
+double op_error(Chunk& c) { return c.result(S_RRN); }
+
+void op_finish_rrn(Chunk& c, long nblocks) { finish<1>(c, nblocks, S_RRN, EP_RRN); }
+
+void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b)
+{
+    a.assign(n + 2, 0.0);
+    b.assign(n + 2, 0.0);
+    if (n <= 0) return;
+    if (n > c.hist_cap) throw Error("CG history shorter than requested");
+    TL_HIP(hipMemcpyAsync(a.data() + 1, c.hist_a + 1, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    TL_HIP(hipMemcpyAsync(b.data() + 1, c.hist_b + 1, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
 }
 
 static void store_coefs(Chunk& c, const double* alphas, const double* betas, int n)
@@ -1004,6 +1170,30 @@ static void store_coefs(Chunk& c, const double* alphas, const double* betas, int
     if (n < 0) throw Error("negative coefficient count");
     c.alphas.assign(alphas, alphas + n);
     c.betas.assign(betas, betas + n);
+}
+
+// Device copy for the fused PPCG passes (tl_ppcg_tb.hip): up to 12 steps
+// run as one pass; more run as passes of 10 and a remainder.  The pass that
+// starts at step k finds a[k..k+m) and then b[k..k+m) from offset 2k.
+static void store_device_coefs(Chunk& c, int n)
+{
+    const int need = 2 * n + 24;
+    if (need > c.d_coef_cap) {
+        c.sync();
+        (void)hipFree(c.d_coef);
+        c.d_coef = nullptr;
+        TL_HIP(hipMalloc(&c.d_coef, sizeof(double) * need));
+        c.d_coef_cap = need;
+    }
+    std::vector<double> h(need, 0.0);
+    for (int k = 0; k < n; k += (n <= 12 ? n : 10)) {
+        const int m = n <= 12 ? n : std::min(10, n - k);
+        for (int q = 0; q < m; ++q) {
+            h[2 * k + q] = c.alphas[k + q];
+            h[2 * k + m + q] = c.betas[k + q];
+        }
+    }
+    TL_HIP(hipMemcpy(c.d_coef, h.data(), sizeof(double) * need, hipMemcpyHostToDevice));
 }
 
 // ext_core.cpp:468-497
@@ -1048,6 +1238,7 @@ void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, do
 {
     c.theta = theta;
     store_coefs(c, alphas, betas, n);
+    store_device_coefs(c, n);
 }
 
 // ext_core.cpp:543-550
@@ -1094,12 +1285,13 @@ void op_jacobi_init(Chunk& c, int coefficient, double dt)
 double op_jacobi_solve(Chunk& c, bool fused)
 {
     if (fused) {
-        TIMED(c, K_JACOBI, STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_U], c.f[F_R], c.reducer(S_ERR)));
+        TIMED(c, K_JACOBI, STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_U], c.f[F_R], c.reducer()));
         std::swap(c.f[F_U], c.f[F_R]);
     } else {
         TL_HIP(hipMemcpyAsync(c.f[F_R], c.f[F_U], sizeof(double) * c.g.alloc, hipMemcpyDeviceToDevice, c.stream));
-        STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_R], c.f[F_U], c.reducer(S_ERR));
+        STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_R], c.f[F_U], c.reducer());
     }
+    finish<1>(c, c.g.nblocks, S_ERR, EP_STORE);
     return c.result(S_ERR);
 }
 
@@ -1112,7 +1304,8 @@ void op_calc_residual(Chunk& c)
 
 double op_calc_2norm(Chunk& c, bool norm_r)
 {
-    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, norm_r ? c.f[F_R] : c.f[F_U0], c.reducer(S_NORM)));
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, norm_r ? c.f[F_R] : c.f[F_U0], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_NORM, EP_STORE);
     return c.result(S_NORM);
 }
 

@@ -285,3 +285,37 @@ def test_random_stencil_inputs(hip, oracle, dim):
     assert_sum(pw[0], pw[1], float(np.sum(np.abs(interior(o, w) * interior(o, o.get("p"))))))
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("shape,nsteps", [((37, 23), 10), ((150, 140), 10), ((150, 140), 3), ((64, 200), 1),
+                                          ((130, 70), 12), ((90, 100), 15), ((45, 66), 25)])
+def test_ppcg_outer_fused(hip, oracle, shape, nsteps):
+    """The driver's temporal-blocked PPCG outer iteration (calc_ur, init_sd,
+    all inner steps and ||r||^2 in one sweep) against the reference's
+    separate sweeps on the oracle: u and r bitwise, rrn to 1e-12."""
+    import ctypes
+    nx, ny = shape
+    g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+    o = hip.ExtChunk(oracle.lib(2), 2, nx, ny)
+    rng = np.random.default_rng(nx * 1000 + ny + nsteps)
+    random_fields(g, o, ["u", "p", "r", "w"], rng, lo=-1.0, hi=1.0)
+    random_fields(g, o, ["kx", "ky"], rng, lo=0.1, hi=40.0)
+    a, b, theta = oracle.calc_ch_coefs(0.7, 150.0, nsteps)
+    alpha = 0.37
+    ca = (ctypes.c_double * nsteps)(*a)
+    cb = (ctypes.c_double * nsteps)(*b)
+    rrn_g = ctypes.c_double()
+    rc = g.L.tl_ppcg_outer_fused(g.c, ctypes.c_double(alpha), ctypes.c_double(theta), ca, cb, nsteps,
+                                 ctypes.byref(rrn_g))
+    assert rc == 0, g.L.tl_last_error()
+    o.ppcg_init(a, b, theta)
+    o.cg_calc_ur(alpha)
+    o.ppcg_init_sd()
+    for step in range(1, nsteps + 1):
+        o.update_halo([FIELD_SD], 1)
+        o.ppcg_inner(step)
+    rrn_o = o.calculate_2norm(True)
+    assert_same(g, o, ["u", "r"], region="interior")
+    assert_sum(rrn_g.value, rrn_o, rrn_o)
+    g.close()
+    o.close()
