@@ -117,6 +117,48 @@ def cpu_baseline(n: int, threads: int, steps: list[dict], budget_s: float = 20.0
     }
 
 
+def kernel_probe(n: int, launches: int) -> dict:
+    """A short program for hardware-counter runs: the 2D n^2 two-rectangle
+    chunk set up through the ext ABI (generate, CG init, 40 CG iterations),
+    then `launches` fused PPCG outer iterations (tl_ppcg_outer_fused, 10
+    inner steps).  Prints the mean launch time from HIP events."""
+    import ctypes
+
+    L = tealeaf_hip.lib(2)
+    ch = tealeaf_hip.ExtChunk(L, 2, n, n)
+    dx = 10.0 / n
+    s = dx / 100.0
+    ch.initialise_chunk(0.0, 0.0, dx, dx)
+    ch.generate_chunk(density=[100.0, 0.1, 0.1], energy=[1e-4, 25.0, 0.1], xmin=[0, s, 1 + s],
+                      xmax=[0, 1 - s, 6 - s], ymin=[0, s, 1 + s], ymax=[0, 2 - s, 6 - s], radius=[0, 0, 0],
+                      geometry=[0, 1, 1])
+    ch.update_halo([1, 2, 3], 2)
+    ch.set_field_kernel()
+    rro = ch.cg_solver_init(1, False, 0.004)
+    ch.copy_u()
+    for _ in range(40):
+        pw = ch.cg_calc_w()
+        rrn = ch.cg_calc_ur(rro / pw)
+        ch.cg_calc_p(rrn / rro)
+        rro = rrn
+    ch.cg_calc_w()
+    a = [0.9 - 0.01 * k for k in range(10)]
+    b = [1e-3 * (k + 1) for k in range(10)]
+    ca = (ctypes.c_double * 10)(*a)
+    cb = (ctypes.c_double * 10)(*b)
+    out = ctypes.c_double()
+    L.tl_ppcg_outer_fused(ch.c, ctypes.c_double(1e-3), ctypes.c_double(2000.0), ca, cb, 10, ctypes.byref(out))
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        rc = L.tl_ppcg_outer_fused(ch.c, ctypes.c_double(1e-3), ctypes.c_double(2000.0), ca, cb, 10,
+                                   ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(L.tl_last_error().decode())
+    dt = (time.perf_counter() - t0) / launches
+    ch.close()
+    return {"probe": "ppcg_outer", "n": n, "launches": launches, "host_s_per_launch": dt}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,7 +168,22 @@ def main() -> int:
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the plain copy kernel (counter calibration)")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        L = tealeaf_hip.lib(2)
+        L.tl_copy_probe.restype = __import__("ctypes").c_double
+        n = 1 << 28  # 2 GiB per buffer
+        print(json.dumps({"probe": "copy", "doubles": n, "bytes_each_way": 8 * n,
+                          "gbs": L.tl_copy_probe(n, 10)}), flush=True)
+        return 0
+    if args.kernel_probe:
+        print(json.dumps(kernel_probe(args.n, 20)), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the solve runs as a single chunk on one MI355X "
@@ -167,6 +224,10 @@ def main() -> int:
                     "kernel": dom_name, "avg_launch_us": round(avg_s * 1e6, 2),
                     "bytes_per_launch": bytes_per_launch}
 
+    import ctypes
+    L.tl_copy_probe.restype = ctypes.c_double
+    copy_gbs = L.tl_copy_probe(1 << 27, 10)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.n, args.cpu_threads, timed)
@@ -195,6 +256,7 @@ def main() -> int:
         "kernels": {k: {"launches": v["count"], "avg_us": round(1e3 * v["ms"] / v["count"], 2),
                         "gbs": round(v["bytes_per_cell"] * cells / (v["ms"] / v["count"] * 1e-3) / 1e9, 1)}
                     for k, v in kern.items()},
+        "copy_gbs": round(copy_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -122,6 +122,11 @@ void tl_set_kernel_timing(int from_step);
 int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
                         int nsteps, double* rrn);
 
+/* GB/s (read + write) of a plain device copy of n doubles repeated reps
+ * times: the attainable streaming rate, and the known-size kernel that
+ * calibrates HBM byte counters.  Negative on error. */
+double tl_copy_probe(long n, int reps);
+
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);
 const char* tl_last_error(void);

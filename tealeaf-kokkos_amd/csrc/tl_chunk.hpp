@@ -172,6 +172,7 @@ void op_ppcg_calc_ur_dev(Chunk& c);  // u += alpha p, r -= alpha w; its r.r is n
 void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in the PPCG loop
 double op_error(Chunk& c);        // waits and returns ds->error
 void op_finish_rrn(Chunk& c, long nblocks);  // second phase of an rrn reduction, PPCG recurrences
+double stream_copy_probe(long n, int reps);   // GB/s of a plain copy of n doubles
 
 // Temporal-blocked PPCG outer iteration (tl_ppcg_tb.hip): after CGCalcW,
 // CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep.

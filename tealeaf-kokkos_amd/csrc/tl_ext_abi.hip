@@ -113,6 +113,16 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
     return rc;
 }
 
+// Bandwidth of a plain device copy of n doubles repeated reps times (GB/s,
+// read + write): the machine's attainable streaming rate, and the known-size
+// kernel that calibrates HBM byte counters.  Returns a negative value on error.
+double tl_copy_probe(long n, int reps)
+{
+    double gbs = -1.0;
+    guarded([&] { gbs = stream_copy_probe(n, reps); });
+    return gbs;
+}
+
 // ---- ext_chunk.cpp:18-33 --------------------------------------------------
 
 #if TL_DIM == 3

@@ -388,6 +388,115 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const doubl
     block_partial<1>(v, red);
 }
 
+#if TL_DIM == 2
+// Lane i receives lane i-1 / i+1 through DPP row shifts (wave_shr:1 /
+// wave_shl:1); the lane at the wave's edge keeps its own value.
+__device__ __forceinline__ double dpp_from_west(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double dpp_from_east(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
+}
+
+// CGCalcP folded into CGCalcW as a streaming 2D sweep: each lane owns two
+// adjacent columns, a wave 128 columns and a strip of `rows` rows; the new p
+// of rows j-1, j, j+1 is kept in registers, x-neighbours come from adjacent
+// lanes (DPP) and, at the wave's edges, from one extra load.  Every input
+// is read from HBM once: 48 bytes per cell.  Same expressions as k_cg_calc_w.
+struct PWStrip {
+    int tiles_x, rows;
+    long nwaves;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k, const double* __restrict__ p,
+                                                       const double* __restrict__ q,
+                                                       const DevScalars* __restrict__ ds, double* __restrict__ p_out,
+                                                       double* __restrict__ w, Reduce red)
+{
+    const int lane = threadIdx.x % WAVE;
+    const long wv = (long)blockIdx.x * WAVES_PER_BLOCK + threadIdx.x / WAVE;
+    double acc = 0.0;
+    if (wv < t.nwaves) {
+        const long tw = xcd_tile(wv, t.nwaves);
+        const int tx = (int)(tw % t.tiles_x), ty = (int)(tw / t.tiles_x);
+        const int nx = g.nx, ny = g.ny;
+        const int c0 = HALO + tx * 2 * WAVE + 2 * lane, c1 = c0 + 1;
+        const bool v0 = c0 <= nx + 1, v1 = c1 <= nx + 1;
+        const int k0 = min(c0, nx + 1), k1 = min(c1, nx + 1);
+        // the column beyond the wave's edge (west of lane 0, east of lane 63)
+        const int ce = lane == 0 ? max(c0 - 1, HALO) : min(c1 + 1, nx + 1);
+        const bool edge = lane == 0 || lane == WAVE - 1;
+        const int j0 = HALO + ty * t.rows, j1 = min(j0 + t.rows, ny + HALO);  // rows [j0, j1)
+        const double beta = ds->beta;
+        auto pnew = [&](int col, int row) {
+            const long i = gidx(g, col, row, 0);
+            return beta * p[i] + q[i];
+        };
+        // window: rows j-1 (s), j (c), j+1 (n) of the new p, two columns
+        const int js = max(j0 - 1, HALO);
+        double s0 = pnew(k0, js), s1 = pnew(k1, js);
+        double c0v = pnew(k0, j0), c1v = pnew(k1, j0);
+        double ec = edge ? pnew(ce, j0) : 0.0;
+        if (j0 == HALO) {  // bottom face: the reflective mirror
+            s0 = c0v;
+            s1 = c1v;
+        }
+        double kyc0 = k.ky[gidx(g, k0, j0, 0)], kyc1 = k.ky[gidx(g, k1, j0, 0)];
+        for (int j = j0; j < j1; ++j) {
+            const int jn = min(j + 1, ny + 1);
+            double n0 = pnew(k0, jn), n1 = pnew(k1, jn);
+            const double en = edge ? pnew(ce, jn) : 0.0;
+            if (j == ny + 1) {  // top face
+                n0 = c0v;
+                n1 = c1v;
+            }
+            const long i0 = gidx(g, k0, j, 0), i1 = gidx(g, k1, j, 0);
+            // kx lives on faces up to nx + 2: the lane past the last column
+            // must hold kx of the right face for its west neighbour's DPP
+            const double kx0 = k.kx[gidx(g, min(c0, nx + 2), j, 0)], kx1 = k.kx[gidx(g, min(c1, nx + 2), j, 0)];
+            const double kyn0 = k.ky[i0 + g.sx], kyn1 = k.ky[i1 + g.sx];
+            double kxe1 = dpp_from_east(kx0);
+            if (lane == WAVE - 1) kxe1 = k.kx[gidx(g, min(c1 + 1, nx + 2), j, 0)];
+            double w0 = dpp_from_west(c1v), e1 = dpp_from_east(c0v);
+            if (lane == 0) w0 = ec;
+            if (lane == WAVE - 1) e1 = ec;
+            double e0 = c1v, w1 = c0v;
+            if (c0 == HALO) w0 = c0v;
+            if (c0 == nx + 1) e0 = c0v;
+            if (c1 == nx + 1) e1 = c1v;
+            const double m0 = (1.0 + (kx1 + kx0) + (kyn0 + kyc0)) * c0v - (kx1 * e0 + kx0 * w0) - (kyn0 * n0 + kyc0 * s0);
+            const double m1 = (1.0 + (kxe1 + kx1) + (kyn1 + kyc1)) * c1v - (kxe1 * e1 + kx1 * w1) - (kyn1 * n1 + kyc1 * s1);
+            if (v0) {
+                p_out[i0] = c0v;
+                w[i0] = m0;
+                acc += m0 * c0v;
+            }
+            if (v1) {
+                p_out[i1] = c1v;
+                w[i1] = m1;
+                acc += m1 * c1v;
+            }
+            s0 = c0v;
+            s1 = c1v;
+            c0v = n0;
+            c1v = n1;
+            ec = en;
+            kyc0 = kyn0;
+            kyc1 = kyn1;
+        }
+    }
+    double v[1] = {acc};
+    block_partial<1>(v, red);
+}
+#endif
+
 // CGCalcUr (:244-264); alpha from the device scalars when ds is given.
 __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, const DevScalars* __restrict__ ds,
                                                       int pre, double* __restrict__ u, double* __restrict__ r,
@@ -704,6 +813,13 @@ inline void check_launch()
                            __VA_ARGS__);                                                          \
         check_launch();                                                                           \
     } while (0)
+
+// Plain device-to-device copy of n doubles, 16 bytes per lane per step: the
+// calibration kernel for HBM byte counters (it moves exactly 8n bytes each way).
+__global__ __launch_bounds__(BLOCK) void k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b, long n2)
+{
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n2; i += (long)gridDim.x * BLOCK) b[i] = a[i];
+}
 
 // Second phase of a reduction whose first phase wrote `nblocks` partials.
 template <int NV>
@@ -1115,9 +1231,26 @@ void op_cg_calc_w_dev(Chunk& c, bool fold)
 {
     const double* q = c.preconditioner ? c.f[F_Z] : c.f[F_R];
     if (fold) {
+#if TL_DIM == 2
+        PWStrip t;
+        t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
+        t.rows = 64;
+        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 8192) t.rows /= 2;
+        t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
+        const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        TIMED(c, K_CG_WP, {
+            hipLaunchKernelGGL(k_cg_pw_strip, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t, kptr(c), c.f[F_P],
+                               q, c.ds, c.p_alt, c.f[F_W], c.reducer());
+            check_launch();
+        });
+        std::swap(c.f[F_P], c.p_alt);
+        finish<1>(c, nb, S_PW, EP_PW);
+        return;
+#else
         TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_calc_w<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
                                       c.reducer()));
         std::swap(c.f[F_P], c.p_alt);
+#endif
     } else {
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
                                       c.reducer()));
@@ -1153,6 +1286,33 @@ void op_2norm_rrn_dev(Chunk& c)
 double op_error(Chunk& c) { return c.result(S_RRN); }
 
 void op_finish_rrn(Chunk& c, long nblocks) { finish<1>(c, nblocks, S_RRN, EP_RRN); }
+
+double stream_copy_probe(long n, int reps)
+{
+    if (n < 2 || reps < 1) throw Error("copy probe needs n >= 2 and reps >= 1");
+    n &= ~1L;
+    double *a = nullptr, *b = nullptr;
+    TL_HIP(hipMalloc(&a, sizeof(double) * n));
+    TL_HIP(hipMalloc(&b, sizeof(double) * n));
+    TL_HIP(hipMemset(a, 0, sizeof(double) * n));
+    hipEvent_t e0, e1;
+    TL_HIP(hipEventCreate(&e0));
+    TL_HIP(hipEventCreate(&e1));
+    const unsigned grid = 256 * 32;
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
+    TL_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
+    TL_HIP(hipEventRecord(e1, 0));
+    TL_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    TL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;  // GB/s, read + write
+}
 
 void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b)
 {

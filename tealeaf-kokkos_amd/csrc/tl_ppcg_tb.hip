@@ -421,10 +421,18 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.partials = c.partials;
         const int ow = 2 * WAVE - 2 * L;
         a.tiles_x = (c.g.nx + ow - 1) / ow;
-        // tallest strips (less halo recomputation) that still give every
-        // SIMD a few waves
-        a.rows = 256;
-        while (a.rows > 16 && (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows) < 4096) a.rows /= 2;
+        // rows per wave: minimise (waves per SIMD, rounded up) x (rows + 2L
+        // rounds per wave), the time of the last SIMD to finish
+        a.rows = 16;
+        double best = 1e300;
+        for (int rows = 16; rows <= 512; rows *= 2) {
+            const long waves = (long)a.tiles_x * ((c.g.ny + rows - 1) / rows);
+            const double cost = (double)((waves + 1023) / 1024) * (std::min(rows, c.g.ny) + 2 * L);
+            if (cost < best) {
+                best = cost;
+                a.rows = rows;
+            }
+        }
         const long tiles_y = (c.g.ny + a.rows - 1) / a.rows;
         a.nblocks = (long)a.tiles_x * tiles_y;
         if (first && last) launch_single(c, L, a, a.nblocks);

@@ -69,6 +69,8 @@ def lib(dim: int) -> ctypes.CDLL:
         L.tl_run_deck.restype = ctypes.c_int
         L.tl_driver_error.restype = ctypes.c_char_p
         L.tl_last_error.restype = ctypes.c_char_p
+        L.tl_copy_probe.argtypes = [ctypes.c_long, ctypes.c_int]
+        L.tl_copy_probe.restype = ctypes.c_double
         L.tl_set_abort_on_error(0)
         _LIBS[dim] = L
     return _LIBS[dim]
