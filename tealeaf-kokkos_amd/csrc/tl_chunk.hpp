@@ -127,6 +127,8 @@ struct Chunk {
     double* h_result_dev = nullptr;       // device alias of h_result
 
     hipStream_t stream = nullptr;
+    int simds = 1024;            // SIMDs of the device (CUs x 4): one resident wave each
+    long partials_cap = 0;       // block partials the reduction scratch holds
     mutable KernelTimer timer;
 
     Chunk(int nx, int ny, int nz, int rank);

@@ -995,7 +995,12 @@ Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
     TL_HIP(hipMalloc(&cellX, sizeof(double) * g.X));
     TL_HIP(hipMalloc(&cellY, sizeof(double) * g.Y));
     TL_HIP(hipMalloc(&cellZ, sizeof(double) * g.Z));
-    TL_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)std::max<long>(g.nblocks, 65536)));
+    partials_cap = std::max<long>(g.nblocks, 65536);
+    TL_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)partials_cap));
+    int dev = 0, cus = 0;
+    TL_HIP(hipGetDevice(&dev));
+    TL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus > 0) simds = 4 * cus;
     TL_HIP(hipMalloc(&ds, sizeof(DevScalars)));
     TL_HIP(hipMemsetAsync(ds, 0, sizeof(DevScalars), stream));
     TL_HIP(hipMalloc(&d_result, sizeof(double) * 4 * S_COUNT));

@@ -48,20 +48,23 @@ struct TBArgs {
     long nblocks;
 };
 
-// Lane i receives lane i-1 (x-neighbour at kk-1); lane 0 keeps its own.
+// Lane i receives lane i-1 (x-neighbour at kk-1); lane 0 receives 0.
+// Lane 0 and lane 63 either sit on a face of the mesh, where the clamps
+// replace the shifted value, or own no output cell, so the fill never
+// reaches a result.
 __device__ __forceinline__ double from_west(double v)
 {
     const int lo = __double2loint(v), hi = __double2hiint(v);
-    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
-                            __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true));
 }
 
-// Lane i receives lane i+1 (x-neighbour at kk+1); lane 63 keeps its own.
+// Lane i receives lane i+1 (x-neighbour at kk+1); lane 63 receives 0.
 __device__ __forceinline__ double from_east(double v)
 {
     const int lo = __double2loint(v), hi = __double2hiint(v);
-    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
-                            __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true));
 }
 
 // Per-wave state of the streaming pass.  Each lane owns two adjacent
@@ -82,6 +85,10 @@ struct TBCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
     int kc0, kc1, kf0, kf1;  // clamped columns (cells; kx faces up to nx + 2)
+    // byte offsets within a row: cells, kx faces, store targets (the cell
+    // for lanes that own one, else a pad column past the halo that nothing
+    // reads) and the pad column alone (rows that own no output)
+    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
     bool wcl0, wcl1, ecl0, ecl1;
     bool out0, out1, out_any;
     double alpha, theta;
@@ -91,32 +98,59 @@ struct TBCtx {
     const double* coef;  // LDS copy of a[0..L), b[0..L)
 };
 
+// Loads of row y into a prefetch slot.  Every load is issued on every
+// round, with the row clamped into the mesh (rows past the top repeat the
+// last one; u and p are only used on the wave's output rows), so that the
+// number of memory operations in flight is the same on every path and the
+// compiler can wait for exactly the oldest slot instead of draining all.
+// Row y of an array, as a wave-uniform pointer.
+__device__ __forceinline__ const double* row_of(const double* a, const Geom& g, int y)
+{
+    return a + g.base + (long)y * g.sx;
+}
+
+__device__ __forceinline__ double ld(const double* row, unsigned boff)
+{
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row) + boff);
+}
+
+__device__ __forceinline__ void st(double* row, unsigned boff, double v)
+{
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
+}
+
+// Loads of row y into a prefetch slot.  Every load is issued on every
+// round, with the row clamped into the mesh (rows past the top repeat the
+// last one; u and p are only used on the wave's output rows), so that the
+// number of memory operations in flight is the same on every path and the
+// compiler can wait for exactly the oldest slot instead of draining all.
+// Addresses are a uniform row pointer plus a per-lane 32-bit byte offset.
 template <bool FIRST>
 __device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
 {
-    if (y <= c.ny + 1) {
-        const long i0 = gidx(c.g, c.kc0, y, 0), i1 = gidx(c.g, c.kc1, y, 0);
-        v[0][0] = A.r_in[i0];
-        v[0][1] = A.r_in[i1];
-        const double* b = FIRST ? A.w : A.sd_in;
-        v[1][0] = b[i0];
-        v[1][1] = b[i1];
-        if (c.out_any && y >= c.Y0 && y <= c.yhi) {
-            v[2][0] = A.u[i0];
-            v[2][1] = A.u[i1];
-            if (FIRST) {
-                v[3][0] = A.p[i0];
-                v[3][1] = A.p[i1];
-            }
-        }
-        v[4][0] = A.kx[gidx(c.g, c.kf0, y, 0)];
-        v[4][1] = A.kx[gidx(c.g, c.kf1, y, 0)];
-        v[5][0] = A.ky[i0];
-        v[5][1] = A.ky[i1];
-    } else if (y == c.ny + 2) {
-        v[5][0] = A.ky[gidx(c.g, c.kc0, y, 0)];  // ky on the top face
-        v[5][1] = A.ky[gidx(c.g, c.kc1, y, 0)];
+    const int yr = min(y, c.ny + 1);
+    const int yk = min(y, c.ny + 2);  // ky also on the top face
+    const int yo = min(max(y, c.Y0), c.yhi);
+    const double* r = row_of(A.r_in, c.g, yr);
+    v[0][0] = ld(r, c.bc0);
+    v[0][1] = ld(r, c.bc1);
+    const double* b = row_of(FIRST ? A.w : A.sd_in, c.g, yr);
+    v[1][0] = ld(b, c.bc0);
+    v[1][1] = ld(b, c.bc1);
+    const double* u = row_of(A.u, c.g, yo);
+    v[2][0] = ld(u, c.bc0);
+    v[2][1] = ld(u, c.bc1);
+    if (FIRST) {
+        const double* p = row_of(A.p, c.g, yo);
+        v[3][0] = ld(p, c.bc0);
+        v[3][1] = ld(p, c.bc1);
     }
+    const double* kx = row_of(A.kx, c.g, yr);
+    v[4][0] = ld(kx, c.bf0);
+    v[4][1] = ld(kx, c.bf1);
+    const double* ky = row_of(A.ky, c.g, yk);
+    v[5][0] = ld(ky, c.bc0);
+    v[5][1] = ld(ky, c.bc1);
 }
 
 __device__ __forceinline__ int ring(int row, int n)
@@ -127,7 +161,9 @@ __device__ __forceinline__ int ring(int row, int n)
 
 // One round: level 0 at row y, then level s at row y - s for s = 1..L.
 // XE: the wave reaches an x face (per-lane clamps); YE: it reaches the
-// bottom or top face (row clamps and rows past the top).
+// bottom or top face (row clamps and rows past the top).  The LDS rings of
+// level s + 1 are read before level s computes, so their latency overlaps
+// level s's arithmetic (the rows differ, so no write of level s is missed).
 template <int L, bool FIRST, bool LAST, bool XE, bool YE, int PH>
 __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y)
 {
@@ -136,12 +172,17 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     const int lane = threadIdx.x;
     const int ny = c.ny;
     double(&v)[6][2] = st.pf[PH % 2];
-
-    // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
-    const int ry = ring(y, L + 1);
     double2* ur = reinterpret_cast<double2*>(c.ur);
     double2* kxr = reinterpret_cast<double2*>(c.kxr);
     double2* kyr = reinterpret_cast<double2*>(c.kyr);
+
+    // inputs of level 1 (row y - 1)
+    int rn = ring(y - 1, L + 1);
+    double2 nkx = kxr[rn * WAVE + lane], nky = kyr[rn * WAVE + lane], nu = ur[rn * WAVE + lane];
+    double na = c.coef[0], nb = c.coef[L];
+
+    // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
+    const int ry = ring(y, L + 1);
     double kyn0, kyn1;
     if (!YE || y <= ny + 1) {
 #pragma unroll
@@ -158,7 +199,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
             st.rw[0][R0][q] = r0;
         }
         double2 u0 = make_double2(0.0, 0.0);
-        if (c.out_any && y >= c.Y0 && y <= c.yhi) {
+        if (y >= c.Y0 && y <= c.yhi) {
             u0.x = FIRST ? v[2][0] + c.alpha * v[3][0] : v[2][0];
             u0.y = FIRST ? v[2][1] + c.alpha * v[3][1] : v[2][1];
         }
@@ -185,9 +226,21 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
         const int z = y - s;
-        const int rz = ring(z, L + 1);
+        const int rz = rn;
+        const double2 kx = nkx, ky = nky;
+        double2 u = nu;
+        const double a = na, b = nb;
+        if (s < L) {
+            rn = ring(z - 1, L + 1);
+            nkx = kxr[rn * WAVE + lane];
+            nky = kyr[rn * WAVE + lane];
+            nu = ur[rn * WAVE + lane];
+            na = c.coef[s];
+            nb = c.coef[L + s];
+        }
+        // level L's results (zero where the level is beyond the top face)
+        double rn0 = 0.0, rn1 = 0.0;
         if (!YE || z <= ny + 1) {
-            const double a = c.coef[s - 1], b = c.coef[L + s - 1];
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
             double s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
             double n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
@@ -204,17 +257,15 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                 if (c.ecl0) e0 = c0;
                 if (c.ecl1) e1 = c1;
             }
-            const double2 kx = kxr[rz * WAVE + lane];
             const double kxe1 = from_east(kx.x);
-            const double2 ky = kyr[rz * WAVE + lane];
             // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z)
             const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
             const double m1 = (1.0 + (kxe1 + kx.y) + (kyn1 + ky.y)) * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
             kyn0 = ky.x;
             kyn1 = ky.y;
             // PPCGCalcU: r -= A sd; u += sd
-            const double rn0 = st.rw[s - 1][R1][0] - m0, rn1 = st.rw[s - 1][R1][1] - m1;
-            double2 u = ur[rz * WAVE + lane];
+            rn0 = st.rw[s - 1][R1][0] - m0;
+            rn1 = st.rw[s - 1][R1][1] - m1;
             u.x = u.x + c0;
             u.y = u.y + c1;
             if (s < L) {
@@ -224,24 +275,8 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                 st.rw[s][R0][0] = rn0;
                 st.rw[s][R0][1] = rn1;
                 ur[rz * WAVE + lane] = u;
-            } else if (c.out_any && z >= c.Y0 && z <= c.yhi) {
-                if (c.out0) {
-                    const long i = gidx(c.g, c.kc0, z, 0);
-                    A.r_out[i] = rn0;
-                    A.u[i] = u.x;
-                    if (!LAST) A.sd_out[i] = a * c0 + b * rn0;
-                    if (LAST) st.acc += rn0 * rn0;
-                }
-                if (c.out1) {
-                    const long i = gidx(c.g, c.kc1, z, 0);
-                    A.r_out[i] = rn1;
-                    A.u[i] = u.y;
-                    if (!LAST) A.sd_out[i] = a * c1 + b * rn1;
-                    if (LAST) st.acc += rn1 * rn1;
-                }
             }
         } else {
-            const double2 ky = kyr[rz * WAVE + lane];
             kyn0 = ky.x;
             kyn1 = ky.y;
             if (s < L) {
@@ -250,6 +285,30 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                     st.sdw[s][S0][q] = st.sdw[s][S1][q];
                     st.rw[s][R0][q] = st.rw[s][R1][q];
                 }
+            }
+        }
+        if (s == L) {
+            // output row: every lane stores, lanes and rows that own no
+            // cell into a pad column, so the store count never depends on a branch
+            const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
+            const bool o0 = zin && c.out0, o1 = zin && c.out1;
+            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            const int zs = min(max(z, 0), ny + 2 * HALO - 1);
+            const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
+            double* ro = const_cast<double*>(row_of(A.r_out, c.g, zs));
+            double* uo = const_cast<double*>(row_of(A.u, c.g, zs));
+            tl::st(ro, b0, rn0);
+            tl::st(uo, b0, u.x);
+            tl::st(ro, b1, rn1);
+            tl::st(uo, b1, u.y);
+            if (!LAST) {
+                double* so = const_cast<double*>(row_of(A.sd_out, c.g, zs));
+                tl::st(so, b0, a * c0 + b * rn0);
+                tl::st(so, b1, a * c1 + b * rn1);
+            }
+            if (LAST) {
+                st.acc += o0 ? rn0 * rn0 : 0.0;
+                st.acc += o1 ? rn1 * rn1 : 0.0;
             }
         }
     }
@@ -294,7 +353,9 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
 template <int L, bool FIRST, bool LAST>
 __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
 {
-    __shared__ double2 lds[3 * (L + 1) * WAVE];
+    __shared__ double2 u_ring[(L + 1) * WAVE];
+    __shared__ double2 kx_ring[(L + 1) * WAVE];
+    __shared__ double2 ky_ring[(L + 1) * WAVE];
     __shared__ double coef[2 * L];
     const int lane = threadIdx.x;
     if (lane < 2 * L) coef[lane] = A.coef[lane];
@@ -313,6 +374,10 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.kc1 = min(max(k1, HALO), c.nx + 1);
     c.kf0 = min(max(k0, HALO), c.nx + 2);
     c.kf1 = min(max(k1, HALO), c.nx + 2);
+    c.bc0 = 8u * (unsigned)c.kc0;
+    c.bc1 = 8u * (unsigned)c.kc1;
+    c.bf0 = 8u * (unsigned)c.kf0;
+    c.bf1 = 8u * (unsigned)c.kf1;
     c.wcl0 = k0 == HALO;
     c.wcl1 = k1 == HALO;
     c.ecl0 = k0 == c.nx + 1;
@@ -320,13 +385,20 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
     c.out_any = c.out0 || c.out1;
+    {
+        // pad columns X .. sx-1 of every row are never read
+        const int npad = (int)(g.sx - g.X);
+        c.bpad = 8u * (unsigned)(g.X + lane % npad);
+        c.bs0 = c.out0 ? c.bc0 : c.bpad;
+        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+    }
     c.ylo = max(c.Y0 - L, HALO);
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
     c.alpha = FIRST ? A.ds->alpha : 0.0;
     c.theta = A.theta;
-    c.ur = reinterpret_cast<double*>(lds);
-    c.kxr = reinterpret_cast<double*>(lds + (L + 1) * WAVE);
-    c.kyr = reinterpret_cast<double*>(lds + 2 * (L + 1) * WAVE);
+    c.ur = reinterpret_cast<double*>(u_ring);
+    c.kxr = reinterpret_cast<double*>(kx_ring);
+    c.kyr = reinterpret_cast<double*>(ky_ring);
     c.coef = coef;
     // waves whose columns or rows reach a face of the mesh need the clamps
     const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
@@ -372,6 +444,28 @@ void launch_last(Chunk& c, int L, TBArgs& a, long nblocks)
 #undef TL_TB_CASE
     default: throw Error("temporal-blocked PPCG pass needs 1..10 levels");
     }
+}
+
+// Output rows per wave.  Every wave holds a whole SIMD (one resident wave
+// each), so the pass takes ceil(waves / simds) turns of the slowest SIMD, each
+// of (rows + 2L) rounds rounded up to the six-round unroll; minimise that,
+// and among equal costs take the fewest waves (least recomputed overlap).
+int tb_rows(int ny, int tiles_x, int L, int simds)
+{
+    int best_rows = 16;
+    long best = -1;
+    for (int rows = 16; rows <= std::max(16, ny); ++rows) {
+        const long tiles_y = (ny + rows - 1) / rows;
+        if (rows > 16 && (ny + rows - 2) / (rows - 1) == tiles_y) continue;  // same tiling as rows - 1
+        const long waves = (long)tiles_x * tiles_y;
+        const long rounds = ((long)std::min(rows, ny) + 2 * L + 5) / 6 * 6;
+        const long cost = (waves + simds - 1) / simds * rounds;
+        if (best < 0 || cost < best || (cost == best && tiles_y < (ny + best_rows - 1) / best_rows)) {
+            best = cost;
+            best_rows = rows;
+        }
+    }
+    return best_rows;
 }
 
 }  // namespace
@@ -421,20 +515,10 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.partials = c.partials;
         const int ow = 2 * WAVE - 2 * L;
         a.tiles_x = (c.g.nx + ow - 1) / ow;
-        // rows per wave: minimise (waves per SIMD, rounded up) x (rows + 2L
-        // rounds per wave), the time of the last SIMD to finish
-        a.rows = 16;
-        double best = 1e300;
-        for (int rows = 16; rows <= 512; rows *= 2) {
-            const long waves = (long)a.tiles_x * ((c.g.ny + rows - 1) / rows);
-            const double cost = (double)((waves + 1023) / 1024) * (std::min(rows, c.g.ny) + 2 * L);
-            if (cost < best) {
-                best = cost;
-                a.rows = rows;
-            }
-        }
+        a.rows = tb_rows(c.g.ny, a.tiles_x, L, c.simds);
         const long tiles_y = (c.g.ny + a.rows - 1) / a.rows;
         a.nblocks = (long)a.tiles_x * tiles_y;
+        if (a.nblocks > c.partials_cap) throw Error("temporal-blocked PPCG: more waves than reduction partials");
         if (first && last) launch_single(c, L, a, a.nblocks);
         else if (first) launch<10, true, false>(c, a, a.nblocks);
         else if (last) launch_last(c, L, a, a.nblocks);
