@@ -1,18 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: TeaLeaf's FP64 linear solve on one MI355X.
+"""Benchmark: TeaLeaf's linear solve on one MI355X.
 
-Workload (the 2D 4096^2 configuration): the two-rectangle layout on a 10x10
-domain, FP64 PPCG (30 CG presteps for the eigenvalue estimate, 10 inner
-Chebyshev steps), eps 1e-10, dt 0.004.  A step is one timestep, i.e. one
-``tea_leaf()`` solve from its first halo update to the finalised energy
-(the reference's "Tea Solve" time per step).  --warmup W timesteps run
-untimed first, then --steps K are timed; the deck runs W + K timesteps and
-the driver times each solve with a device synchronisation at its end.
+Workload c2 (default, the 2D 4096^2 configuration): the two-rectangle layout
+on a 10x10 domain, FP64 PPCG (30 CG presteps for the eigenvalue estimate,
+10 inner Chebyshev steps), eps 1e-10, dt 0.004.  A step is one timestep,
+i.e. one ``tea_leaf()`` solve from its first halo update to the finalised
+energy (the reference's "Tea Solve" time per step).  --warmup W timesteps
+run untimed first, then --steps K are timed; the deck runs W + K timesteps
+and the driver times each solve with a device synchronisation at its end.
+c3 is the 3D 256^3 cube, c4 the 2D 8192^2 layout of 32 seeded random
+rectangles; --solver picks CG, Chebyshev or Jacobi instead of PPCG.
 
-Printed: one JSON line with the mean seconds per timestep over the K timed
-steps, the roofline of the dominant kernel (its algorithmic bytes per launch
-over its mean launch time, from HIP events on the chunk's stream during the
-timed steps), and a CPU baseline from the oracle (see cpu_baseline()).
+Printed: one JSON line with the mean seconds per timestep of the FP64 solve
+over the K timed steps (`value`), the roofline of its dominant kernel (its
+algorithmic bytes per launch over its mean launch time, from HIP events on
+the chunk's stream during the timed steps), under `forms` the
+mixed-precision PPCG solve of the same deck (time, speedup, per-step error
+against the FP64 solve, true residual), and a CPU baseline from the oracle
+(see cpu_baseline()).
 """
 from __future__ import annotations
 
@@ -159,13 +164,81 @@ def kernel_probe(n: int, launches: int) -> dict:
     return {"probe": "ppcg_outer", "n": n, "launches": launches, "host_s_per_launch": dt}
 
 
+def run_solve(deck: str, dim: int, warmup: int, timing: bool = True) -> dict:
+    """Run a deck through the C++ driver; the timed steps follow `warmup`."""
+    L = tealeaf_hip.lib(dim)
+    L.tl_set_kernel_timing(warmup + 1 if timing else 0)
+    t0 = time.perf_counter()
+    rep = tealeaf_hip.run_deck(deck, dim=dim)
+    rep["wall_s"] = time.perf_counter() - t0
+    L.tl_set_kernel_timing(0)
+    rep["timed"] = rep["steps"][warmup:]
+    rep["s_per_step"] = statistics.mean(s["solve_s"] for s in rep["timed"])
+    return rep
+
+
+def kernel_table(rep: dict) -> dict:
+    cells = rep["cells"]
+    return {k: {"launches": v["count"], "avg_us": round(1e3 * v["ms"] / v["count"], 2),
+                "gbs": round(v["bytes_per_cell"] * cells / (v["ms"] / v["count"] * 1e-3) / 1e9, 1)}
+            for k, v in rep.get("kernels", {}).items()}
+
+
+def roofline_of(rep: dict, n: int) -> dict | None:
+    """Roofline of the kernel with the most time: algorithmic bytes per
+    launch over its mean HIP-event launch time."""
+    kern = rep.get("kernels", {})
+    if not kern:
+        return None
+    name, k = max(kern.items(), key=lambda kv: kv[1]["ms"])
+    avg_s = k["ms"] / k["count"] * 1e-3
+    bytes_per_launch = k["bytes_per_cell"] * rep["cells"]
+    achieved = bytes_per_launch / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(f"{name}@{n}")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": name,
+            "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
+
+
+def mixed_form(ref: dict, mix: dict) -> dict:
+    """The mixed-precision PPCG solve of the same deck against the FP64 one:
+    time per timestep and its ratio, and per timestep the relative
+    difference of every field-summary total from the FP64 solve, the true
+    residual ||u0 - A u||^2 the mixed solve ended with, iterations and
+    residual replacements."""
+    per_step = []
+    for a, b in zip(mix["timed"], ref["timed"]):
+        per_step.append({
+            "step": a["step"],
+            "temp_rel_err": abs(a["temp"] - b["temp"]) / abs(b["temp"]),
+            "max_summary_rel_err": max(abs(a[k] - b[k]) / abs(b[k]) for k in ("vol", "mass", "ie", "temp")),
+            "true_rrn": a["exact_error"], "iters": a["iters"], "fp64_iters": b["iters"],
+            "replacements": a["replacements"], "fallback": a["mixed_fallback"],
+        })
+    return {
+        "s_per_step": round(mix["s_per_step"], 6),
+        "speedup_vs_fp64": round(ref["s_per_step"] / mix["s_per_step"], 3),
+        "max_temp_rel_err": max(p["temp_rel_err"] for p in per_step),
+        "per_step": per_step,
+        "kernels": kernel_table(mix),
+        "roofline": roofline_of(mix, mix["x_cells"]),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=9, help="timed timesteps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps before them")
-    ap.add_argument("--n", type=int, default=4096, help="cells per side")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: 2D 4096^2 two rectangles; c3: 3D 256^3 cube; c4: 2D 8192^2, 32 random rectangles")
+    ap.add_argument("--n", type=int, default=None, help="cells per side (default 4096, 256 for c3, 8192 for c4)")
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
+    ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision PPCG solve")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -173,16 +246,17 @@ def main() -> int:
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     args = ap.parse_args()
+    n = args.n or {"c2": 4096, "c3": 256, "c4": 8192}[args.workload]
 
     if args.copy_probe:
         L = tealeaf_hip.lib(2)
         L.tl_copy_probe.restype = __import__("ctypes").c_double
-        n = 1 << 28  # 2 GiB per buffer
-        print(json.dumps({"probe": "copy", "doubles": n, "bytes_each_way": 8 * n,
-                          "gbs": L.tl_copy_probe(n, 10)}), flush=True)
+        m = 1 << 28  # 2 GiB per buffer
+        print(json.dumps({"probe": "copy", "doubles": m, "bytes_each_way": 8 * m,
+                          "gbs": L.tl_copy_probe(m, 10)}), flush=True)
         return 0
     if args.kernel_probe:
-        print(json.dumps(kernel_probe(args.n, 20)), flush=True)
+        print(json.dumps(kernel_probe(n, 20)), flush=True)
         return 0
 
     if args.gpus != 1:
@@ -194,46 +268,42 @@ def main() -> int:
         return 2
 
     total = args.warmup + args.steps
-    deck = decks.two_rect_2d(args.n, args.solver, steps=total, presteps=30, inner=10,
-                             max_iters=10000 if args.solver != "jacobi" else 20000)
-    L = tealeaf_hip.lib(2)
-    L.tl_set_kernel_timing(args.warmup + 1)
-    t0 = time.perf_counter()
-    rep = tealeaf_hip.run_deck(deck, dim=2)
-    wall = time.perf_counter() - t0
-    L.tl_set_kernel_timing(0)
+    max_iters = 10000 if args.solver != "jacobi" else 20000
+    extra = ("tl_check_result",)
+    dim = 3 if args.workload == "c3" else 2
+    if args.workload == "c2":
+        deck = decks.two_rect_2d(n, args.solver, steps=total, presteps=30, inner=10, max_iters=max_iters,
+                                 extra=extra)
+        what = f"2D {n}x{n} {args.solver.upper()} two-rectangle deck"
+    elif args.workload == "c3":
+        deck = decks.cube_3d(n, args.solver, steps=total, presteps=30, inner=10, max_iters=max_iters, extra=extra)
+        what = f"3D {n}^3 {args.solver.upper()} hot-box/cuboid deck"
+    else:
+        deck = decks.random_rects_2d(n, args.seed, steps=total, solver=args.solver, presteps=30, inner=10,
+                                     max_iters=max_iters, extra=extra)
+        what = f"2D {n}x{n} {args.solver.upper()} 32 random rectangles (splitmix64 seed {args.seed})"
 
-    timed = rep["steps"][args.warmup:]
-    solve = [s["solve_s"] for s in timed]
-    s_per_step = statistics.mean(solve)
-    cells = rep["cells"]
-
-    kern = rep.get("kernels", {})
-    dom_name, dom = max(kern.items(), key=lambda kv: kv[1]["ms"]) if kern else (None, None)
-    roofline = None
-    if dom:
-        avg_s = dom["ms"] / dom["count"] * 1e-3
-        bytes_per_launch = dom["bytes_per_cell"] * cells
-        achieved = bytes_per_launch / avg_s / 1e9
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            with open(TRAFFIC_FILE) as f:
-                traffic = json.load(f).get(f"{dom_name}@{args.n}")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": dom_name, "avg_launch_us": round(avg_s * 1e6, 2),
-                    "bytes_per_launch": bytes_per_launch}
+    print(f"bench: {what}, {args.warmup} + {args.steps} timesteps", file=sys.stderr, flush=True)
+    ref = run_solve(deck, dim, args.warmup)
+    forms = {}
+    if dim == 2 and args.solver == "ppcg" and not args.no_mixed:
+        print("bench: mixed-precision PPCG on the same deck", file=sys.stderr, flush=True)
+        mix = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg_mixed"), dim, args.warmup)
+        forms["mixed_ppcg"] = mixed_form(ref, mix)
 
     import ctypes
+    L = tealeaf_hip.lib(2)
     L.tl_copy_probe.restype = ctypes.c_double
     copy_gbs = L.tl_copy_probe(1 << 27, 10)
 
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.n, args.cpu_threads, timed)
+    if not args.no_cpu_baseline and args.workload == "c2" and args.solver == "ppcg":
+        cpu = cpu_baseline(n, args.cpu_threads, ref["timed"])
 
+    timed = ref["timed"]
+    s_per_step = ref["s_per_step"]
     out = {
-        "metric": "solve s/timestep @eps 1e-10 (FP64 PPCG); HBM GB/s",
+        "metric": "solve s/timestep @eps 1e-10 (learned-FCG, mixed PPCG vs FP64 PPCG); HBM GB/s",
         "value": round(s_per_step, 6),
         "unit": "s/timestep",
         "n_gpus": 1,
@@ -245,19 +315,18 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"2D {args.n}x{args.n} {args.solver.upper()} two-rectangle deck, eps 1e-10, "
-                               f"dt 0.004, presteps 30, inner steps 10",
-                   "cells": cells, "parallelism": "single chunk, one GPU"},
-        "median_s_per_step": round(statistics.median(solve), 6),
+        "config": {"workload": f"{what}, FP64, eps 1e-10, dt 0.004, presteps 30, inner steps 10",
+                   "cells": ref["cells"], "parallelism": "single chunk, one GPU"},
+        "median_s_per_step": round(statistics.median(s["solve_s"] for s in timed), 6),
         "iterations": [s["iters"] for s in timed],
         "temp": [s["temp"] for s in timed],
-        "step1_s": rep["steps"][0]["solve_s"] if args.warmup else None,
-        "wall_s": round(wall, 3),
-        "kernels": {k: {"launches": v["count"], "avg_us": round(1e3 * v["ms"] / v["count"], 2),
-                        "gbs": round(v["bytes_per_cell"] * cells / (v["ms"] / v["count"] * 1e-3) / 1e9, 1)}
-                    for k, v in kern.items()},
+        "true_rrn": [s["exact_error"] for s in timed],
+        "step1_s": ref["steps"][0]["solve_s"] if args.warmup else None,
+        "wall_s": round(ref["wall_s"], 3),
+        "kernels": kernel_table(ref),
         "copy_gbs": round(copy_gbs, 1),
-        "roofline": roofline,
+        "roofline": roofline_of(ref, n),
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

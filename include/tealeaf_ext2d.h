@@ -121,6 +121,14 @@ void tl_set_kernel_timing(int from_step);
 int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
                         int nsteps, double* rrn);
 
+/* Mixed-precision form of tl_ppcg_outer_fused (no counterpart in the
+ * reference): the nsteps inner steps in FP32 on the FP32 rounding of the
+ * residual, u and r updated in FP64 (u += alpha p + sum sd, r = r' + the FP32
+ * change of the residual), ||r||^2 in FP64.  2D, no preconditioner, nsteps
+ * 1, 3 or 10; otherwise returns -1 with tl_last_error() set. */
+int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
+                        int nsteps, double* rrn);
+
 /* GB/s (read + write) of a plain device copy of n doubles repeated reps
  * times: the attainable streaming rate, and the known-size kernel that
  * calibrates HBM byte counters.  Negative on error. */

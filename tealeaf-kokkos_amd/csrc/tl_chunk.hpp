@@ -50,7 +50,7 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 // Kernels timed by the optional per-launch event timer.
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
-    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_COUNT
+    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -105,16 +105,26 @@ struct Chunk {
     double* p_alt = nullptr;   // ping-pong partner of p (CGCalcP folded into CGCalcW)
     double* r_alt = nullptr;   // ping-pong partner of r (temporal-blocked PPCG)
 
+    // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
+    // elements) and of the inner-step coefficients, made at the switch.
+    float* kx32 = nullptr;
+    float* ky32 = nullptr;
+    float* d_coef32 = nullptr;  // a[0..n) then b[0..n)
+    int mixed_steps = 0;
+    double inv_theta = 0.0;
+
     // Mesh coordinates (device, reference indexing).
     double *vertexX = nullptr, *vertexY = nullptr, *vertexZ = nullptr;
     double *cellX = nullptr, *cellY = nullptr, *cellZ = nullptr;
 
-    // Chebyshev / PPCG coefficients, copied from the caller at init; the
-    // device copy holds, per block of up to 12 steps starting at step k,
-    // a[k..k+n) followed by b[k..k+n) at offset 2k.
+    // Chebyshev / PPCG coefficients, copied from the caller at init.  The
+    // device copy serves the temporal-blocked PPCG passes: for the pass that
+    // starts at inner step k with m levels, a[k..k+m) then b[k..k+m) at
+    // offset 2k; laid out for d_coef_steps inner steps (0: not yet).
     std::vector<double> alphas, betas;
     double* d_coef = nullptr;
     int d_coef_cap = 0;
+    int d_coef_steps = 0;
 
     // Reduction scratch and device-resident solver scalars.
     double* partials = nullptr;
@@ -179,8 +189,23 @@ double stream_copy_probe(long n, int reps);   // GB/s of a plain copy of n doubl
 // Temporal-blocked PPCG outer iteration (tl_ppcg_tb.hip): after CGCalcW,
 // CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep.
 // Returns the number of rrn partials; op_finish_rrn closes the reduction.
+// Output rows per wave of a streaming pass with L levels over tiles_x
+// column tiles, for `slots` resident waves.
+int tb_rows(int ny, int tiles_x, int L, int slots);
+// Level counts of the passes of a temporal-blocked PPCG outer iteration
+// with the given inner steps (each a compiled pass size).
+std::vector<int> tb_passes(int inner_steps);
 bool ppcg_tb_supported(const Chunk& c, int inner_steps);
 long op_ppcg_outer_tb(Chunk& c, int inner_steps);
+
+// Mixed-precision PPCG (tl_ppcg_mixed.hip): FP32 inner steps inside the
+// FP64 outer iteration.  prepare makes the FP32 coefficient copies after
+// op_ppcg_init; the outer op mirrors op_ppcg_outer_tb.
+bool ppcg_mixed_supported(const Chunk& c, int inner_steps);
+void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int inner_steps, double theta);
+long op_ppcg_outer_mixed(Chunk& c, int inner_steps);
+// rrn = ||r||^2 of a replaced residual; beta from the rro before this iteration
+void op_2norm_replace_dev(Chunk& c);
 void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b);  // 1..n
 
 // Chebyshev, PPCG and Jacobi come in two forms.  The plain form keeps the

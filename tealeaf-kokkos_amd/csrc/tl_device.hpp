@@ -127,7 +127,8 @@ struct Reduce {
 // recurrences without a host round trip (same IEEE divisions as the host).
 struct DevScalars {
     double rro, pw, alpha, rrn, beta, error;
-    double pad[2];
+    double rro_prev;  // rro before the last rrn epilogue (a replaced residual recomputes beta from it)
+    double pad;
 };
 
 template <int NV>

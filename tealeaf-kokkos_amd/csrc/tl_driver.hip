@@ -50,6 +50,10 @@ struct Deck {
     int ppcg_inner_steps = 10;
     bool preconditioner = false;
     bool use_cheby = false, use_cg = false, use_ppcg = false, use_jacobi = true;
+    // Mixed-precision PPCG (no counterpart in the reference): FP32 inner
+    // steps, the FP64 residual replaced once ||r||^2 fell by this factor.
+    bool ppcg_mixed = false;
+    double mixed_replace = 1.0e-4;
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -197,10 +201,12 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_check_result") d.check_result = true;
             else if (k == "tl_ch_cg_errswitch") d.errswitch = true;
             else if (k == "tl_preconditioner_on") d.preconditioner = true;
-            else if (k == "tl_use_jacobi") { d.use_cheby = d.use_cg = d.use_ppcg = false; d.use_jacobi = true; }
-            else if (k == "tl_use_cg") { d.use_cheby = d.use_ppcg = d.use_jacobi = false; d.use_cg = true; }
-            else if (k == "tl_use_ppcg") { d.use_cheby = d.use_cg = d.use_jacobi = false; d.use_ppcg = true; }
-            else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = false; d.use_cheby = true; }
+            else if (k == "tl_use_jacobi") { d.use_cheby = d.use_cg = d.use_ppcg = d.ppcg_mixed = false; d.use_jacobi = true; }
+            else if (k == "tl_use_cg") { d.use_cheby = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = false; d.use_cg = true; }
+            else if (k == "tl_use_ppcg") { d.use_cheby = d.use_cg = d.use_jacobi = d.ppcg_mixed = false; d.use_ppcg = true; }
+            else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = false; d.use_ppcg = d.ppcg_mixed = true; }
+            else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
+            else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();
             else if (k == "tl_coefficient_density") d.coefficient = CONDUCTIVITY;
@@ -345,6 +351,10 @@ void calc_ch_coefs(std::vector<double>& ca, std::vector<double>& cb, double emin
 struct StepInfo {
     int iters = 0, switch_step = 0, est_itc = 0, cg_steps = 0, cheby_steps = 0;
     double error = 0, exact_error = 0, eigmin = 0, eigmax = 0, solve_s = 0;
+    // mixed-precision PPCG: FP32 outer iterations, residual replacements,
+    // whether the FP64 inner steps took over
+    int mixed_steps = 0, replacements = 0;
+    bool mixed_fallback = false;
 };
 
 const int kAllExternal[6] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
@@ -373,6 +383,11 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
     int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
     bool p_pending = false;
+    // mixed-precision PPCG state: rrn at the last residual replacement, and
+    // true-residual checks that failed without halving the true rrn
+    bool mixed = d.use_ppcg && d.ppcg_mixed && ppcg_mixed_supported(c, d.ppcg_inner_steps);
+    double rrn_replaced = 0.0, best_true = HUGE_VAL;
+    int stalls = 0;
     std::vector<double> cg_alphas, cg_betas;
     std::vector<double> ch_alphas(max_iters + 3, 0.0), ch_betas(max_iters + 3, 0.0);
     const bool want_history = d.use_cheby || d.use_ppcg;
@@ -434,10 +449,40 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     }
                     op_calc_residual(c);
                     switch_step = n;
+                    if (mixed) {
+                        op_mixed_prepare(c, &ch_alphas[1], &ch_betas[1], d.ppcg_inner_steps, theta);
+                        rrn_replaced = error;
+                    }
                 }
                 // rro on the device is the last rrn (= error), as at tea_solve.f90:187,304
                 op_cg_calc_w_dev(c, p_pending);
-                if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
+                if (mixed) {
+                    TL_TIMED(c, K_PPCG_MIXED, op_finish_rrn(c, op_ppcg_outer_mixed(c, d.ppcg_inner_steps)));
+                    ++si.mixed_steps;
+                    p_pending = true;
+                    error = op_error(c);
+                    const bool converged = std::fabs(error) < d.eps;
+                    if (converged || error < rrn_replaced * d.mixed_replace) {
+                        // FP64 residual replacement r = u0 - A u; beta is
+                        // recomputed from it.  Convergence is accepted only
+                        // on the true residual.
+                        op_calc_residual(c);
+                        op_2norm_replace_dev(c);
+                        error = op_error(c);
+                        rrn_replaced = error;
+                        ++si.replacements;
+                        if (converged && !(std::fabs(error) < d.eps)) {
+                            // the FP32 steps no longer reduce the true
+                            // residual: finish with the FP64 inner steps
+                            stalls = error > 0.5 * best_true ? stalls + 1 : 0;
+                            best_true = std::min(best_true, error);
+                            if (stalls >= 3) {
+                                mixed = false;
+                                si.mixed_fallback = true;
+                            }
+                        }
+                    }
+                } else if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
                     TL_TIMED(c, K_PPCG_OUTER, op_finish_rrn(c, op_ppcg_outer_tb(c, d.ppcg_inner_steps)));
                 } else {
@@ -534,7 +579,8 @@ std::string run_deck(const std::string& text, int max_steps)
     o.f("{\"dim\": %d, \"x_cells\": %d, \"y_cells\": %d, \"z_cells\": %d, \"device\": \"hip\", ", TL_DIM, d.x_cells,
         d.y_cells, nz);
     o.f("\"solver\": \"%s\", \"eps\": %.17g, \"dtinit\": %.17g, \"max_iters\": %d, ",
-        d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? "ppcg" : "jacobi", d.eps, d.dtinit, d.max_iters);
+        d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? (d.ppcg_mixed ? "ppcg_mixed" : "ppcg") : "jacobi",
+        d.eps, d.dtinit, d.max_iters);
     o.f("\"initial\": {");
     summary(o, *c, 0, 0.0);
     o.f("}, \"steps\": [");
@@ -553,9 +599,9 @@ std::string run_deck(const std::string& text, int max_steps)
         o.f("%s{", step > 1 ? ", " : "");
         o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
             "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
-            "\"dt\": %.17g, ",
+            "\"dt\": %.17g, \"mixed_steps\": %d, \"replacements\": %d, \"mixed_fallback\": %d, ",
             si.iters, si.error, si.exact_error, si.switch_step, si.est_itc, si.eigmin, si.eigmax, si.cg_steps,
-            si.cheby_steps, si.solve_s, dt);
+            si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0);
         if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
             summary(o, *c, step, time);
         else

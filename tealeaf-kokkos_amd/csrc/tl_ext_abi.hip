@@ -113,6 +113,28 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
     return rc;
 }
 
+// The mixed-precision form of the same outer iteration: FP32 inner steps,
+// u and r updated in FP64 (2D only; nsteps with a compiled kernel).
+int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
+                        int nsteps, double* rrn)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = chunk_at(chunk);
+        if (!ppcg_mixed_supported(*c, nsteps)) throw Error("mixed-precision PPCG does not apply to this chunk");
+        DevScalars s{};
+        s.rro = 1.0;
+        s.alpha = alpha;
+        TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+        op_ppcg_init(*c, alphas, betas, nsteps, theta);
+        op_mixed_prepare(*c, alphas, betas, nsteps, theta);
+        op_finish_rrn(*c, op_ppcg_outer_mixed(*c, nsteps));
+        *rrn = op_error(*c);
+        rc = 0;
+    });
+    return rc;
+}
+
 // Bandwidth of a plain device copy of n doubles repeated reps times (GB/s,
 // read + write): the machine's attainable streaming rate, and the known-size
 // kernel that calibrates HBM byte counters.  Returns a negative value on error.

@@ -537,7 +537,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, cons
 // Second phase of every reduction: one workgroup adds the block partials in
 // block order and applies the scalar recurrence of tea_solve.f90 that
 // consumes the total, so the next kernel can read it from `ds`.
-enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN };
+enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE };
 
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
@@ -576,6 +576,16 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
             hist_a[hist_n] = ds->alpha;
             hist_b[hist_n] = beta;
         }
+        ds->rro_prev = ds->rro;
+        ds->rro = v[0];
+        ds->error = v[0];
+        out[1] = beta;
+        break;
+    }
+    case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
+        const double beta = v[0] / ds->rro_prev;
+        ds->rrn = v[0];
+        ds->beta = beta;
         ds->rro = v[0];
         ds->error = v[0];
         out[1] = beta;
@@ -844,7 +854,7 @@ const char* kernel_name(int k)
     static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_p_w", "cg_calc_ur", "cg_calc_p",
                                          "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
-                                         "field_summary", "update_halo"};
+                                         "field_summary", "update_halo", "ppcg_outer_mixed"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -866,6 +876,7 @@ double kernel_bytes_per_cell(int k)
     case K_COPY_U: return 16.0;
     case K_FINALISE: return 24.0;
     case K_SUMMARY: return 32.0;
+    case K_PPCG_MIXED: return 48.0 + kb / 2;  // u, p, r, w, FP32 coefficients -> u, r
     default: return 0.0;
     }
 }
@@ -1020,6 +1031,9 @@ Chunk::~Chunk()
     (void)hipFree(sd_alt);
     (void)hipFree(p_alt);
     (void)hipFree(r_alt);
+    (void)hipFree(kx32);
+    (void)hipFree(ky32);
+    (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);
     (void)hipFree(hist_a);
@@ -1292,6 +1306,12 @@ double op_error(Chunk& c) { return c.result(S_RRN); }
 
 void op_finish_rrn(Chunk& c, long nblocks) { finish<1>(c, nblocks, S_RRN, EP_RRN); }
 
+void op_2norm_replace_dev(Chunk& c)
+{
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN_REPLACE);
+}
+
 double stream_copy_probe(long n, int reps)
 {
     if (n < 2 || reps < 1) throw Error("copy probe needs n >= 2 and reps >= 1");
@@ -1340,27 +1360,6 @@ static void store_coefs(Chunk& c, const double* alphas, const double* betas, int
 // Device copy for the fused PPCG passes (tl_ppcg_tb.hip): up to 12 steps
 // run as one pass; more run as passes of 10 and a remainder.  The pass that
 // starts at step k finds a[k..k+m) and then b[k..k+m) from offset 2k.
-static void store_device_coefs(Chunk& c, int n)
-{
-    const int need = 2 * n + 24;
-    if (need > c.d_coef_cap) {
-        c.sync();
-        (void)hipFree(c.d_coef);
-        c.d_coef = nullptr;
-        TL_HIP(hipMalloc(&c.d_coef, sizeof(double) * need));
-        c.d_coef_cap = need;
-    }
-    std::vector<double> h(need, 0.0);
-    for (int k = 0; k < n; k += (n <= 12 ? n : 10)) {
-        const int m = n <= 12 ? n : std::min(10, n - k);
-        for (int q = 0; q < m; ++q) {
-            h[2 * k + q] = c.alphas[k + q];
-            h[2 * k + m + q] = c.betas[k + q];
-        }
-    }
-    TL_HIP(hipMemcpy(c.d_coef, h.data(), sizeof(double) * need, hipMemcpyHostToDevice));
-}
-
 // ext_core.cpp:468-497
 void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, double theta, bool preconditioner,
                    bool fused)
@@ -1403,7 +1402,7 @@ void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, do
 {
     c.theta = theta;
     store_coefs(c, alphas, betas, n);
-    store_device_coefs(c, n);
+    c.d_coef_steps = 0;  // the temporal-blocked passes lay their coefficients out again
 }
 
 // ext_core.cpp:543-550

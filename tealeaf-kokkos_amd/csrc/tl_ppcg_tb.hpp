@@ -1,0 +1,445 @@
+// Temporal-blocked PPCG outer iteration for the 2D 5-point stencil: the
+// device code, instantiated per pass size in tl_ppcg_tb_l*.hip.
+//
+// One PPCG outer iteration of the reference (tea_solve.f90:307-374) makes,
+// after CGCalcW, these full sweeps: CGCalcUr (u += alpha p, r -= alpha w),
+// PPCGInitSd (sd = r / theta), L inner steps of PPCGCalcU + PPCGCalcSd
+// (r -= A sd; u += sd; sd = a_s sd + b_s r; ext_ppcg_kernel.hpp:37-100) and
+// Calculate2Norm (rrn = r.r).  None of them needs a global result, so one
+// kernel does all of them in a single pass over HBM: a wavefront streams
+// down a strip of rows and keeps, per inner level, a three-row register
+// window of sd and a two-row window of r; level s at row y - s is updated as
+// soon as level s - 1 has produced row y - s + 1.  x-neighbours come from
+// the adjacent lanes through DPP row shifts; each level therefore consumes
+// one halo column per side, so a 64-lane wave writes 64 - 2L columns.
+//
+// Every cell value is produced by the reference's expressions in the
+// reference's order (calc_ur, then init_sd, then step after step), so u and
+// r are bitwise those of the unfused sweeps; only rrn's summation order is
+// the kernel's own.  HBM traffic per outer iteration drops from
+// 48+16+L*(40+24 + 16)+8 bytes per cell (the reference's sweeps) to the
+// kernel's 48 + 16 (read u, p, r, w, kx, ky; write u, r).
+#pragma once
+
+#include "tl_chunk.hpp"
+
+#include <algorithm>
+#include <string>
+
+namespace tl {
+
+#if TL_DIM == 2
+namespace tb {
+
+struct TBArgs {
+    double* u;
+    const double* p;
+    const double* r_in;
+    const double* w;
+    const double* sd_in;
+    double* r_out;
+    double* sd_out;
+    const double* kx;
+    const double* ky;
+    const DevScalars* ds;
+    double theta;
+    const double* coef;  // a[0..L) then b[0..L) of this pass (device)
+    double* partials;
+    int tiles_x;
+    int rows;  // output rows per wave
+    long nblocks;
+};
+
+// Lane i receives lane i-1 (x-neighbour at kk-1); lane 0 receives 0.
+// Lane 0 and lane 63 either sit on a face of the mesh, where the clamps
+// replace the shifted value, or own no output cell, so the fill never
+// reaches a result.
+__device__ __forceinline__ double from_west(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true));
+}
+
+// Lane i receives lane i+1 (x-neighbour at kk+1); lane 63 receives 0.
+__device__ __forceinline__ double from_east(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true));
+}
+
+// Per-wave state of the streaming pass.  Each lane owns two adjacent
+// columns (c0 = X0 - L + 2 lane, c1 = c0 + 1), so a wave spans 128 columns
+// and writes the 128 - 2L in its middle.  Register windows rotate by round
+// number instead of moving data: within a block of six rounds the slot of
+// every window is a compile-time constant.  The windows that span L+1 rows
+// (u being accumulated, kx, ky) live in LDS rings indexed by row.
+template <int L>
+struct TBState {
+    double sdw[L][3][2];  // sd of level s, slot (round - ylo) % 3, column
+    double rw[L][2][2];   // r of level s, slot (round - ylo) % 2, column
+    double pf[2][6][2];   // prefetched rows: r, w|sd, u, p, kx, ky
+    double acc;
+};
+
+struct TBCtx {
+    Geom g;
+    int nx, ny, ylo, Y0, yhi;
+    int kc0, kc1, kf0, kf1;  // clamped columns (cells; kx faces up to nx + 2)
+    // byte offsets within a row: cells, kx faces, store targets (the cell
+    // for lanes that own one, else a pad column past the halo that nothing
+    // reads) and the pad column alone (rows that own no output)
+    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
+    bool wcl0, wcl1, ecl0, ecl1;
+    bool out0, out1, out_any;
+    double alpha, theta;
+    double* ur;  // LDS rings [L+1][64][2]
+    double* kxr;
+    double* kyr;
+    const double* coef;  // LDS copy of a[0..L), b[0..L)
+};
+
+// Loads of row y into a prefetch slot.  Every load is issued on every
+// round, with the row clamped into the mesh (rows past the top repeat the
+// last one; u and p are only used on the wave's output rows), so that the
+// number of memory operations in flight is the same on every path and the
+// compiler can wait for exactly the oldest slot instead of draining all.
+// Row y of an array, as a wave-uniform pointer.
+__device__ __forceinline__ const double* row_of(const double* a, const Geom& g, int y)
+{
+    return a + g.base + (long)y * g.sx;
+}
+
+__device__ __forceinline__ double ld(const double* row, unsigned boff)
+{
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row) + boff);
+}
+
+__device__ __forceinline__ void st(double* row, unsigned boff, double v)
+{
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
+}
+
+// Loads of row y into a prefetch slot.  Every load is issued on every
+// round, with the row clamped into the mesh (rows past the top repeat the
+// last one; u and p are only used on the wave's output rows), so that the
+// number of memory operations in flight is the same on every path and the
+// compiler can wait for exactly the oldest slot instead of draining all.
+// Addresses are a uniform row pointer plus a per-lane 32-bit byte offset.
+template <bool FIRST>
+__device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
+{
+    const int yr = min(y, c.ny + 1);
+    const int yk = min(y, c.ny + 2);  // ky also on the top face
+    const int yo = min(max(y, c.Y0), c.yhi);
+    const double* r = row_of(A.r_in, c.g, yr);
+    v[0][0] = ld(r, c.bc0);
+    v[0][1] = ld(r, c.bc1);
+    const double* b = row_of(FIRST ? A.w : A.sd_in, c.g, yr);
+    v[1][0] = ld(b, c.bc0);
+    v[1][1] = ld(b, c.bc1);
+    const double* u = row_of(A.u, c.g, yo);
+    v[2][0] = ld(u, c.bc0);
+    v[2][1] = ld(u, c.bc1);
+    if (FIRST) {
+        const double* p = row_of(A.p, c.g, yo);
+        v[3][0] = ld(p, c.bc0);
+        v[3][1] = ld(p, c.bc1);
+    }
+    const double* kx = row_of(A.kx, c.g, yr);
+    v[4][0] = ld(kx, c.bf0);
+    v[4][1] = ld(kx, c.bf1);
+    const double* ky = row_of(A.ky, c.g, yk);
+    v[5][0] = ld(ky, c.bc0);
+    v[5][1] = ld(ky, c.bc1);
+}
+
+__device__ __forceinline__ int ring(int row, int n)
+{
+    const int m = row % n;
+    return m < 0 ? m + n : m;
+}
+
+// One round: level 0 at row y, then level s at row y - s for s = 1..L.
+// XE: the wave reaches an x face (per-lane clamps); YE: it reaches the
+// bottom or top face (row clamps and rows past the top).  The LDS rings of
+// level s + 1 are read before level s computes, so their latency overlaps
+// level s's arithmetic (the rows differ, so no write of level s is missed).
+template <int L, bool FIRST, bool LAST, bool XE, bool YE, int PH>
+__device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y)
+{
+    constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
+    constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
+    const int lane = threadIdx.x;
+    const int ny = c.ny;
+    double(&v)[6][2] = st.pf[PH % 2];
+    double2* ur = reinterpret_cast<double2*>(c.ur);
+    double2* kxr = reinterpret_cast<double2*>(c.kxr);
+    double2* kyr = reinterpret_cast<double2*>(c.kyr);
+
+    // inputs of level 1 (row y - 1)
+    int rn = ring(y - 1, L + 1);
+    double2 nkx = kxr[rn * WAVE + lane], nky = kyr[rn * WAVE + lane], nu = ur[rn * WAVE + lane];
+    double na = c.coef[0], nb = c.coef[L];
+
+    // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
+    const int ry = ring(y, L + 1);
+    double kyn0, kyn1;
+    if (!YE || y <= ny + 1) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            double r0, sd0;
+            if (FIRST) {
+                r0 = v[0][q] - c.alpha * v[1][q];
+                sd0 = r0 / c.theta;
+            } else {
+                r0 = v[0][q];
+                sd0 = v[1][q];
+            }
+            st.sdw[0][S0][q] = sd0;
+            st.rw[0][R0][q] = r0;
+        }
+        double2 u0 = make_double2(0.0, 0.0);
+        if (y >= c.Y0 && y <= c.yhi) {
+            u0.x = FIRST ? v[2][0] + c.alpha * v[3][0] : v[2][0];
+            u0.y = FIRST ? v[2][1] + c.alpha * v[3][1] : v[2][1];
+        }
+        ur[ry * WAVE + lane] = u0;
+        kxr[ry * WAVE + lane] = make_double2(v[4][0], v[4][1]);
+        kyr[ry * WAVE + lane] = make_double2(v[5][0], v[5][1]);
+        kyn0 = v[5][0];
+        kyn1 = v[5][1];
+    } else {
+        // beyond the top face: the reflective mirror repeats the last row
+        const bool face = y == ny + 2;
+        kyn0 = face ? v[5][0] : 0.0;
+        kyn1 = face ? v[5][1] : 0.0;
+        kyr[ry * WAVE + lane] = make_double2(kyn0, kyn1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            st.sdw[0][S0][q] = st.sdw[0][S1][q];
+            st.rw[0][R0][q] = st.rw[0][R1][q];
+        }
+    }
+    // prefetch row y + 2 into the slot just consumed
+    tb_load<FIRST>(A, c, y + 2, v);
+
+#pragma unroll
+    for (int s = 1; s <= L; ++s) {
+        const int z = y - s;
+        const int rz = rn;
+        const double2 kx = nkx, ky = nky;
+        double2 u = nu;
+        const double a = na, b = nb;
+        if (s < L) {
+            rn = ring(z - 1, L + 1);
+            nkx = kxr[rn * WAVE + lane];
+            nky = kyr[rn * WAVE + lane];
+            nu = ur[rn * WAVE + lane];
+            na = c.coef[s];
+            nb = c.coef[L + s];
+        }
+        // level L's results (zero where the level is beyond the top face)
+        double rn0 = 0.0, rn1 = 0.0;
+        if (!YE || z <= ny + 1) {
+            const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
+            double s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
+            double n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
+            if (YE) {
+                if (z == HALO) { s0 = c0; s1 = c1; }
+                if (z == ny + 1) { n0 = c0; n1 = c1; }
+            }
+            // x-neighbours: c0's west is the previous lane's c1, c1's east the next lane's c0
+            double w0 = from_west(c1), e1 = from_east(c0);
+            double e0 = c1, w1 = c0;
+            if (XE) {
+                if (c.wcl0) w0 = c0;
+                if (c.wcl1) w1 = c1;
+                if (c.ecl0) e0 = c0;
+                if (c.ecl1) e1 = c1;
+            }
+            const double kxe1 = from_east(kx.x);
+            // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z)
+            const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
+            const double m1 = (1.0 + (kxe1 + kx.y) + (kyn1 + ky.y)) * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
+            kyn0 = ky.x;
+            kyn1 = ky.y;
+            // PPCGCalcU: r -= A sd; u += sd
+            rn0 = st.rw[s - 1][R1][0] - m0;
+            rn1 = st.rw[s - 1][R1][1] - m1;
+            u.x = u.x + c0;
+            u.y = u.y + c1;
+            if (s < L) {
+                // PPCGCalcSd with the coefficients of inner step s
+                st.sdw[s][S0][0] = a * c0 + b * rn0;
+                st.sdw[s][S0][1] = a * c1 + b * rn1;
+                st.rw[s][R0][0] = rn0;
+                st.rw[s][R0][1] = rn1;
+                ur[rz * WAVE + lane] = u;
+            }
+        } else {
+            kyn0 = ky.x;
+            kyn1 = ky.y;
+            if (s < L) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    st.sdw[s][S0][q] = st.sdw[s][S1][q];
+                    st.rw[s][R0][q] = st.rw[s][R1][q];
+                }
+            }
+        }
+        if (s == L) {
+            // output row: every lane stores, lanes and rows that own no
+            // cell into a pad column, so the store count never depends on a branch
+            const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
+            const bool o0 = zin && c.out0, o1 = zin && c.out1;
+            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            const int zs = min(max(z, 0), ny + 2 * HALO - 1);
+            const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
+            double* ro = const_cast<double*>(row_of(A.r_out, c.g, zs));
+            double* uo = const_cast<double*>(row_of(A.u, c.g, zs));
+            tb::st(ro, b0, rn0);
+            tb::st(uo, b0, u.x);
+            tb::st(ro, b1, rn1);
+            tb::st(uo, b1, u.y);
+            if (!LAST) {
+                double* so = const_cast<double*>(row_of(A.sd_out, c.g, zs));
+                tb::st(so, b0, a * c0 + b * rn0);
+                tb::st(so, b1, a * c1 + b * rn1);
+            }
+            if (LAST) {
+                st.acc += o0 ? rn0 * rn0 : 0.0;
+                st.acc += o1 ? rn1 * rn1 : 0.0;
+            }
+        }
+    }
+}
+
+template <int L, bool FIRST, bool LAST, bool XE, bool YE>
+__device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
+{
+    TBState<L> st;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            st.sdw[s][0][q] = st.sdw[s][1][q] = st.sdw[s][2][q] = 0.0;
+            st.rw[s][0][q] = st.rw[s][1][q] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        st.pf[0][k][0] = st.pf[0][k][1] = st.pf[1][k][0] = st.pf[1][k][1] = 0.0;
+    }
+    st.acc = 0.0;
+    tb_load<FIRST>(A, c, c.ylo, st.pf[0]);
+    tb_load<FIRST>(A, c, c.ylo + 1, st.pf[1]);
+    // rounds run in blocks of six; rounds past the last output row only
+    // touch windows, never memory outside the mesh
+    const int yend = c.yhi + L;
+    for (int y = c.ylo; y <= yend; y += 6) {
+        tb_round<L, FIRST, LAST, XE, YE, 0>(A, c, st, y);
+        tb_round<L, FIRST, LAST, XE, YE, 1>(A, c, st, y + 1);
+        tb_round<L, FIRST, LAST, XE, YE, 2>(A, c, st, y + 2);
+        tb_round<L, FIRST, LAST, XE, YE, 3>(A, c, st, y + 3);
+        tb_round<L, FIRST, LAST, XE, YE, 4>(A, c, st, y + 4);
+        tb_round<L, FIRST, LAST, XE, YE, 5>(A, c, st, y + 5);
+    }
+    if (LAST) {
+        double v[1] = {st.acc};
+        block_partial<1, 1>(v, Reduce{A.partials});
+    }
+}
+
+template <int L, bool FIRST, bool LAST>
+__global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
+{
+    __shared__ double2 u_ring[(L + 1) * WAVE];
+    __shared__ double2 kx_ring[(L + 1) * WAVE];
+    __shared__ double2 ky_ring[(L + 1) * WAVE];
+    __shared__ double coef[2 * L];
+    const int lane = threadIdx.x;
+    if (lane < 2 * L) coef[lane] = A.coef[lane];
+    __syncthreads();
+    const long t = xcd_tile(blockIdx.x, A.nblocks);
+    const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
+    constexpr int OW = 2 * WAVE - 2 * L;
+    TBCtx c;
+    c.g = g;
+    c.nx = g.nx;
+    c.ny = g.ny;
+    const int X0 = HALO + tx * OW;
+    c.Y0 = HALO + ty * A.rows;
+    const int k0 = X0 - L + 2 * lane, k1 = k0 + 1;
+    c.kc0 = min(max(k0, HALO), c.nx + 1);
+    c.kc1 = min(max(k1, HALO), c.nx + 1);
+    c.kf0 = min(max(k0, HALO), c.nx + 2);
+    c.kf1 = min(max(k1, HALO), c.nx + 2);
+    c.bc0 = 8u * (unsigned)c.kc0;
+    c.bc1 = 8u * (unsigned)c.kc1;
+    c.bf0 = 8u * (unsigned)c.kf0;
+    c.bf1 = 8u * (unsigned)c.kf1;
+    c.wcl0 = k0 == HALO;
+    c.wcl1 = k1 == HALO;
+    c.ecl0 = k0 == c.nx + 1;
+    c.ecl1 = k1 == c.nx + 1;
+    c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
+    c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
+    c.out_any = c.out0 || c.out1;
+    {
+        // pad columns X .. sx-1 of every row are never read
+        const int npad = (int)(g.sx - g.X);
+        c.bpad = 8u * (unsigned)(g.X + lane % npad);
+        c.bs0 = c.out0 ? c.bc0 : c.bpad;
+        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+    }
+    c.ylo = max(c.Y0 - L, HALO);
+    c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
+    c.alpha = FIRST ? A.ds->alpha : 0.0;
+    c.theta = A.theta;
+    c.ur = reinterpret_cast<double*>(u_ring);
+    c.kxr = reinterpret_cast<double*>(kx_ring);
+    c.kyr = reinterpret_cast<double*>(ky_ring);
+    c.coef = coef;
+    // waves whose columns or rows reach a face of the mesh need the clamps
+    const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
+    const bool yedge = c.ylo == HALO || c.yhi + L >= c.ny + 1;
+    if (xedge) {
+        if (yedge) tb_wave<L, FIRST, LAST, true, true>(A, c);
+        else tb_wave<L, FIRST, LAST, true, false>(A, c);
+    } else {
+        if (yedge) tb_wave<L, FIRST, LAST, false, true>(A, c);
+        else tb_wave<L, FIRST, LAST, false, false>(A, c);
+    }
+}
+
+template <int L, bool FIRST, bool LAST>
+void launch(Chunk& c, TBArgs& a, long nblocks)
+{
+    hipLaunchKernelGGL((k_ppcg_tb<L, FIRST, LAST>), dim3((unsigned)nblocks), dim3(WAVE), 0, c.stream, c.g, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+// Launchers of the compiled pass sizes (one translation unit each, so they
+// build in parallel): first = the pass starts from CGCalcUr, last = it ends
+// with ||r||^2 instead of handing sd over.
+void tb_launch_1(Chunk& c, TBArgs& a, bool first, bool last);
+void tb_launch_2(Chunk& c, TBArgs& a, bool first, bool last);
+void tb_launch_5(Chunk& c, TBArgs& a, bool first, bool last);
+void tb_launch_10(Chunk& c, TBArgs& a, bool first, bool last);
+
+#define TL_TB_DEFINE_LAUNCH(N)                                               \
+    void tb_launch_##N(Chunk& c, TBArgs& a, bool first, bool last)           \
+    {                                                                        \
+        if (first && last) launch<N, true, true>(c, a, a.nblocks);           \
+        else if (first) launch<N, true, false>(c, a, a.nblocks);             \
+        else if (last) launch<N, false, true>(c, a, a.nblocks);              \
+        else launch<N, false, false>(c, a, a.nblocks);                       \
+    }
+
+}  // namespace tb
+#endif
+
+}  // namespace tl

@@ -1,0 +1,10 @@
+// Temporal-blocked PPCG passes of 1 level (see tl_ppcg_tb.hpp).
+#include "tl_ppcg_tb.hpp"
+
+namespace tl {
+#if TL_DIM == 2
+namespace tb {
+TL_TB_DEFINE_LAUNCH(1)
+}  // namespace tb
+#endif
+}  // namespace tl

@@ -12,7 +12,9 @@ Random layouts use splitmix64 so a seed gives the same deck everywhere.
 from __future__ import annotations
 
 SOLVER_KEYWORD = {"cg": "tl_use_cg", "chebyshev": "tl_use_chebyshev", "ppcg": "tl_use_ppcg",
-                  "jacobi": "tl_use_jacobi"}
+                  "jacobi": "tl_use_jacobi",
+                  # mixed-precision PPCG of the HIP driver (no counterpart in the reference)
+                  "ppcg_mixed": "tl_use_ppcg_mixed"}
 
 
 def fnum(x: float) -> str:
@@ -46,9 +48,9 @@ def _common(solver: str, steps: int, eps: float, dt: float, max_iters: int, pres
         "use_ext_kernels",
         SOLVER_KEYWORD[solver],
     ]
-    if solver in ("chebyshev", "ppcg"):
+    if solver in ("chebyshev", "ppcg", "ppcg_mixed"):
         out.append(f"tl_ch_cg_presteps={presteps}")
-    if solver == "ppcg":
+    if solver in ("ppcg", "ppcg_mixed"):
         out.append(f"tl_ppcg_inner_steps={inner}")
     return out + list(extra)
 
