@@ -167,6 +167,7 @@ def kernel_probe(n: int, launches: int) -> dict:
 def run_solve(deck: str, dim: int, warmup: int, timing: bool = True) -> dict:
     """Run a deck through the C++ driver; the timed steps follow `warmup`."""
     L = tealeaf_hip.lib(dim)
+    L.tl_set_progress(1)
     L.tl_set_kernel_timing(warmup + 1 if timing else 0)
     t0 = time.perf_counter()
     rep = tealeaf_hip.run_deck(deck, dim=dim)

@@ -114,6 +114,8 @@ const char* tl_driver_error(void);
  * timesteps from `from_step` on (0 = off); tl_run_deck's report then has a
  * "kernels" table: launches, summed milliseconds, algorithmic bytes/cell. */
 void tl_set_kernel_timing(int from_step);
+/* A progress line on stderr after every timestep of tl_run_deck (0 = off). */
+void tl_set_progress(int on);
 
 /* The fused PPCG outer iteration the driver uses (2D, no preconditioner),
  * on a chunk made by ext_init_: with the given alpha, CGCalcUr + PPCGInitSd
@@ -134,6 +136,40 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
  * times: the attainable streaming rate, and the known-size kernel that
  * calibrates HBM byte counters.  Negative on error. */
 double tl_copy_probe(long n, int reps);
+
+/* ---- Learned preconditioner (no counterpart in the reference) ------------
+ * A 4-level BF16 U-Net (3x3 convs, 16/32/64/128 channels, FP32 accumulate)
+ * trained on the GPU on (residual, PPCG correction) pairs, applied by the
+ * driver's flexible CG for decks that select tl_use_fcg_learned (2D, x_cells
+ * and y_cells multiples of 8; FP64 PPCG solves the timestep when the flexible
+ * CG stalls).  Handles are opaque; calls return -1 (or NULL) on error with
+ * tl_last_error() set. */
+void* tl_learn_create(unsigned long long seed);
+void tl_learn_destroy(void* learner);
+/* FP64 PPCG solve of the deck's first timestep; (residual, correction) pairs
+ * of crop x crop cells from up to `snapshots` outer iterations, `crops` each.
+ * Returns the number of pairs added. */
+long tl_learn_add_deck(void* learner, const char* deck, int snapshots, int crops, int crop,
+                       unsigned long long seed);
+long tl_learn_pairs(void* learner);
+/* Adam on random batches of the pairs; losses[steps] receives each step's loss. */
+int tl_learn_train(void* learner, int steps, int batch, double lr, double* losses);
+/* The network tl_run_deck's flexible CG uses (NULL: none). */
+int tl_learn_use(void* learner);
+/* Parameters in layer order, each layer W[cout][9][cin] then b[cout]. */
+long tl_learn_param_count(void* learner);
+int tl_learn_get_params(void* learner, float* out);
+int tl_learn_set_params(void* learner, const float* in);
+/* Loss of the output against target and its parameter gradients (host
+ * planes of n x h x w; h, w multiples of 8).  With kx, ky (each pixel's west
+ * and south face coefficients) the A-norm (d.A d)/(t.A t), d = out - target,
+ * averaged over the n samples, as in training; with NULL the mean squared
+ * error. */
+double tl_learn_loss_grad(void* learner, const float* r, const float* f, const float* target, const float* kx,
+                          const float* ky, int n, int h, int w, float* grads);
+/* Network output (channel 0) for host planes r (scaled residual) and f
+ * (density feature). */
+int tl_learn_apply(void* learner, const float* r, const float* f, float* z, int n, int h, int w);
 
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);

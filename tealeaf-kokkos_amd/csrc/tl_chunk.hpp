@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "tl_device.hpp"
@@ -197,6 +198,55 @@ int tb_rows(int ny, int tiles_x, int L, int slots);
 std::vector<int> tb_passes(int inner_steps);
 bool ppcg_tb_supported(const Chunk& c, int inner_steps);
 long op_ppcg_outer_tb(Chunk& c, int inner_steps);
+
+// Flexible CG (tl_learn.hip): a . b over the interior; p = z + beta p.
+double op_dot(Chunk& c, const double* a, const double* b);
+void op_fcg_update_p(Chunk& c, double beta);
+
+// Observers of the driver's FP64 PPCG solve (no counterpart in the
+// reference): after every outer iteration (u, r current) and at the end of
+// the solve before the energy is finalised.  Used to record training pairs.
+struct SolveHooks {
+    std::function<void(Chunk&, int)> on_outer;
+    std::function<void(Chunk&)> on_end;
+};
+SolveHooks& solve_hooks();
+
+// The learned preconditioner the driver's flexible CG applies (tl_learn.hip).
+struct Learner;
+Learner* active_learner();
+// Flexible CG on the chunk set up by the driver (u, u0, r = u0 - A u, rro in
+// the device scalars): true when it reached eps on the true residual, false
+// when it stalled (the caller then solves with FP64 PPCG).
+// z = s N(r / s) (network on) or 0, then `smooth` weighted-Jacobi sweeps
+// z += omega D^-1 (r - A z).
+struct FcgOptions {
+    bool network = true;
+    int smooth = 4;
+    double omega = 0.8;
+};
+bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, int max_iters, int& iters,
+                       double& error);
+
+// The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps
+// timesteps when that is positive; returns the JSON report.
+std::string driver_run_deck(const std::string& text, int max_steps);
+
+// Learned preconditioner (tl_learn.hip): training pairs recorded from the
+// FP64 PPCG solves of decks, training of the network on them, and the
+// network the driver's flexible CG uses.
+Learner* learn_create(unsigned long long seed);
+void learn_destroy(Learner* L);
+// Runs the first timestep of `deck` with FP64 PPCG, records (residual,
+// correction) at up to `snapshots` outer iterations and cuts `crops` random
+// crop x crop pairs from each; returns the number of pairs added.
+long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop,
+                    unsigned long long seed);
+long learn_pairs(const Learner& L);
+void learn_train(Learner& L, int steps, int batch, double lr, double* losses);
+void learn_set_active(Learner* L);
+struct UNet;
+UNet& learn_net(Learner& L);
 
 // Mixed-precision PPCG (tl_ppcg_mixed.hip): FP32 inner steps inside the
 // FP64 outer iteration.  prepare makes the FP32 coefficient copies after

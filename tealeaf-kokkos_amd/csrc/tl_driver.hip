@@ -54,6 +54,10 @@ struct Deck {
     // steps, the FP64 residual replaced once ||r||^2 fell by this factor.
     bool ppcg_mixed = false;
     double mixed_replace = 1.0e-4;
+    // Flexible CG with the learned preconditioner (no counterpart in the
+    // reference), FP64 PPCG as its fallback solver.
+    bool use_fcg = false;
+    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_no_network
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -201,12 +205,16 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_check_result") d.check_result = true;
             else if (k == "tl_ch_cg_errswitch") d.errswitch = true;
             else if (k == "tl_preconditioner_on") d.preconditioner = true;
-            else if (k == "tl_use_jacobi") { d.use_cheby = d.use_cg = d.use_ppcg = d.ppcg_mixed = false; d.use_jacobi = true; }
-            else if (k == "tl_use_cg") { d.use_cheby = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = false; d.use_cg = true; }
-            else if (k == "tl_use_ppcg") { d.use_cheby = d.use_cg = d.use_jacobi = d.ppcg_mixed = false; d.use_ppcg = true; }
-            else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = false; d.use_ppcg = d.ppcg_mixed = true; }
+            else if (k == "tl_use_jacobi") { d.use_cheby = d.use_cg = d.use_ppcg = d.ppcg_mixed = d.use_fcg = false; d.use_jacobi = true; }
+            else if (k == "tl_use_cg") { d.use_cheby = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cg = true; }
+            else if (k == "tl_use_ppcg") { d.use_cheby = d.use_cg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_ppcg = true; }
+            else if (k == "tl_use_fcg_learned") { d.use_cheby = d.use_cg = d.use_jacobi = d.ppcg_mixed = false; d.use_ppcg = d.use_fcg = true; }
+            else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = d.use_fcg = false; d.use_ppcg = d.ppcg_mixed = true; }
             else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
-            else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = false; d.use_cheby = true; }
+            else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
+            else if (k == "tl_fcg_omega") d.fcg.omega = R();
+            else if (k == "tl_fcg_no_network") d.fcg.network = false;
+            else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();
             else if (k == "tl_coefficient_density") d.coefficient = CONDUCTIVITY;
@@ -355,6 +363,9 @@ struct StepInfo {
     // whether the FP64 inner steps took over
     int mixed_steps = 0, replacements = 0;
     bool mixed_fallback = false;
+    // learned flexible CG: its iterations, and whether FP64 PPCG took over
+    int fcg_iters = 0;
+    bool fcg = false, fcg_fallback = false;
 };
 
 const int kAllExternal[6] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
@@ -401,8 +412,29 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     }
     op_copy_u(c);
 
-    int n;
-    for (n = 1; n <= max_iters; ++n) {
+    int n = 1;
+    Learner* learner = d.use_fcg ? active_learner() : nullptr;
+    if (d.use_fcg && d.fcg.network && !learner) throw Error("tl_use_fcg_learned: no learned preconditioner is loaded");
+    bool fcg_done = false;
+    if (d.use_fcg) {
+        int it = 0;
+        double err = 0.0;
+        si.fcg = true;
+        fcg_done = fcg_learned_solve(c, learner, d.fcg, d.eps, max_iters, it, err);
+        si.fcg_iters = it;
+        if (fcg_done) {
+            error = err;
+            n = it + 1;
+        } else {
+            // the flexible CG stalled: solve this timestep with FP64 PPCG from the start
+            si.fcg_fallback = true;
+            halo(c, {FIELD_ENERGY1, FIELD_DENSITY}, 2);
+            (void)op_cg_init(c, d.coefficient, d.preconditioner, dt);
+            op_copy_u(c);
+        }
+    }
+
+    for (n = fcg_done ? n : 1; !fcg_done && n <= max_iters; ++n) {
         bool ch_switch_check;
         if (d.errswitch)
             ch_switch_check = (cheby_calc_steps > 0) || ((error <= d.epslim) && (n >= 20));
@@ -482,6 +514,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                             }
                         }
                     }
+                    if (solve_hooks().on_outer) solve_hooks().on_outer(c, cheby_calc_steps);
                 } else if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
                     TL_TIMED(c, K_PPCG_OUTER, op_finish_rrn(c, op_ppcg_outer_tb(c, d.ppcg_inner_steps)));
@@ -494,6 +527,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 }
                 p_pending = true;
                 error = op_error(c);
+                if (solve_hooks().on_outer) solve_hooks().on_outer(c, cheby_calc_steps);
             }
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
@@ -511,6 +545,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
         }
         if (std::fabs(error) < d.eps) break;
     }
+    if (solve_hooks().on_end) solve_hooks().on_end(c);
     if (d.check_result) {
         op_calc_residual(c);
         exact_error = op_calc_2norm(c, true);
@@ -553,6 +588,8 @@ void summary(Json& o, Chunk& c, int step, double time)
 }
 
 // initialise -> start -> diffuse (start.f90:24-162, diffuse.f90:23-160).
+bool g_progress = false;  // a line on stderr after every timestep
+
 std::string run_deck(const std::string& text, int max_steps)
 {
     const Deck d = parse_deck(text);
@@ -579,7 +616,7 @@ std::string run_deck(const std::string& text, int max_steps)
     o.f("{\"dim\": %d, \"x_cells\": %d, \"y_cells\": %d, \"z_cells\": %d, \"device\": \"hip\", ", TL_DIM, d.x_cells,
         d.y_cells, nz);
     o.f("\"solver\": \"%s\", \"eps\": %.17g, \"dtinit\": %.17g, \"max_iters\": %d, ",
-        d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? (d.ppcg_mixed ? "ppcg_mixed" : "ppcg") : "jacobi",
+        d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? (d.use_fcg ? "fcg_learned" : d.ppcg_mixed ? "ppcg_mixed" : "ppcg") : "jacobi",
         d.eps, d.dtinit, d.max_iters);
     o.f("\"initial\": {");
     summary(o, *c, 0, 0.0);
@@ -596,12 +633,18 @@ std::string run_deck(const std::string& text, int max_steps)
         const StepInfo si = tea_leaf(*c, d, dt);
         total += si.solve_s;
         time = time + dt;
+        if (g_progress) {
+            std::fprintf(stderr, "timestep %d: %d iterations, %.3f s\n", step, si.iters, si.solve_s);
+            std::fflush(stderr);
+        }
         o.f("%s{", step > 1 ? ", " : "");
         o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
             "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
-            "\"dt\": %.17g, \"mixed_steps\": %d, \"replacements\": %d, \"mixed_fallback\": %d, ",
+            "\"dt\": %.17g, \"mixed_steps\": %d, \"replacements\": %d, \"mixed_fallback\": %d, "
+            "\"fcg\": %d, \"fcg_iters\": %d, \"fcg_fallback\": %d, ",
             si.iters, si.error, si.exact_error, si.switch_step, si.est_itc, si.eigmin, si.eigmax, si.cg_steps,
-            si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0);
+            si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0,
+            si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0);
         if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
             summary(o, *c, step, time);
         else
@@ -632,6 +675,9 @@ std::string run_deck(const std::string& text, int max_steps)
 std::string g_driver_error;
 
 }  // namespace
+
+std::string driver_run_deck(const std::string& text, int max_steps) { return run_deck(text, max_steps); }
+
 }  // namespace tl
 
 extern "C" {
@@ -664,5 +710,8 @@ const char* tl_driver_error(void) { return tl::g_driver_error.c_str(); }
 // "kernels" table (launch count, summed milliseconds, algorithmic bytes per
 // cell) to its report.
 void tl_set_kernel_timing(int from_step) { tl::set_kernel_timing_from(from_step); }
+
+// A line on stderr after every timestep of tl_run_deck (0 turns it off).
+void tl_set_progress(int on) { tl::g_progress = on != 0; }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 // with tl_set_abort_on_error(0), in which case the call returns and the
 // message is kept for tl_last_error().
 #include "tl_chunk.hpp"
+#include "tl_unet.hpp"
 
 #include <sys/time.h>
 
@@ -130,6 +131,127 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
         op_mixed_prepare(*c, alphas, betas, nsteps, theta);
         op_finish_rrn(*c, op_ppcg_outer_mixed(*c, nsteps));
         *rrn = op_error(*c);
+        rc = 0;
+    });
+    return rc;
+}
+
+// ---- learned preconditioner (tl_learn.hip, tl_unet.hip) -------------------
+
+void* tl_learn_create(unsigned long long seed)
+{
+    void* h = nullptr;
+    guarded([&] { h = learn_create(seed); });
+    return h;
+}
+
+void tl_learn_destroy(void* h) { guarded([&] { learn_destroy(static_cast<Learner*>(h)); }); }
+
+long tl_learn_add_deck(void* h, const char* deck, int snapshots, int crops, int crop, unsigned long long seed)
+{
+    long n = -1;
+    guarded([&] {
+        if (!h || !deck) throw Error("tl_learn_add_deck: null argument");
+        n = learn_add_deck(*static_cast<Learner*>(h), deck, snapshots, crops, crop, seed);
+    });
+    return n;
+}
+
+long tl_learn_pairs(void* h) { return h ? learn_pairs(*static_cast<Learner*>(h)) : -1; }
+
+int tl_learn_train(void* h, int steps, int batch, double lr, double* losses)
+{
+    int rc = -1;
+    guarded([&] {
+        if (!h) throw Error("tl_learn_train: null learner");
+        learn_train(*static_cast<Learner*>(h), steps, batch, lr, losses);
+        rc = 0;
+    });
+    return rc;
+}
+
+int tl_learn_use(void* h)
+{
+    learn_set_active(static_cast<Learner*>(h));
+    return 0;
+}
+
+long tl_learn_param_count(void* h) { return h ? unet_param_count(learn_net(*static_cast<Learner*>(h))) : -1; }
+
+int tl_learn_get_params(void* h, float* out)
+{
+    int rc = -1;
+    guarded([&] {
+        unet_get_params(learn_net(*static_cast<Learner*>(h)), out);
+        rc = 0;
+    });
+    return rc;
+}
+
+int tl_learn_set_params(void* h, const float* in)
+{
+    int rc = -1;
+    guarded([&] {
+        unet_set_params(learn_net(*static_cast<Learner*>(h)), in);
+        rc = 0;
+    });
+    return rc;
+}
+
+namespace {
+
+struct DevPlanes {
+    float* d[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~DevPlanes()
+    {
+        for (float* p : d) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+// Loss and parameter gradients of the network on host planes r, f, target
+// (n x h x w each), with kx, ky the A-norm loss, else the mean squared
+// error; grads receives tl_learn_param_count floats.
+double tl_learn_loss_grad(void* h, const float* r, const float* f, const float* target, const float* kx,
+                          const float* ky, int n, int hh, int ww, float* grads)
+{
+    double loss = -1.0;
+    guarded([&] {
+        UNet& u = learn_net(*static_cast<Learner*>(h));
+        const size_t bytes = sizeof(float) * (size_t)n * hh * ww;
+        DevPlanes p, q;
+        for (int k = 0; k < 3; ++k) TL_HIP(hipMalloc(&p.d[k], bytes));
+        TL_HIP(hipMemcpy(p.d[0], r, bytes, hipMemcpyHostToDevice));
+        TL_HIP(hipMemcpy(p.d[1], f, bytes, hipMemcpyHostToDevice));
+        TL_HIP(hipMemcpy(p.d[2], target, bytes, hipMemcpyHostToDevice));
+        const bool anorm = kx && ky;
+        if (anorm) {
+            for (int k = 0; k < 2; ++k) TL_HIP(hipMalloc(&q.d[k], bytes));
+            TL_HIP(hipMemcpy(q.d[0], kx, bytes, hipMemcpyHostToDevice));
+            TL_HIP(hipMemcpy(q.d[1], ky, bytes, hipMemcpyHostToDevice));
+        }
+        loss = unet_loss_grad(u, p.d[0], p.d[1], p.d[2], anorm ? q.d[0] : nullptr, anorm ? q.d[1] : nullptr, n, hh,
+                              ww);
+        unet_get_grads(u, grads);
+    });
+    return loss;
+}
+
+// The network's output channel 0 for host planes r, f (n x h x w) into z.
+int tl_learn_apply(void* h, const float* r, const float* f, float* z, int n, int hh, int ww)
+{
+    int rc = -1;
+    guarded([&] {
+        UNet& u = learn_net(*static_cast<Learner*>(h));
+        const size_t bytes = sizeof(float) * (size_t)n * hh * ww;
+        DevPlanes p;
+        for (int k = 0; k < 3; ++k) TL_HIP(hipMalloc(&p.d[k], bytes));
+        TL_HIP(hipMemcpy(p.d[0], r, bytes, hipMemcpyHostToDevice));
+        TL_HIP(hipMemcpy(p.d[1], f, bytes, hipMemcpyHostToDevice));
+        unet_apply(u, p.d[0], p.d[1], p.d[2], n, hh, ww);
+        TL_HIP(hipStreamSynchronize(unet_stream(u)));
+        TL_HIP(hipMemcpy(z, p.d[2], bytes, hipMemcpyDeviceToHost));
         rc = 0;
     });
     return rc;

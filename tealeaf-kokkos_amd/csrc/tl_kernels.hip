@@ -786,6 +786,24 @@ __global__ __launch_bounds__(BLOCK) void k_2norm(Geom g, const double* __restric
     block_partial<1>(v, red);
 }
 
+// a . b over the interior (flexible CG)
+__global__ __launch_bounds__(BLOCK) void k_dot(Geom g, const double* __restrict__ a, const double* __restrict__ b,
+                                               Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) v[0] += a[s.i] * b[s.i];
+    block_partial<1>(v, red);
+}
+
+// p = z + beta p over the interior (flexible CG direction)
+__global__ __launch_bounds__(BLOCK) void k_fcg_p(Geom g, double beta, const double* __restrict__ z,
+                                                 double* __restrict__ p)
+{
+    Strip s = make_strip(g);
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) p[s.i] = z[s.i] + beta * p[s.i];
+}
+
 // Finalise (:104-111)
 __global__ __launch_bounds__(BLOCK) void k_finalise(Geom g, const double* __restrict__ u,
                                                     const double* __restrict__ density, double* __restrict__ energy)
@@ -1465,6 +1483,15 @@ void op_calc_residual(Chunk& c)
 {
     TIMED(c, K_RESIDUAL, STRIP_LAUNCH(k_residual, c, kptr(c), c.f[F_U], c.f[F_U0], c.f[F_R]));
 }
+
+double op_dot(Chunk& c, const double* a, const double* b)
+{
+    STRIP_LAUNCH(k_dot, c, a, b, c.reducer());
+    finish<1>(c, c.g.nblocks, S_NORM, EP_STORE);
+    return c.result(S_NORM);
+}
+
+void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, c.f[F_Z], c.f[F_P]); }
 
 double op_calc_2norm(Chunk& c, bool norm_r)
 {

@@ -1,0 +1,443 @@
+// Learned preconditioner for TeaLeaf's implicit solve (no counterpart in
+// the reference): training pairs from the FP64 PPCG solver, training of the
+// BF16 U-Net (tl_unet.hip) on the GPU, and flexible CG with the network as
+// its preconditioner, with FP64 PPCG as the fallback when it stalls.
+//
+// Training pairs.  While the driver's FP64 PPCG solves a deck's first
+// timestep, the solve hooks copy u_k and r_k = u0 - A u_k at a set of outer
+// iterations k, and u* at convergence.  The PPCG correction of snapshot k is
+// e_k = u* - u_k (A e_k = r_k up to the solver tolerance).  Random crop x crop
+// windows of (r_k, density, e_k) become pairs: the network input is r / s
+// (s = rms of the crop's r) with the density feature (log10(rho) - 0.5)/1.5,
+// the target e / (s G), G one global scale over the data set.  Residual and
+// correction share the factor s, so the map stays linear in r as A^-1 is.
+//
+// Flexible CG (Notay's FCG with one stored direction): z = s N(r / s),
+// p = z - (z.w_prev / p_prev.w_prev) p_prev, w = A p, alpha = p.r / p.w,
+// u += alpha p, r -= alpha w, all vectors and dot products in FP64.  It is
+// accepted only on the true residual ||u0 - A u||^2 < eps; when the
+// residual has not halved over a window of iterations the driver solves the
+// timestep again with FP64 PPCG.
+#include "tl_chunk.hpp"
+#include "tl_unet.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace tl {
+
+struct Learner {
+    UNet* net = nullptr;
+    uint64_t rng = 0;
+    int crop = 0;
+    long pairs = 0;
+    float *dr = nullptr, *df = nullptr, *de = nullptr;  // [pairs][crop][crop]
+    float *dkx = nullptr, *dky = nullptr;               // the pair's face coefficients
+    long dr_cap = 0, df_cap = 0, de_cap = 0, dkx_cap = 0, dky_cap = 0;
+    double escale = 0.0;                                // G; 0 until the first training call
+    float *br = nullptr, *bf = nullptr, *be = nullptr;  // one training batch
+    float *bkx = nullptr, *bky = nullptr;
+    long br_cap = 0, bf_cap = 0, be_cap = 0, bkx_cap = 0, bky_cap = 0;
+    float *pr = nullptr, *pf = nullptr, *pz = nullptr;  // FCG planes
+    long pr_cap = 0, pf_cap = 0, pz_cap = 0;
+    double* scratch = nullptr;                          // per-crop sums
+    long scratch_cap = 0;
+};
+
+namespace {
+
+Learner* g_active = nullptr;
+SolveHooks g_hooks;
+
+uint64_t next(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <class T>
+void regrow(T*& p, long& cap, long need, long keep)
+{
+    if (need <= cap) return;
+    T* q = nullptr;
+    TL_HIP(hipMalloc(&q, sizeof(T) * (size_t)need));
+    if (p && keep > 0) TL_HIP(hipMemcpy(q, p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice));
+    (void)hipFree(p);
+    p = q;
+    cap = need;
+}
+
+// One workgroup per crop: copy the window at (x0, y0) (interior cell
+// coordinates) of r_k, the density feature, e = u* - u_k and the face
+// coefficients kx, ky, and add the crop's sum of r^2 into ss[crop].
+__global__ void k_extract(Geom g, const double* __restrict__ uk, const double* __restrict__ ustar,
+                          const double* __restrict__ rk, const double* __restrict__ density,
+                          const double* __restrict__ kx, const double* __restrict__ ky, const int* __restrict__ xy,
+                          int crop, float* __restrict__ out_r, float* __restrict__ out_f, float* __restrict__ out_e,
+                          float* __restrict__ out_kx, float* __restrict__ out_ky, double* __restrict__ ss)
+{
+    const int q = blockIdx.x;
+    const int x0 = xy[2 * q], y0 = xy[2 * q + 1];
+    const long base = (long)q * crop * crop;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < crop * crop; i += blockDim.x) {
+        const int x = i % crop, y = i / crop;
+        const long gi = gidx(g, HALO + x0 + x, HALO + y0 + y, 0);
+        const double r = rk[gi];
+        out_r[base + i] = (float)r;
+        out_f[base + i] = (float)((std::log10(density[gi]) - 0.5) / 1.5);
+        out_e[base + i] = (float)(ustar[gi] - uk[gi]);
+        out_kx[base + i] = (float)kx[gi];
+        out_ky[base + i] = (float)ky[gi];
+        s += r * r;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if (threadIdx.x % WAVE == 0) atomicAdd(ss + q, s);
+}
+
+// r /= s, e /= s G with s = rms of the crop's r
+__global__ void k_normalise(int crop, const double* __restrict__ ss, double inv_g, float* __restrict__ r,
+                            float* __restrict__ e)
+{
+    const int q = blockIdx.x;
+    const long base = (long)q * crop * crop;
+    const double rms = std::sqrt(ss[q] / ((double)crop * crop));
+    const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
+    for (int i = threadIdx.x; i < crop * crop; i += blockDim.x) {
+        r[base + i] = (float)(r[base + i] * inv);
+        e[base + i] = (float)(e[base + i] * inv * inv_g);
+    }
+}
+
+__global__ void k_sumsq(const float* __restrict__ a, long n, double* __restrict__ out)
+{
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s += (double)a[i] * a[i];
+    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if (threadIdx.x % WAVE == 0) atomicAdd(out, s);
+}
+
+__global__ void k_scale(float* __restrict__ a, long n, float f)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] *= f;
+}
+
+// FCG planes: r / s and the density feature over the interior (row-major ny x nx)
+__global__ void k_plane_in(Geom g, const double* __restrict__ r, const double* __restrict__ density, double inv_s,
+                           int with_feature, float* __restrict__ pr, float* __restrict__ pf)
+{
+    const long n = (long)g.nx * g.ny;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % g.nx), y = (int)(i / g.nx);
+        const long gi = gidx(g, HALO + x, HALO + y, 0);
+        pr[i] = (float)(r[gi] * inv_s);
+        if (with_feature) pf[i] = (float)((std::log10(density[gi]) - 0.5) / 1.5);
+    }
+}
+
+// z = s * network output, FP64, interior
+__global__ void k_plane_out(Geom g, const float* __restrict__ pz, double s, double* __restrict__ z)
+{
+    const long n = (long)g.nx * g.ny;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % g.nx), y = (int)(i / g.nx);
+        z[gidx(g, HALO + x, HALO + y, 0)] = s * (double)pz[i];
+    }
+}
+
+// One weighted Jacobi sweep on A z = r: z_out = z + omega (r - A z) / D,
+// D the diagonal of A (1 + the coefficients of the cell's interior faces;
+// a face on the mesh boundary carries no flux).  Interior cells.
+__global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double* __restrict__ ky,
+                             const double* __restrict__ r, const double* __restrict__ z, double omega,
+                             double* __restrict__ zo)
+{
+    const long n = (long)g.nx * g.ny;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % g.nx) + HALO, y = (int)(i / g.nx) + HALO;
+        const long c = gidx(g, x, y, 0);
+        const double zc = z[c];
+        double az = zc, d = 1.0;
+        if (x < g.nx + 1) { az += kx[c + 1] * (zc - z[c + 1]); d += kx[c + 1]; }
+        if (x > HALO) { az += kx[c] * (zc - z[c - 1]); d += kx[c]; }
+        if (y < g.ny + 1) { az += ky[c + g.sx] * (zc - z[c + g.sx]); d += ky[c + g.sx]; }
+        if (y > HALO) { az += ky[c] * (zc - z[c - g.sx]); d += ky[c]; }
+        zo[c] = zc + omega * (r[c] - az) / d;
+    }
+}
+
+unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 16384)); }
+
+void launched(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Outer iterations at which a solve is sampled: spread over the whole
+// convergence history (early, large residuals to late, small ones).
+const int kSchedule[] = {1,   2,   3,   5,   8,   13,   21,   34,   55,   89,   144,  233,
+                         377, 610, 987, 1597, 2584, 4181, 6765, 10946};
+
+}  // namespace
+
+SolveHooks& solve_hooks() { return g_hooks; }
+Learner* active_learner() { return g_active; }
+void learn_set_active(Learner* L) { g_active = L; }
+UNet& learn_net(Learner& L) { return *L.net; }
+long learn_pairs(const Learner& L) { return L.pairs; }
+
+Learner* learn_create(unsigned long long seed)
+{
+    auto L = std::make_unique<Learner>();
+    L->net = unet_create(seed);
+    L->rng = seed ^ 0x5DEECE66Dull;
+    return L.release();
+}
+
+void learn_destroy(Learner* L)
+{
+    if (!L) return;
+    if (g_active == L) g_active = nullptr;
+    unet_destroy(L->net);
+    for (float* p : {L->dr, L->df, L->de, L->dkx, L->dky, L->br, L->bf, L->be, L->bkx, L->bky, L->pr, L->pf, L->pz})
+        (void)hipFree(p);
+    (void)hipFree(L->scratch);
+    delete L;
+}
+
+long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop, unsigned long long seed)
+{
+    if (crop < 8 || crop % 8) throw Error("training crops must be a multiple of 8 pixels");
+    if (L.crop && L.crop != crop) throw Error("all training crops must have the same size");
+    L.crop = crop;
+    snapshots = std::max(1, std::min<int>(snapshots, (int)(sizeof kSchedule / sizeof kSchedule[0])));
+    struct Snap {
+        double *u = nullptr, *r = nullptr;
+    };
+    std::vector<Snap> snaps;
+    double *ustar = nullptr, *dens = nullptr, *kx = nullptr, *ky = nullptr;
+    Geom geom{};
+    bool seen = false;
+    auto release = [&] {
+        for (Snap& s : snaps) {
+            (void)hipFree(s.u);
+            (void)hipFree(s.r);
+        }
+        (void)hipFree(ustar);
+        (void)hipFree(dens);
+        (void)hipFree(kx);
+        (void)hipFree(ky);
+    };
+    g_hooks.on_outer = [&](Chunk& c, int k) {
+        if (TL_DIM != 2 || (int)snaps.size() >= snapshots || k != kSchedule[snaps.size()]) return;
+        const size_t bytes = sizeof(double) * (size_t)c.g.alloc;
+        Snap s;
+        TL_HIP(hipMalloc(&s.u, bytes));
+        TL_HIP(hipMalloc(&s.r, bytes));
+        TL_HIP(hipMemcpyAsync(s.u, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(s.r, c.f[F_R], bytes, hipMemcpyDeviceToDevice, c.stream));
+        snaps.push_back(s);
+    };
+    g_hooks.on_end = [&](Chunk& c) {
+        if (seen) return;  // the first timestep only
+        seen = true;
+        geom = c.g;
+        const size_t bytes = sizeof(double) * (size_t)c.g.alloc;
+        TL_HIP(hipMalloc(&ustar, bytes));
+        TL_HIP(hipMalloc(&dens, bytes));
+        TL_HIP(hipMemcpyAsync(ustar, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(dens, c.f[F_DENSITY], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMalloc(&kx, bytes));
+        TL_HIP(hipMalloc(&ky, bytes));
+        TL_HIP(hipMemcpyAsync(kx, c.f[F_KX], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(ky, c.f[F_KY], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipStreamSynchronize(c.stream));
+    };
+    try {
+        (void)driver_run_deck(deck, 1);
+    } catch (...) {
+        g_hooks = SolveHooks{};
+        release();
+        throw;
+    }
+    g_hooks = SolveHooks{};
+    if (!seen || snaps.empty() || geom.nx < crop || geom.ny < crop) {
+        release();
+        throw Error("the deck's solve gave no training pairs (2D PPCG deck larger than the crop needed)");
+    }
+    const long n_new = (long)snaps.size() * crops;
+    const long px = (long)crop * crop;
+    const long need = (L.pairs + n_new) * px, keep = L.pairs * px;
+    regrow(L.dr, L.dr_cap, need, keep);
+    regrow(L.df, L.df_cap, need, keep);
+    regrow(L.de, L.de_cap, need, keep);
+    regrow(L.dkx, L.dkx_cap, need, keep);
+    regrow(L.dky, L.dky_cap, need, keep);
+    regrow(L.scratch, L.scratch_cap, crops, 0);
+    int* dxy = nullptr;
+    TL_HIP(hipMalloc(&dxy, sizeof(int) * 2 * crops));
+    uint64_t rs = seed;
+    std::vector<int> xy(2 * crops);
+    for (size_t k = 0; k < snaps.size(); ++k) {
+        for (int q = 0; q < crops; ++q) {
+            xy[2 * q] = (int)(next(rs) % (uint64_t)(geom.nx - crop + 1));
+            xy[2 * q + 1] = (int)(next(rs) % (uint64_t)(geom.ny - crop + 1));
+        }
+        TL_HIP(hipMemcpy(dxy, xy.data(), sizeof(int) * 2 * crops, hipMemcpyHostToDevice));
+        TL_HIP(hipMemset(L.scratch, 0, sizeof(double) * crops));
+        const long off = L.pairs * px;
+        hipLaunchKernelGGL(k_extract, dim3(crops), dim3(256), 0, nullptr, geom, snaps[k].u, ustar, snaps[k].r, dens, kx,
+                           ky, dxy, crop, L.dr + off, L.df + off, L.de + off, L.dkx + off, L.dky + off, L.scratch);
+        launched("training pair extraction");
+        const double inv_g = L.escale > 0.0 ? 1.0 / L.escale : 1.0;
+        hipLaunchKernelGGL(k_normalise, dim3(crops), dim3(256), 0, nullptr, crop, L.scratch, inv_g, L.dr + off,
+                           L.de + off);
+        launched("training pair normalisation");
+        TL_HIP(hipDeviceSynchronize());
+        L.pairs += crops;
+    }
+    (void)hipFree(dxy);
+    release();
+    return n_new;
+}
+
+void learn_train(Learner& L, int steps, int batch, double lr, double* losses)
+{
+    if (L.pairs < 1) throw Error("no training pairs");
+    if (batch < 1 || steps < 1) throw Error("training needs steps >= 1 and batch >= 1");
+    const long px = (long)L.crop * L.crop;
+    hipStream_t s = unet_stream(*L.net);
+    if (L.escale <= 0.0) {
+        // one global target scale: unit rms over the data set
+        regrow(L.scratch, L.scratch_cap, 1, 0);
+        TL_HIP(hipMemset(L.scratch, 0, sizeof(double)));
+        hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(L.pairs * px)), dim3(256), 0, nullptr, L.de, L.pairs * px,
+                           L.scratch);
+        launched("target scale");
+        double ss = 0.0;
+        TL_HIP(hipMemcpy(&ss, L.scratch, sizeof(double), hipMemcpyDeviceToHost));
+        L.escale = ss > 0.0 ? std::sqrt(ss / (double)(L.pairs * px)) : 1.0;
+        hipLaunchKernelGGL(k_scale, dim3(blocks_for(L.pairs * px)), dim3(256), 0, nullptr, L.de, L.pairs * px,
+                           (float)(1.0 / L.escale));
+        launched("target scale");
+        TL_HIP(hipDeviceSynchronize());
+    }
+    regrow(L.br, L.br_cap, batch * px, 0);
+    regrow(L.bf, L.bf_cap, batch * px, 0);
+    regrow(L.be, L.be_cap, batch * px, 0);
+    regrow(L.bkx, L.bkx_cap, batch * px, 0);
+    regrow(L.bky, L.bky_cap, batch * px, 0);
+    for (int it = 0; it < steps; ++it) {
+        for (int b = 0; b < batch; ++b) {
+            const long q = (long)(next(L.rng) % (uint64_t)L.pairs);
+            TL_HIP(hipMemcpyAsync(L.br + b * px, L.dr + q * px, sizeof(float) * px, hipMemcpyDeviceToDevice, s));
+            TL_HIP(hipMemcpyAsync(L.bf + b * px, L.df + q * px, sizeof(float) * px, hipMemcpyDeviceToDevice, s));
+            TL_HIP(hipMemcpyAsync(L.be + b * px, L.de + q * px, sizeof(float) * px, hipMemcpyDeviceToDevice, s));
+            TL_HIP(hipMemcpyAsync(L.bkx + b * px, L.dkx + q * px, sizeof(float) * px, hipMemcpyDeviceToDevice, s));
+            TL_HIP(hipMemcpyAsync(L.bky + b * px, L.dky + q * px, sizeof(float) * px, hipMemcpyDeviceToDevice, s));
+        }
+        // the A-norm of the error: the norm flexible CG minimises
+        const float loss = unet_loss_grad(*L.net, L.br, L.bf, L.be, L.bkx, L.bky, batch, L.crop, L.crop);
+        // cosine learning-rate decay to 5% of lr
+        const double frac = (double)it / (double)steps;
+        const double lr_t = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac)));
+        unet_adam(*L.net, (float)lr_t);
+        if (losses) losses[it] = loss;
+    }
+    TL_HIP(hipStreamSynchronize(s));
+}
+
+bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, int max_iters, int& iters,
+                       double& error)
+{
+    iters = 0;
+    error = 0.0;
+    if (TL_DIM != 2 || c.g.nx % 8 || c.g.ny % 8 || c.preconditioner || (!L && opt.network)) return false;
+    const long n = (long)c.g.nx * c.g.ny;
+    float *pf = nullptr, *pz = nullptr;
+    hipStream_t keep = nullptr;
+    if (opt.network) {
+        regrow(L->pr, L->pr_cap, n, 0);
+        regrow(L->pf, L->pf_cap, n, 0);
+        regrow(L->pz, L->pz_cap, n, 0);
+        pf = L->pf;
+        pz = L->pz;
+        keep = unet_stream(*L->net);
+        unet_set_stream(*L->net, c.stream);
+    }
+    struct Restore {
+        UNet* net;
+        hipStream_t s;
+        ~Restore()
+        {
+            if (net) unet_set_stream(*net, s);
+        }
+    } restore{opt.network ? L->net : nullptr, keep};
+
+    const double cells = (double)n;
+    double rrn = op_calc_2norm(c, true);
+    if (rrn < eps) {
+        error = rrn;
+        return true;
+    }
+    double pw_prev = 0.0, best = rrn;
+    int since_best = 0;
+    const int window = 200;
+    bool first = true;
+    for (int it = 1; it <= max_iters; ++it) {
+        // z = s N(r / s), then weighted Jacobi sweeps on A z = r from it
+        const double s = std::sqrt(rrn / cells);
+        if (opt.network) {
+            hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
+                               c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, first ? 1 : 0, L->pr, pf);
+            launched("FCG input");
+            unet_apply(*L->net, L->pr, pf, pz, 1, c.g.ny, c.g.nx);
+            hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s, c.f[F_Z]);
+            launched("FCG output");
+        } else {
+            TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
+        }
+        for (int k = 0; k < opt.smooth; ++k) {
+            hipLaunchKernelGGL(k_fcg_smooth, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_KX], c.f[F_KY],
+                               c.f[F_R], c.f[F_Z], opt.omega, c.f[F_SD]);
+            launched("FCG smoothing");
+            std::swap(c.f[F_Z], c.f[F_SD]);
+        }
+        double beta = 0.0;
+        if (!first) beta = -op_dot(c, c.f[F_Z], c.f[F_W]) / pw_prev;
+        op_fcg_update_p(c, beta);
+        first = false;
+        const double pw = op_cg_calc_w(c);
+        const double pr = op_dot(c, c.f[F_P], c.f[F_R]);
+        if (!(pw > 0.0) || !std::isfinite(pr)) return false;
+        rrn = op_cg_calc_ur(c, pr / pw);
+        pw_prev = pw;
+        iters = it;
+        if (!std::isfinite(rrn)) return false;
+        if (rrn < eps) {
+            // accept on the true residual only
+            op_calc_residual(c);
+            rrn = op_calc_2norm(c, true);
+            if (rrn < eps) {
+                error = rrn;
+                return true;
+            }
+        }
+        if (rrn < 0.5 * best) {
+            best = rrn;
+            since_best = 0;
+        } else if (++since_best >= window) {
+            return false;  // stalled
+        }
+    }
+    return false;
+}
+
+}  // namespace tl

@@ -1,0 +1,34 @@
+// Host interface of the BF16 U-Net (tl_unet.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tl {
+
+struct UNet;
+
+UNet* unet_create(uint64_t seed);
+void unet_destroy(UNet* u);
+// Forward pass on device planes: r (scaled residual) and f (density
+// feature), n x h x w pixels (h, w multiples of 8); output channel 0 into z.
+void unet_apply(UNet& u, const float* r, const float* f, float* z, int n, int h, int w);
+// Loss of output channel 0 against target and the gradients of every
+// parameter; returns the loss.  With kx, ky (per-sample face coefficients,
+// kx the west face and ky the south face of each pixel) the loss is the
+// A-norm (d.A d)/(t.A t) of d = out - target averaged over the batch, A the
+// 5-point operator with no flux across the sample's edge; without them the
+// mean squared error.
+float unet_loss_grad(UNet& u, const float* r, const float* f, const float* target, const float* kx, const float* ky,
+                     int n, int h, int w);
+void unet_adam(UNet& u, float lr);
+void unet_set_stream(UNet& u, hipStream_t s);  // the stream all kernels of u run on
+hipStream_t unet_stream(UNet& u);
+long unet_param_count(const UNet& u);
+// Parameters and gradients in layer order, each layer W[cout][9][cin] then b[cout].
+void unet_get_params(const UNet& u, float* host);
+void unet_set_params(UNet& u, const float* host);
+void unet_get_grads(const UNet& u, float* host);
+
+}  // namespace tl

@@ -71,6 +71,26 @@ def lib(dim: int) -> ctypes.CDLL:
         L.tl_last_error.restype = ctypes.c_char_p
         L.tl_copy_probe.argtypes = [ctypes.c_long, ctypes.c_int]
         L.tl_copy_probe.restype = ctypes.c_double
+        vp, fp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+        sigs = {
+            "tl_learn_create": (vp, [ctypes.c_ulonglong]),
+            "tl_learn_destroy": (None, [vp]),
+            "tl_learn_add_deck": (ctypes.c_long, [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_ulonglong]),
+            "tl_learn_pairs": (ctypes.c_long, [vp]),
+            "tl_learn_train": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]),
+            "tl_learn_use": (ctypes.c_int, [vp]),
+            "tl_learn_param_count": (ctypes.c_long, [vp]),
+            "tl_learn_get_params": (ctypes.c_int, [vp, fp]),
+            "tl_learn_set_params": (ctypes.c_int, [vp, fp]),
+            "tl_learn_loss_grad": (ctypes.c_double, [vp, fp, fp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                     fp]),
+            "tl_learn_apply": (ctypes.c_int, [vp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         L.tl_set_abort_on_error(0)
         _LIBS[dim] = L
     return _LIBS[dim]
@@ -264,3 +284,78 @@ class ExtChunk:
     def finalise(self):
         self.L.ext_solver_finalise_(self.c)
         self._check()
+
+
+class Learner:
+    """The learned preconditioner of the 2D library: training pairs from FP64
+    PPCG solves, GPU training of the BF16 U-Net, and the network the driver's
+    flexible CG (tl_use_fcg_learned) applies once ``use()`` is called."""
+
+    def __init__(self, seed: int = 1):
+        self.L = lib(2)
+        self.h = self.L.tl_learn_create(seed)
+        if not self.h:
+            raise RuntimeError(self.L.tl_last_error().decode())
+
+    def _check(self, rc):
+        if rc is None or rc < 0:
+            raise RuntimeError(self.L.tl_last_error().decode())
+        return rc
+
+    def add_deck(self, deck: str, snapshots: int = 12, crops: int = 64, crop: int = 64, seed: int = 0) -> int:
+        return self._check(self.L.tl_learn_add_deck(self.h, deck.encode(), snapshots, crops, crop, seed))
+
+    @property
+    def pairs(self) -> int:
+        return self.L.tl_learn_pairs(self.h)
+
+    def train(self, steps: int, batch: int = 8, lr: float = 1e-3):
+        import numpy as np
+        losses = np.zeros(steps)
+        self._check(self.L.tl_learn_train(self.h, steps, batch, lr,
+                                          losses.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return losses
+
+    def use(self, on: bool = True):
+        self.L.tl_learn_use(self.h if on else None)
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(self.L.tl_learn_param_count(self.h), dtype=np.float32)
+        self._check(self.L.tl_learn_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.L.tl_learn_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def loss_grad(self, r, f, target, kx=None, ky=None):
+        """Loss and parameter gradients: the A-norm loss with kx, ky, else MSE."""
+        import numpy as np
+        n, h, w = r.shape
+        fp = ctypes.POINTER(ctypes.c_float)
+        planes = [np.ascontiguousarray(a, dtype=np.float32) for a in (r, f, target)]
+        ks = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (kx, ky)]
+        kp = [None if a is None else a.ctypes.data_as(fp) for a in ks]
+        grads = np.zeros(self.L.tl_learn_param_count(self.h), dtype=np.float32)
+        loss = self.L.tl_learn_loss_grad(self.h, *[a.ctypes.data_as(fp) for a in planes], kp[0], kp[1], n, h, w,
+                                         grads.ctypes.data_as(fp))
+        if loss < 0:
+            raise RuntimeError(self.L.tl_last_error().decode())
+        return loss, grads
+
+    def apply(self, r, f):
+        import numpy as np
+        n, h, w = r.shape
+        fp = ctypes.POINTER(ctypes.c_float)
+        planes = [np.ascontiguousarray(a, dtype=np.float32) for a in (r, f)]
+        z = np.zeros((n, h, w), dtype=np.float32)
+        self._check(self.L.tl_learn_apply(self.h, planes[0].ctypes.data_as(fp), planes[1].ctypes.data_as(fp),
+                                          z.ctypes.data_as(fp), n, h, w))
+        return z
+
+    def close(self):
+        if self.h:
+            self.L.tl_learn_destroy(self.h)
+            self.h = None

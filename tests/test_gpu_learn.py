@@ -1,0 +1,155 @@
+"""The learned preconditioner on the GPU: the BF16 U-Net's forward pass and
+gradients against the CPU oracle (oracle/unet_oracle.py, which rounds to BF16
+where the kernels store), training on pairs recorded from FP64 PPCG, and
+flexible CG with the trained network against the oracle's FP64 PPCG, with
+the ablation against the same method without training.
+
+Bars (no counterpart in the reference, stated here): per-layer gradients
+within 5% relative L2 of the oracle's with cosine >= 0.995 (BF16 rounding
+of activations flips the odd value); output within 2% relative L2; a
+learned-FCG solve meets eps on the true residual and its field summaries
+agree with FP64 PPCG to 1e-7.
+"""
+import numpy as np
+import pytest
+
+import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 5e-2
+GRAD_COS = 0.995
+OUT_RTOL = 2e-2
+FCG_RTOL = 1e-7
+
+
+@pytest.fixture(scope="module")
+def learner(hip):
+    L = hip.Learner(seed=11)
+    yield L
+    L.close()
+
+
+def _inputs(seed, n=2, h=16, w=16):
+    rng = np.random.default_rng(seed)
+    r = rng.normal(size=(n, h, w)).astype(np.float32)
+    f = rng.uniform(-1, 1, size=(n, h, w)).astype(np.float32)
+    t = (0.3 * rng.normal(size=(n, h, w))).astype(np.float32)
+    return r, f, t
+
+
+def _layers(flat):
+    out, k = [], 0
+    for cin, cout, _ in U.SHAPES:
+        nw = cout * 9 * cin
+        out.append((flat[k:k + nw], flat[k + nw:k + nw + cout]))
+        k += nw + cout
+    return out
+
+
+def test_forward_matches_oracle(learner):
+    print("unet forward vs oracle", flush=True)
+    params = learner.params()
+    r, f, _ = _inputs(1)
+    z = learner.apply(r, f)
+    act, _ = U.forward(params, r, f, bf=True)
+    ref = act["out"][..., 0]
+    err = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+    print(f"  relative L2 {err:.3e}", flush=True)
+    assert err <= OUT_RTOL
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_gradients_match_oracle(learner, seed):
+    print(f"unet gradients vs oracle seed={seed}", flush=True)
+    params = learner.params()
+    r, f, t = _inputs(seed)
+    loss_g, grad_g = learner.loss_grad(r, f, t)
+    loss_o, grad_o = U.loss_grad(params, r, f, t, bf=True)
+    assert abs(loss_g - loss_o) <= 1e-2 * loss_o
+    worst = 0.0
+    for i, ((gw, gb), (ow, ob)) in enumerate(zip(_layers(grad_g), _layers(grad_o))):
+        a = np.concatenate([gw, gb]).astype(np.float64)
+        b = np.concatenate([ow, ob])
+        nb = np.linalg.norm(b)
+        if nb == 0.0:
+            assert np.linalg.norm(a) == 0.0
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        cos = float(a @ b / (np.linalg.norm(a) * nb))
+        worst = max(worst, rel)
+        assert rel <= GRAD_RTOL and cos >= GRAD_COS, (i, rel, cos)
+    print(f"  worst layer relative L2 {worst:.3e}", flush=True)
+
+
+def test_anorm_gradients_match_oracle(learner):
+    """The training loss: A-norm of the error with per-pixel face coefficients."""
+    print("unet A-norm gradients vs oracle", flush=True)
+    params = learner.params()
+    r, f, t = _inputs(4)
+    rng = np.random.default_rng(40)
+    kx, ky = rng.uniform(0.05, 20.0, size=(2,) + r.shape).astype(np.float32)
+    loss_g, grad_g = learner.loss_grad(r, f, t, kx, ky)
+    loss_o, grad_o = U.loss_grad(params, r, f, t, bf=True, kx=kx.astype(np.float64), ky=ky.astype(np.float64))
+    assert abs(loss_g - loss_o) <= 1e-2 * loss_o
+    worst = 0.0
+    for i, ((gw, gb), (ow, ob)) in enumerate(zip(_layers(grad_g), _layers(grad_o))):
+        a = np.concatenate([gw, gb]).astype(np.float64)
+        b = np.concatenate([ow, ob])
+        nb = np.linalg.norm(b)
+        if nb == 0.0:
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        worst = max(worst, rel)
+        assert rel <= GRAD_RTOL and float(a @ b / (np.linalg.norm(a) * nb)) >= GRAD_COS, (i, rel)
+    print(f"  worst layer relative L2 {worst:.3e}", flush=True)
+
+
+def test_train_fcg_and_ablation(hip, oracle):
+    """Train on pairs from FP64 PPCG solves of seeded random layouts, then
+    solve held-out layouts with learned FCG (network, then 4 weighted Jacobi
+    sweeps): it must meet eps on the true residual, agree with the oracle's
+    FP64 PPCG, and need fewer iterations than the same flexible CG without
+    the network (sweeps only) and than with the network untrained."""
+    import decks
+    n = 128
+    trained = hip.Learner(seed=5)
+    untrained = hip.Learner(seed=5)
+    try:
+        for s in range(1000, 1004):
+            print(f"training pairs from seed {s}", flush=True)
+            trained.add_deck(decks.random_shapes_2d(n, s, steps=1), snapshots=10, crops=24, crop=32, seed=s)
+        losses = trained.train(300, batch=16, lr=2e-3)
+        first, last = float(np.mean(losses[:20])), float(np.mean(losses[-20:]))
+        print(f"  pairs {trained.pairs}, loss {first:.4f} -> {last:.4f}", flush=True)
+        assert last < 0.7 * first
+
+        for s in (0, 1):
+            deck = decks.random_shapes_2d(n, s, steps=1, extra=("tl_check_result",))
+            fcg_deck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned")
+            ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
+            cg = oracle.run_deck(deck.replace("tl_use_ppcg", "tl_use_cg"), dim=2, reduce_parts=8, threads=8)
+            trained.use()
+            got = hip.run_deck(fcg_deck, dim=2)
+            untrained.use()
+            base = hip.run_deck(fcg_deck, dim=2)
+            trained.use(False)
+            sweeps = hip.run_deck(fcg_deck.replace("tl_use_fcg_learned", "tl_use_fcg_learned\ntl_fcg_no_network"),
+                                  dim=2)
+            sg, so, sb, sw = got["steps"][0], ref["steps"][0], base["steps"][0], sweeps["steps"][0]
+            print(f"  seed {s}: learned FCG {sg['fcg_iters']} it (fallback {sg['fcg_fallback']}), untrained "
+                  f"{sb['fcg_iters']} it (fallback {sb['fcg_fallback']}), sweeps only {sw['fcg_iters']} it "
+                  f"(fallback {sw['fcg_fallback']}), FP64 PPCG {so['iters']} it "
+                  f"({so['cg_steps']} CG + {so['cheby_steps']} outer), plain CG {cg['steps'][0]['iters']} it",
+                  flush=True)
+            assert sg["fcg"] == 1 and sg["fcg_fallback"] == 0
+            assert sg["exact_error"] < 1e-10
+            for key in ("vol", "mass", "ie", "temp"):
+                assert abs(sg[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sg[key], so[key])
+            # the ablation: the trained network is what the iterations are saved by
+            for other in (sb, sw):
+                assert other["fcg_fallback"] or sg["fcg_iters"] < other["fcg_iters"]
+    finally:
+        trained.use(False)
+        trained.close()
+        untrained.close()
