@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -229,13 +230,94 @@ def mixed_form(ref: dict, mix: dict) -> dict:
     }
 
 
+def learned_workload(args, n: int) -> dict:
+    """The learned-FCG configuration: train the BF16 U-Net on (residual,
+    PPCG correction) pairs from FP64 PPCG solves of seeded random layouts of
+    8-64 rectangles and circles (training seeds from 1000), then solve
+    held-out layouts (seeds 0..) with flexible CG (network + 4 weighted Jacobi
+    sweeps, FP64 PPCG as fallback) and with FP64 PPCG.  Returns the bench
+    line's fields: `value` is the FP64 PPCG time, `forms.learned_fcg` the
+    learned solve against it, plus the flexible CG without the network."""
+    learner = tealeaf_hip.Learner(seed=args.seed + 17)
+    t0 = time.perf_counter()
+    for k in range(args.train_decks):
+        s = 1000 + k
+        print(f"bench: training pairs from layout seed {s}", file=sys.stderr, flush=True)
+        learner.add_deck(decks.random_shapes_2d(n, s, steps=1), snapshots=args.snapshots, crops=args.crops,
+                         crop=64, seed=s)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = []
+    chunks = max(1, args.train_steps // 100)
+    for k in range(chunks):
+        lr = 2e-3 * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * k / chunks)))
+        losses += list(learner.train(args.train_steps // chunks, batch=32, lr=lr))
+        print(f"bench: training step {len(losses)}, loss {statistics.mean(losses[-20:]):.4f}", file=sys.stderr,
+              flush=True)
+    t_train = time.perf_counter() - t0
+    fwd_ms, macs = learner.time_forward(1, n, n, reps=3)
+    tflops = 2.0 * macs / (fwd_ms * 1e-3) / 1e12
+
+    steps = args.warmup + args.steps
+    per_seed, fcg_s, ref_s, sweep_s = [], [], [], []
+    for s in range(args.eval_decks):
+        deck = decks.random_shapes_2d(n, s, steps=steps, extra=("tl_check_result",))
+        print(f"bench: held-out layout seed {s}: FP64 PPCG", file=sys.stderr, flush=True)
+        ref = run_solve(deck, 2, args.warmup, timing=False)
+        learner.use()
+        print(f"bench: held-out layout seed {s}: learned FCG", file=sys.stderr, flush=True)
+        got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}"), 2,
+                        args.warmup, timing=False)
+        learner.use(False)
+        print(f"bench: held-out layout seed {s}: flexible CG, sweeps only", file=sys.stderr, flush=True)
+        sw = run_solve(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network"), 2, args.warmup,
+                       timing=False)
+        ref_s.append(ref["s_per_step"])
+        fcg_s.append(got["s_per_step"])
+        sweep_s.append(sw["s_per_step"])
+        for a, b, c in zip(got["timed"], ref["timed"], sw["timed"]):
+            per_seed.append({
+                "seed": s, "step": a["step"],
+                "fcg_s": a["solve_s"], "ppcg_s": b["solve_s"], "sweeps_only_s": c["solve_s"],
+                "fcg_iters": a["fcg_iters"], "fcg_fallback": a["fcg_fallback"],
+                "sweeps_only_iters": c["fcg_iters"], "sweeps_only_fallback": c["fcg_fallback"],
+                "ppcg_iters": b["iters"], "true_rrn": a["exact_error"],
+                "max_summary_rel_err": max(abs(a[k] - b[k]) / abs(b[k]) for k in ("vol", "mass", "ie", "temp")),
+            })
+    pairs = learner.pairs
+    learner.close()
+    return {
+        "ref_s": statistics.mean(ref_s), "cells": n * n,
+        "learned_fcg": {
+            "s_per_step": round(statistics.mean(fcg_s), 6),
+            "speedup_vs_fp64": round(statistics.mean(ref_s) / statistics.mean(fcg_s), 3),
+            "sweeps_only_s_per_step": round(statistics.mean(sweep_s), 6),
+            "per_step": per_seed,
+            "training": {"pairs": int(pairs), "decks": args.train_decks,
+                         "steps": len(losses), "loss_first": float(statistics.mean(losses[:20])),
+                         "loss_last": float(statistics.mean(losses[-20:])), "data_s": round(t_data, 2),
+                         "train_s": round(t_train, 2)},
+            "unet_roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                              "frac": round(tflops / 2500.0, 5), "forward_ms": round(fwd_ms, 3),
+                              "macs_per_forward": macs, "dtype": "bf16 (fp32 accumulate)"},
+        },
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=9, help="timed timesteps")
     ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps before them")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
-                    help="c2: 2D 4096^2 two rectangles; c3: 3D 256^3 cube; c4: 2D 8192^2, 32 random rectangles")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: 2D 4096^2 two rectangles; c3: 3D 256^3 cube; c4: 2D 8192^2, 32 random rectangles; "
+                         "c5: 2D 4096^2 learned FCG on random rectangles and circles")
+    ap.add_argument("--train-decks", type=int, default=8, help="c5: layouts that give training pairs")
+    ap.add_argument("--snapshots", type=int, default=16, help="c5: PPCG iterations sampled per layout")
+    ap.add_argument("--crops", type=int, default=160, help="c5: 64x64 pairs cut from each sample")
+    ap.add_argument("--train-steps", type=int, default=1500, help="c5: Adam steps (batch 32)")
+    ap.add_argument("--eval-decks", type=int, default=2, help="c5: held-out layouts")
+    ap.add_argument("--net-every", type=int, default=1, help="c5: apply the network every m-th FCG iteration")
     ap.add_argument("--n", type=int, default=None, help="cells per side (default 4096, 256 for c3, 8192 for c4)")
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
     ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")
@@ -247,7 +329,7 @@ def main() -> int:
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     args = ap.parse_args()
-    n = args.n or {"c2": 4096, "c3": 256, "c4": 8192}[args.workload]
+    n = args.n or {"c2": 4096, "c3": 256, "c4": 8192, "c5": 4096}[args.workload]
 
     if args.copy_probe:
         L = tealeaf_hip.lib(2)
@@ -267,6 +349,22 @@ def main() -> int:
     if args.steps < 1 or args.warmup < 0:
         print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
         return 2
+
+    if args.workload == "c5":
+        r = learned_workload(args, n)
+        out = {
+            "metric": "solve s/timestep @eps 1e-10 (learned-FCG, mixed PPCG vs FP64 PPCG); HBM GB/s",
+            "value": round(r["ref_s"], 6), "unit": "s/timestep", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * r["ref_s"], 3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"2D {n}x{n} random rectangles and circles (held-out seeds 0-{args.eval_decks - 1}),"
+                                   f" FP64 PPCG vs learned FCG (BF16 U-Net), eps 1e-10, dt 0.004",
+                       "cells": r["cells"], "parallelism": "single chunk, one GPU"},
+            "forms": {"learned_fcg": r["learned_fcg"]},
+            "roofline": r["learned_fcg"]["unet_roofline"], "cpu_baseline": None,
+        }
+        print(json.dumps(out), flush=True)
+        return 0
 
     total = args.warmup + args.steps
     max_iters = 10000 if args.solver != "jacobi" else 20000

@@ -162,14 +162,17 @@ int tl_learn_get_params(void* learner, float* out);
 int tl_learn_set_params(void* learner, const float* in);
 /* Loss of the output against target and its parameter gradients (host
  * planes of n x h x w; h, w multiples of 8).  With kx, ky (each pixel's west
- * and south face coefficients) the A-norm (d.A d)/(t.A t), d = out - target,
- * averaged over the n samples, as in training; with NULL the mean squared
+ * and south face coefficients) the A-norm sum(d.A d) / sum(t.A t) over the n
+ * samples, d = out - target, as in training; with NULL the mean squared
  * error. */
 double tl_learn_loss_grad(void* learner, const float* r, const float* f, const float* target, const float* kx,
                           const float* ky, int n, int h, int w, float* grads);
 /* Network output (channel 0) for host planes r (scaled residual) and f
  * (density feature). */
 int tl_learn_apply(void* learner, const float* r, const float* f, float* z, int n, int h, int w);
+/* Mean milliseconds of one forward pass on device-resident n x h x w planes
+ * over `reps` timed passes (HIP events); *macs = its multiply-accumulates. */
+double tl_learn_time_forward(void* learner, int n, int h, int w, int reps, double* macs);
 
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);

@@ -13,7 +13,7 @@ backward pass of the network built by tealeaf-kokkos_amd/csrc/tl_unet.hip:
   ReLU, channel 0 is the output);
 * 2x2 average pooling, nearest 2x upsampling concatenated before the skip;
 * loss = mean over pixels of (out[..., 0] - target)^2, or the A-norm of
-  d = out[..., 0] - target, (d.A d)/(t.A t) averaged over the samples, A the
+  d = out[..., 0] - target over the batch, sum(d.A d) / sum(t.A t), A the
   5-point operator of per-pixel face coefficients (the training loss).
 
 With ``bf16=True`` values are rounded to bfloat16 (round to nearest even)
@@ -173,11 +173,10 @@ def loss_grad(params, r, f, target, bf: bool = True, kx=None, ky=None):
         loss = float(np.sum(d * d) / pixels)
         g_out[..., 0] = q(2.0 * d / pixels)
     else:
-        n = target.shape[0]
         ad = a_apply(d, kx, ky)
-        den = np.sum(target * a_apply(target, kx, ky), axis=(1, 2))
-        loss = float(np.mean(np.sum(d * ad, axis=(1, 2)) / den))
-        g_out[..., 0] = q(2.0 * ad / (den[:, None, None] * n))
+        den = float(np.sum(target * a_apply(target, kx, ky)))
+        loss = float(np.sum(d * ad) / den)
+        g_out[..., 0] = q(2.0 * ad / den)
     grads = [None] * len(SHAPES)
 
     def back(i, x, y, g, need_dx=True):

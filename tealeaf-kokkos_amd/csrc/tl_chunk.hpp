@@ -218,10 +218,11 @@ Learner* active_learner();
 // Flexible CG on the chunk set up by the driver (u, u0, r = u0 - A u, rro in
 // the device scalars): true when it reached eps on the true residual, false
 // when it stalled (the caller then solves with FP64 PPCG).
-// z = s N(r / s) (network on) or 0, then `smooth` weighted-Jacobi sweeps
-// z += omega D^-1 (r - A z).
+// z = s N(r / s) (network on, every net_every-th iteration) or 0, then
+// `smooth` weighted-Jacobi sweeps z += omega D^-1 (r - A z).
 struct FcgOptions {
     bool network = true;
+    int net_every = 1;
     int smooth = 4;
     double omega = 0.8;
 };

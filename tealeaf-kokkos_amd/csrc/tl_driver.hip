@@ -57,7 +57,7 @@ struct Deck {
     // Flexible CG with the learned preconditioner (no counterpart in the
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
-    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_no_network
+    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_net_every=<m>, tl_fcg_no_network
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -214,6 +214,7 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
             else if (k == "tl_fcg_omega") d.fcg.omega = R();
             else if (k == "tl_fcg_no_network") d.fcg.network = false;
+            else if (k == "tl_fcg_net_every") d.fcg.net_every = I();
             else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();

@@ -257,6 +257,38 @@ int tl_learn_apply(void* h, const float* r, const float* f, float* z, int n, int
     return rc;
 }
 
+// Mean milliseconds of one network forward pass on device-resident n x h x w
+// planes (HIP events over `reps` passes after one untimed pass), and the
+// multiply-accumulates of one pass in *macs.
+double tl_learn_time_forward(void* h, int n, int hh, int ww, int reps, double* macs)
+{
+    double ms = -1.0;
+    guarded([&] {
+        UNet& u = learn_net(*static_cast<Learner*>(h));
+        const size_t px = (size_t)n * hh * ww;
+        DevPlanes p;
+        for (int k = 0; k < 3; ++k) {
+            TL_HIP(hipMalloc(&p.d[k], sizeof(float) * px));
+            TL_HIP(hipMemset(p.d[k], 0, sizeof(float) * px));
+        }
+        unet_apply(u, p.d[0], p.d[1], p.d[2], n, hh, ww);
+        hipEvent_t e0, e1;
+        TL_HIP(hipEventCreate(&e0));
+        TL_HIP(hipEventCreate(&e1));
+        TL_HIP(hipEventRecord(e0, unet_stream(u)));
+        for (int k = 0; k < reps; ++k) unet_apply(u, p.d[0], p.d[1], p.d[2], n, hh, ww);
+        TL_HIP(hipEventRecord(e1, unet_stream(u)));
+        TL_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        TL_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ms = t / reps;
+        if (macs) *macs = unet_forward_macs(n, hh, ww);
+    });
+    return ms;
+}
+
 // Bandwidth of a plain device copy of n doubles repeated reps times (GB/s,
 // read + write): the machine's attainable streaming rate, and the known-size
 // kernel that calibrates HBM byte counters.  Returns a negative value on error.

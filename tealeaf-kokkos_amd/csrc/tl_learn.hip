@@ -8,9 +8,11 @@
 // iterations k, and u* at convergence.  The PPCG correction of snapshot k is
 // e_k = u* - u_k (A e_k = r_k up to the solver tolerance).  Random crop x crop
 // windows of (r_k, density, e_k) become pairs: the network input is r / s
-// (s = rms of the crop's r) with the density feature (log10(rho) - 0.5)/1.5,
-// the target e / (s G), G one global scale over the data set.  Residual and
-// correction share the factor s, so the map stays linear in r as A^-1 is.
+// (s = rms of r_k over the whole mesh, as the flexible CG scales its input)
+// with the density feature (log10(rho) - 0.5)/1.5, the target e / (s G), G
+// one global scale over the data set.  Residual and correction share the
+// factor s, so the map stays linear in r as A^-1 is.  Training minimises the
+// A-norm of the error over each batch, sum (z - e).A(z - e) / sum e.A e.
 //
 // Flexible CG (Notay's FCG with one stored direction): z = s N(r / s),
 // p = z - (z.w_prev / p_prev.w_prev) p_prev, w = A p, alpha = p.r / p.w,
@@ -100,18 +102,28 @@ __global__ void k_extract(Geom g, const double* __restrict__ uk, const double* _
     if (threadIdx.x % WAVE == 0) atomicAdd(ss + q, s);
 }
 
-// r /= s, e /= s G with s = rms of the crop's r
-__global__ void k_normalise(int crop, const double* __restrict__ ss, double inv_g, float* __restrict__ r,
-                            float* __restrict__ e)
+// r /= s, e /= s G over every crop of one snapshot (s the snapshot's rms residual)
+__global__ void k_normalise(int crop, double inv, double inv_g, float* __restrict__ r, float* __restrict__ e)
 {
     const int q = blockIdx.x;
     const long base = (long)q * crop * crop;
-    const double rms = std::sqrt(ss[q] / ((double)crop * crop));
-    const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
     for (int i = threadIdx.x; i < crop * crop; i += blockDim.x) {
         r[base + i] = (float)(r[base + i] * inv);
         e[base + i] = (float)(e[base + i] * inv * inv_g);
     }
+}
+
+// sum of a^2 over the interior of a field
+__global__ void k_field_sumsq(Geom g, const double* __restrict__ a, double* __restrict__ out)
+{
+    const long n = (long)g.nx * g.ny;
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = a[gidx(g, HALO + (int)(i % g.nx), HALO + (int)(i / g.nx), 0)];
+        s += v * v;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
+    if (threadIdx.x % WAVE == 0) atomicAdd(out, s);
 }
 
 __global__ void k_sumsq(const float* __restrict__ a, long n, double* __restrict__ out)
@@ -280,7 +292,7 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
     regrow(L.de, L.de_cap, need, keep);
     regrow(L.dkx, L.dkx_cap, need, keep);
     regrow(L.dky, L.dky_cap, need, keep);
-    regrow(L.scratch, L.scratch_cap, crops, 0);
+    regrow(L.scratch, L.scratch_cap, crops + 1, 0);
     int* dxy = nullptr;
     TL_HIP(hipMalloc(&dxy, sizeof(int) * 2 * crops));
     uint64_t rs = seed;
@@ -296,9 +308,19 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         hipLaunchKernelGGL(k_extract, dim3(crops), dim3(256), 0, nullptr, geom, snaps[k].u, ustar, snaps[k].r, dens, kx,
                            ky, dxy, crop, L.dr + off, L.df + off, L.de + off, L.dkx + off, L.dky + off, L.scratch);
         launched("training pair extraction");
+        // the snapshot's residual rms over the whole mesh scales r and e of
+        // all its crops, as the flexible CG scales the network's input
+        double* fs = L.scratch + crops;
+        TL_HIP(hipMemset(fs, 0, sizeof(double)));
+        hipLaunchKernelGGL(k_field_sumsq, dim3(blocks_for((long)geom.nx * geom.ny)), dim3(256), 0, nullptr, geom,
+                           snaps[k].r, fs);
+        launched("snapshot residual norm");
+        double ss = 0.0;
+        TL_HIP(hipMemcpy(&ss, fs, sizeof(double), hipMemcpyDeviceToHost));
+        const double rms = std::sqrt(ss / ((double)geom.nx * geom.ny));
         const double inv_g = L.escale > 0.0 ? 1.0 / L.escale : 1.0;
-        hipLaunchKernelGGL(k_normalise, dim3(crops), dim3(256), 0, nullptr, crop, L.scratch, inv_g, L.dr + off,
-                           L.de + off);
+        hipLaunchKernelGGL(k_normalise, dim3(crops), dim3(256), 0, nullptr, crop, rms > 0.0 ? 1.0 / rms : 0.0, inv_g,
+                           L.dr + off, L.de + off);
         launched("training pair normalisation");
         TL_HIP(hipDeviceSynchronize());
         L.pairs += crops;
@@ -394,9 +416,9 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     for (int it = 1; it <= max_iters; ++it) {
         // z = s N(r / s), then weighted Jacobi sweeps on A z = r from it
         const double s = std::sqrt(rrn / cells);
-        if (opt.network) {
+        if (opt.network && (it - 1) % std::max(1, opt.net_every) == 0) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
-                               c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, first ? 1 : 0, L->pr, pf);
+                               c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
             launched("FCG input");
             unet_apply(*L->net, L->pr, pf, pz, 1, c.g.ny, c.g.nx);
             hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s, c.f[F_Z]);

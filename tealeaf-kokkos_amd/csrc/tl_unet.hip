@@ -335,10 +335,11 @@ __global__ void k_mse(const u16* out, int cout, const float* target, u16* dout, 
     if (threadIdx.x % WAVE == 0) atomicAdd(loss, s / (float)pixels);
 }
 
-// A-norm loss, one workgroup per sample of h x w pixels: with d = out - t
-// and A the 5-point operator of the sample's face coefficients (kx: west
-// face of each cell, ky: south face; faces on the sample's edge carry no
-// flux), loss += (d.A d)/(t.A t)/nb and dout = 2 A d/(t.A t)/nb.
+// A-norm loss over a batch of h x w samples: with d = out - t and A the
+// 5-point operator of each sample's face coefficients (kx: west face of each
+// cell, ky: south face; faces on the sample's edge carry no flux),
+// loss = sum(d.A d) / sum(t.A t) over the batch and dout = 2 A d / sum(t.A t).
+// First pass, one workgroup per sample: d, and the two sums (into sums[0..1]).
 __device__ inline float a_apply(const float* v, const float* kx, const float* ky, int x, int y, int h, int w)
 {
     const int i = y * w + x;
@@ -351,9 +352,8 @@ __device__ inline float a_apply(const float* v, const float* kx, const float* ky
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_anorm(const u16* out, int cout, const float* target, const float* kx,
-                                               const float* ky, u16* dout, float* dbuf, int h, int w, int nb,
-                                               float* loss)
+__global__ __launch_bounds__(256) void k_anorm_sums(const u16* out, int cout, const float* target, const float* kx,
+                                                    const float* ky, float* dbuf, int h, int w, float* sums)
 {
     __shared__ float red[2][256 / WAVE];
     const int q = blockIdx.x;
@@ -379,18 +379,30 @@ __global__ __launch_bounds__(256) void k_anorm(const u16* out, int cout, const f
         red[1][threadIdx.x / WAVE] = den;
     }
     __syncthreads();
-    num = den = 0.0f;
-    for (int k = 0; k < (int)blockDim.x / WAVE; ++k) {
-        num += red[0][k];
-        den += red[1][k];
+    if (threadIdx.x == 0) {
+        num = den = 0.0f;
+        for (int k = 0; k < (int)blockDim.x / WAVE; ++k) {
+            num += red[0][k];
+            den += red[1][k];
+        }
+        atomicAdd(sums, num);
+        atomicAdd(sums + 1, den);
     }
-    const float inv = den > 0.0f ? 1.0f / (den * (float)nb) : 0.0f;
+}
+
+// Second pass: dout = 2 A d / sum(t.A t) (channel 0), loss = sum(d.A d) / sum(t.A t).
+__global__ __launch_bounds__(256) void k_anorm_grad(const float* kx, const float* ky, const float* dbuf, u16* dout,
+                                                    int cout, int h, int w, const float* sums, float* loss)
+{
+    const int q = blockIdx.x;
+    const long px = (long)h * w, base = (long)q * px;
+    const float inv = sums[1] > 0.0f ? 1.0f / sums[1] : 0.0f;
     for (long i = threadIdx.x; i < px; i += blockDim.x) {
         const int x = (int)(i % w), y = (int)(i / w);
-        const float g = 2.0f * a_apply(d, kxq, kyq, x, y, h, w) * inv;
+        const float g = 2.0f * a_apply(dbuf + base, kx + base, ky + base, x, y, h, w) * inv;
         for (int c = 0; c < cout; ++c) dout[(base + i) * cout + c] = c == 0 ? f2bf(g) : (u16)0;
     }
-    if (threadIdx.x == 0) atomicAdd(loss, num * inv);
+    if (q == 0 && threadIdx.x == 0) *loss = sums[0] * inv;
 }
 
 __global__ void k_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
@@ -470,6 +482,7 @@ struct UNet {
     long target_cap = 0;
     float* dbuf = nullptr;  // A-norm loss: out - target
     long dbuf_cap = 0;
+    float* sums = nullptr;  // A-norm loss: sum(d.A d), sum(t.A t)
     float* loss = nullptr;
     float* stage_r = nullptr;  // input staging (FP32 planes)
     float* stage_f = nullptr;
@@ -567,6 +580,7 @@ UNet::UNet(uint64_t seed)
         TL_HIP(hipMemcpy(k.w, hw.data(), sizeof(float) * k.nw, hipMemcpyHostToDevice));
     }
     loss = dalloc<float>(1);
+    sums = dalloc<float>(2);
     pack_weights();
     TL_HIP(hipStreamSynchronize(stream));
 }
@@ -586,6 +600,7 @@ UNet::~UNet()
     }
     (void)hipFree(target);
     (void)hipFree(dbuf);
+    (void)hipFree(sums);
     (void)hipFree(loss);
     (void)hipFree(stage_r);
     (void)hipFree(stage_f);
@@ -777,8 +792,11 @@ float UNet::loss_and_backward(int n, int h, int w, const float* kx, const float*
             dbuf = dalloc<float>(p0);
             dbuf_cap = p0;
         }
-        hipLaunchKernelGGL(k_anorm, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky, grad[OUT].d, dbuf, h,
-                           w, n, loss);
+        TL_HIP(hipMemsetAsync(sums, 0, sizeof(float) * 2, stream));
+        hipLaunchKernelGGL(k_anorm_sums, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky, dbuf, h, w,
+                           sums);
+        hipLaunchKernelGGL(k_anorm_grad, dim3(n), dim3(256), 0, stream, kx, ky, dbuf, grad[OUT].d, 16, h, w, sums,
+                           loss);
     } else {
         hipLaunchKernelGGL(k_mse, dim3(grid_for(p0)), dim3(256), 0, stream, act[OUT].d, 16, target, grad[OUT].d, p0,
                            loss);
@@ -873,6 +891,18 @@ float unet_loss_grad(UNet& u, const float* r, const float* f, const float* targe
 }
 
 void unet_adam(UNet& u, float lr) { u.adam(lr); }
+
+double unet_forward_macs(int n, int h, int w)
+{
+    // layer -> pyramid level of its output
+    static const int level[UNet::NL] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    double macs = 0.0;
+    for (int l = 0; l < UNet::NL; ++l) {
+        const double px = (double)n * h * w / (double)(1 << (2 * level[l]));
+        macs += px * 9.0 * kShapes[l][0] * kShapes[l][1];
+    }
+    return macs;
+}
 hipStream_t unet_stream(UNet& u) { return u.stream; }
 void unet_set_stream(UNet& u, hipStream_t s) { u.stream = s; }
 long unet_param_count(const UNet& u) { return u.param_count(); }

@@ -17,7 +17,7 @@ void unet_apply(UNet& u, const float* r, const float* f, float* z, int n, int h,
 // Loss of output channel 0 against target and the gradients of every
 // parameter; returns the loss.  With kx, ky (per-sample face coefficients,
 // kx the west face and ky the south face of each pixel) the loss is the
-// A-norm (d.A d)/(t.A t) of d = out - target averaged over the batch, A the
+// A-norm sum(d.A d) / sum(t.A t) of d = out - target over the batch, A the
 // 5-point operator with no flux across the sample's edge; without them the
 // mean squared error.
 float unet_loss_grad(UNet& u, const float* r, const float* f, const float* target, const float* kx, const float* ky,
@@ -26,6 +26,8 @@ void unet_adam(UNet& u, float lr);
 void unet_set_stream(UNet& u, hipStream_t s);  // the stream all kernels of u run on
 hipStream_t unet_stream(UNet& u);
 long unet_param_count(const UNet& u);
+// Multiply-accumulates of one forward pass over n x h x w pixels.
+double unet_forward_macs(int n, int h, int w);
 // Parameters and gradients in layer order, each layer W[cout][9][cin] then b[cout].
 void unet_get_params(const UNet& u, float* host);
 void unet_set_params(UNet& u, const float* host);

@@ -86,6 +86,8 @@ def lib(dim: int) -> ctypes.CDLL:
             "tl_learn_loss_grad": (ctypes.c_double, [vp, fp, fp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                      fp]),
             "tl_learn_apply": (ctypes.c_int, [vp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "tl_learn_time_forward": (ctypes.c_double, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                        dp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -354,6 +356,14 @@ class Learner:
         self._check(self.L.tl_learn_apply(self.h, planes[0].ctypes.data_as(fp), planes[1].ctypes.data_as(fp),
                                           z.ctypes.data_as(fp), n, h, w))
         return z
+
+    def time_forward(self, n: int, h: int, w: int, reps: int = 5):
+        """(ms per forward pass, multiply-accumulates per pass)."""
+        macs = ctypes.c_double()
+        ms = self.L.tl_learn_time_forward(self.h, n, h, w, reps, ctypes.byref(macs))
+        if ms < 0:
+            raise RuntimeError(self.L.tl_last_error().decode())
+        return ms, macs.value
 
     def close(self):
         if self.h:
