@@ -244,14 +244,15 @@ def learned_workload(args, n: int) -> dict:
         s = 1000 + k
         print(f"bench: training pairs from layout seed {s}", file=sys.stderr, flush=True)
         learner.add_deck(decks.random_shapes_2d(n, s, steps=1), snapshots=args.snapshots, crops=args.crops,
-                         crop=64, seed=s)
+                         crop=args.crop, seed=s)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     losses = []
     chunks = max(1, args.train_steps // 100)
     for k in range(chunks):
         lr = 2e-3 * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * k / chunks)))
-        losses += list(learner.train(args.train_steps // chunks, batch=32, lr=lr))
+        losses += list(learner.train(args.train_steps // chunks, batch=max(2, 32 * 64 * 64 // (args.crop * args.crop)),
+                                     lr=lr))
         print(f"bench: training step {len(losses)}, loss {statistics.mean(losses[-20:]):.4f}", file=sys.stderr,
               flush=True)
     t_train = time.perf_counter() - t0
@@ -318,6 +319,7 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=1500, help="c5: Adam steps (batch 32)")
     ap.add_argument("--eval-decks", type=int, default=2, help="c5: held-out layouts")
     ap.add_argument("--net-every", type=int, default=1, help="c5: apply the network every m-th FCG iteration")
+    ap.add_argument("--crop", type=int, default=64, help="c5: training pair size (cells per side)")
     ap.add_argument("--n", type=int, default=None, help="cells per side (default 4096, 256 for c3, 8192 for c4)")
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
     ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")

@@ -53,7 +53,7 @@ struct Deck {
     // Mixed-precision PPCG (no counterpart in the reference): FP32 inner
     // steps, the FP64 residual replaced once ||r||^2 fell by this factor.
     bool ppcg_mixed = false;
-    double mixed_replace = 1.0e-4;
+    double mixed_replace = 1.0e-2;
     // Flexible CG with the learned preconditioner (no counterpart in the
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
@@ -397,8 +397,11 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     bool p_pending = false;
     // mixed-precision PPCG state: rrn at the last residual replacement, and
     // true-residual checks that failed without halving the true rrn
-    bool mixed = d.use_ppcg && d.ppcg_mixed && ppcg_mixed_supported(c, d.ppcg_inner_steps);
-    double rrn_replaced = 0.0, best_true = HUGE_VAL;
+    // mixed_form: the solve uses residual replacement and accepts only the
+    // true residual; mixed: the FP32 inner steps are (still) in use
+    const bool mixed_form = d.use_ppcg && d.ppcg_mixed && ppcg_mixed_supported(c, d.ppcg_inner_steps);
+    bool mixed = mixed_form;
+    double rrn_replaced = 0.0, last_true = HUGE_VAL;
     int stalls = 0;
     std::vector<double> cg_alphas, cg_betas;
     std::vector<double> ch_alphas(max_iters + 3, 0.0), ch_betas(max_iters + 3, 0.0);
@@ -492,30 +495,6 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 if (mixed) {
                     TL_TIMED(c, K_PPCG_MIXED, op_finish_rrn(c, op_ppcg_outer_mixed(c, d.ppcg_inner_steps)));
                     ++si.mixed_steps;
-                    p_pending = true;
-                    error = op_error(c);
-                    const bool converged = std::fabs(error) < d.eps;
-                    if (converged || error < rrn_replaced * d.mixed_replace) {
-                        // FP64 residual replacement r = u0 - A u; beta is
-                        // recomputed from it.  Convergence is accepted only
-                        // on the true residual.
-                        op_calc_residual(c);
-                        op_2norm_replace_dev(c);
-                        error = op_error(c);
-                        rrn_replaced = error;
-                        ++si.replacements;
-                        if (converged && !(std::fabs(error) < d.eps)) {
-                            // the FP32 steps no longer reduce the true
-                            // residual: finish with the FP64 inner steps
-                            stalls = error > 0.5 * best_true ? stalls + 1 : 0;
-                            best_true = std::min(best_true, error);
-                            if (stalls >= 3) {
-                                mixed = false;
-                                si.mixed_fallback = true;
-                            }
-                        }
-                    }
-                    if (solve_hooks().on_outer) solve_hooks().on_outer(c, cheby_calc_steps);
                 } else if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
                     TL_TIMED(c, K_PPCG_OUTER, op_finish_rrn(c, op_ppcg_outer_tb(c, d.ppcg_inner_steps)));
@@ -529,6 +508,30 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 p_pending = true;
                 error = op_error(c);
                 if (solve_hooks().on_outer) solve_hooks().on_outer(c, cheby_calc_steps);
+                if (mixed_form) {
+                    const bool converged = std::fabs(error) < d.eps;
+                    if (converged || error < rrn_replaced * d.mixed_replace) {
+                        // FP64 residual replacement r = u0 - A u; beta is
+                        // recomputed from it.  Convergence is accepted only
+                        // on the true residual.
+                        op_calc_residual(c);
+                        op_2norm_replace_dev(c);
+                        error = op_error(c);
+                        rrn_replaced = error;
+                        ++si.replacements;
+                        if (converged && !(std::fabs(error) < d.eps)) {
+                            // checks that keep failing without 10% progress:
+                            // the FP32 steps have stalled, finish in FP64
+                            stalls = error > 0.9 * last_true ? stalls + 1 : 0;
+                            last_true = error;
+                            if (mixed && stalls >= 5) {
+                                mixed = false;
+                                stalls = 0;
+                                si.mixed_fallback = true;
+                            }
+                        }
+                    }
+                }
             }
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {

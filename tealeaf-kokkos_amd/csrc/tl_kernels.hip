@@ -844,9 +844,13 @@ inline void check_launch()
 
 // Plain device-to-device copy of n doubles, 16 bytes per lane per step: the
 // calibration kernel for HBM byte counters (it moves exactly 8n bytes each way).
-__global__ __launch_bounds__(BLOCK) void k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b, long n2)
+constexpr int COPY_BLOCK = 512;
+constexpr unsigned COPY_GRID = 65536;  // the fastest shape of a launch sweep on MI355X
+
+__global__ __launch_bounds__(COPY_BLOCK) void k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                                            long n2)
 {
-    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n2; i += (long)gridDim.x * BLOCK) b[i] = a[i];
+    for (long i = (long)blockIdx.x * COPY_BLOCK + threadIdx.x; i < n2; i += (long)gridDim.x * COPY_BLOCK) b[i] = a[i];
 }
 
 // Second phase of a reduction whose first phase wrote `nblocks` partials.
@@ -1341,11 +1345,10 @@ double stream_copy_probe(long n, int reps)
     hipEvent_t e0, e1;
     TL_HIP(hipEventCreate(&e0));
     TL_HIP(hipEventCreate(&e1));
-    const unsigned grid = 256 * 32;
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
+    hipLaunchKernelGGL(k_stream_copy, dim3(COPY_GRID), dim3(COPY_BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
     TL_HIP(hipEventRecord(e0, 0));
     for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
+        hipLaunchKernelGGL(k_stream_copy, dim3(COPY_GRID), dim3(COPY_BLOCK), 0, 0, (const double2*)a, (double2*)b, n / 2);
     TL_HIP(hipEventRecord(e1, 0));
     TL_HIP(hipEventSynchronize(e1));
     float ms = 0.0f;
