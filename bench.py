@@ -126,8 +126,9 @@ def cpu_baseline(n: int, threads: int, steps: list[dict], budget_s: float = 20.0
 def kernel_probe(n: int, launches: int) -> dict:
     """A short program for hardware-counter runs: the 2D n^2 two-rectangle
     chunk set up through the ext ABI (generate, CG init, 40 CG iterations),
-    then `launches` fused PPCG outer iterations (tl_ppcg_outer_fused, 10
-    inner steps).  Prints the mean launch time from HIP events."""
+    then `launches` + 1 fused PPCG outer iterations (tl_ppcg_outer_fused, 10
+    inner steps) and as many mixed-precision ones (tl_ppcg_outer_mixed).
+    Prints the mean host time per FP64 launch."""
     import ctypes
 
     L = tealeaf_hip.lib(2)
@@ -161,8 +162,14 @@ def kernel_probe(n: int, launches: int) -> dict:
         if rc != 0:
             raise RuntimeError(L.tl_last_error().decode())
     dt = (time.perf_counter() - t0) / launches
+    # the same outer iteration with FP32 inner steps
+    for _ in range(launches + 1):
+        rc = L.tl_ppcg_outer_mixed(ch.c, ctypes.c_double(1e-3), ctypes.c_double(2000.0), ca, cb, 10,
+                                   ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError(L.tl_last_error().decode())
     ch.close()
-    return {"probe": "ppcg_outer", "n": n, "launches": launches, "host_s_per_launch": dt}
+    return {"probe": "ppcg_outer, ppcg_outer_mixed", "n": n, "launches": launches, "host_s_per_launch": dt}
 
 
 def run_solve(deck: str, dim: int, warmup: int, timing: bool = True) -> dict:
