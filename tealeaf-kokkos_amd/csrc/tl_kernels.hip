@@ -449,21 +449,38 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             s1 = c1v;
         }
         double kyc0 = k.ky[gidx(g, k0, j0, 0)], kyc1 = k.ky[gidx(g, k1, j0, 0)];
+        // Raw inputs are loaded one row ahead of their use, so each row's
+        // loads are in flight while the previous row is computed: for row
+        // j + 1 the new p (n, en) and ky; for row j the x-face coefficients.
+        const int jn = min(j0 + 1, ny + 1);
+        double n0 = pnew(k0, jn), n1 = pnew(k1, jn);
+        double en = edge ? pnew(ce, jn) : 0.0;
+        double kyn0 = k.ky[gidx(g, k0, j0 + 1, 0)], kyn1 = k.ky[gidx(g, k1, j0 + 1, 0)];  // rows <= ny + 2
+        double kx0 = k.kx[gidx(g, min(c0, nx + 2), j0, 0)], kx1 = k.kx[gidx(g, min(c1, nx + 2), j0, 0)];
+        double kxl = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j0, 0)] : 0.0;
         for (int j = j0; j < j1; ++j) {
-            const int jn = min(j + 1, ny + 1);
-            double n0 = pnew(k0, jn), n1 = pnew(k1, jn);
-            const double en = edge ? pnew(ce, jn) : 0.0;
+            // next row's raw inputs (row j + 2 for p, q, ky; row j + 1 for kx)
+            const int j2 = min(j + 2, ny + 1), j1k = min(j + 1, ny + 1);
+            const long a0 = gidx(g, k0, j2, 0), a1 = gidx(g, k1, j2, 0), ae = gidx(g, ce, j2, 0);
+            const double pp0 = p[a0], pp1 = p[a1], qq0 = q[a0], qq1 = q[a1];
+            const double ppe = edge ? p[ae] : 0.0, qqe = edge ? q[ae] : 0.0;
+            // ky of row j + 2's south face = the north face of row j + 1
+            const double kyq0 = k.ky[gidx(g, k0, min(j + 2, ny + 2), 0)];
+            const double kyq1 = k.ky[gidx(g, k1, min(j + 2, ny + 2), 0)];
+            const double kxq0 = k.kx[gidx(g, min(c0, nx + 2), j1k, 0)];
+            const double kxq1 = k.kx[gidx(g, min(c1, nx + 2), j1k, 0)];
+            const double kxql = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j1k, 0)] : 0.0;
+
+            double nn0 = n0, nn1 = n1;
             if (j == ny + 1) {  // top face
-                n0 = c0v;
-                n1 = c1v;
+                nn0 = c0v;
+                nn1 = c1v;
             }
             const long i0 = gidx(g, k0, j, 0), i1 = gidx(g, k1, j, 0);
             // kx lives on faces up to nx + 2: the lane past the last column
             // must hold kx of the right face for its west neighbour's DPP
-            const double kx0 = k.kx[gidx(g, min(c0, nx + 2), j, 0)], kx1 = k.kx[gidx(g, min(c1, nx + 2), j, 0)];
-            const double kyn0 = k.ky[i0 + g.sx], kyn1 = k.ky[i1 + g.sx];
             double kxe1 = dpp_from_east(kx0);
-            if (lane == WAVE - 1) kxe1 = k.kx[gidx(g, min(c1 + 1, nx + 2), j, 0)];
+            if (lane == WAVE - 1) kxe1 = kxl;
             double w0 = dpp_from_west(c1v), e1 = dpp_from_east(c0v);
             if (lane == 0) w0 = ec;
             if (lane == WAVE - 1) e1 = ec;
@@ -471,8 +488,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             if (c0 == HALO) w0 = c0v;
             if (c0 == nx + 1) e0 = c0v;
             if (c1 == nx + 1) e1 = c1v;
-            const double m0 = (1.0 + (kx1 + kx0) + (kyn0 + kyc0)) * c0v - (kx1 * e0 + kx0 * w0) - (kyn0 * n0 + kyc0 * s0);
-            const double m1 = (1.0 + (kxe1 + kx1) + (kyn1 + kyc1)) * c1v - (kxe1 * e1 + kx1 * w1) - (kyn1 * n1 + kyc1 * s1);
+            const double m0 = (1.0 + (kx1 + kx0) + (kyn0 + kyc0)) * c0v - (kx1 * e0 + kx0 * w0) - (kyn0 * nn0 + kyc0 * s0);
+            const double m1 = (1.0 + (kxe1 + kx1) + (kyn1 + kyc1)) * c1v - (kxe1 * e1 + kx1 * w1) - (kyn1 * nn1 + kyc1 * s1);
             if (v0) {
                 p_out[i0] = c0v;
                 w[i0] = m0;
@@ -490,6 +507,15 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             ec = en;
             kyc0 = kyn0;
             kyc1 = kyn1;
+            kx0 = kxq0;
+            kx1 = kxq1;
+            kxl = kxql;
+            // the new p of row j + 2 from the prefetched values (CGCalcP's beta p + r)
+            n0 = beta * pp0 + qq0;
+            n1 = beta * pp1 + qq1;
+            en = beta * ppe + qqe;
+            kyn0 = kyq0;
+            kyn1 = kyq1;
         }
     }
     double v[1] = {acc};
