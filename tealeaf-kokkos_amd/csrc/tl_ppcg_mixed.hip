@@ -26,9 +26,9 @@
 // The sweep is the temporal-blocked wavefront of the FP64 kernel: a wave
 // owns 128 columns (two per lane), streams down a strip of rows and keeps
 // per level a three-row window of sd, two-row windows of r and du; the FP32
-// kx/ky rows live in LDS rings.  u', r' are formed again from u, p, r, w
-// reloaded at the output row (the second read of r, w mostly hits the
-// caches), so no FP64 value is carried across rows.
+// kx/ky rows and the FP64 r' = r - alpha w of level 0 live in LDS rings
+// until the output row, where u' = u + alpha p is formed from u, p loaded
+// there, so every input is read from memory once.
 #include "tl_chunk.hpp"
 
 #include <algorithm>
@@ -82,7 +82,7 @@ struct MXState {
     float duw[L][2][2];   // du after level s (s >= 1), slot (round) % 2
     double pin[2][2][2];  // prefetched level-0 rows: r, w
     float pk[2][2][2];    // prefetched level-0 rows: kx, ky
-    double po[2][4][2];   // prefetched output rows: r, w, u, p
+    double po[2][2][2];   // prefetched output rows: u, p
     double acc;
 };
 
@@ -94,20 +94,11 @@ struct MXCtx {
     bool wcl0, wcl1, ecl0, ecl1;
     bool out0, out1;
     double alpha, inv_theta;
-    float4* kxr;  // LDS ring [L+1][64]: kx at c0, c1, c1 + 1
-    float2* kyr;  // LDS ring [L+1][64]: ky at c0, c1
+    float4* kxr;   // LDS ring [L+1][64]: kx at c0, c1, c1 + 1
+    float2* kyr;   // LDS ring [L+1][64]: ky at c0, c1
+    double2* rpr;  // LDS ring [L+1][64]: r' at c0, c1
     const float* coef;
 };
-
-__device__ __forceinline__ const double* drow(const double* a, const Geom& g, int y)
-{
-    return a + g.base + (long)y * g.sx;
-}
-
-__device__ __forceinline__ const float* frow(const float* a, const Geom& g, int y)
-{
-    return a + g.base + (long)y * g.sx;
-}
 
 template <class T>
 __device__ __forceinline__ T ldb(const T* row, unsigned boff)
@@ -127,38 +118,32 @@ __device__ __forceinline__ int ring(int row, int n)
 }
 
 // Prefetch for round y + 2: level-0 inputs of row y + 2 and the output row
-// y + 2 - L.  Every load is issued on every round with clamped rows, so the
-// count of loads in flight never depends on a branch.
-template <int L>
+// y + 2 - L.  Every load is issued on every round, so the count of loads in
+// flight never depends on a branch.  Waves that reach a y face (YE) clamp
+// the rows; the others read rows y + 2 and y + 2 - L as they are (the wave
+// placement keeps them inside the mesh), one row offset for every array.
+template <int L, bool YE>
 __device__ __forceinline__ void mx_load(const MXArgs& A, const MXCtx& c, int y, int slot, MXState<L>& st)
 {
-    const int yr = min(y, c.ny + 1);
-    const int yk = min(y, c.ny + 2);  // ky also on the top face
-    const double* r = drow(A.r_in, c.g, yr);
-    const double* w = drow(A.w, c.g, yr);
-    st.pin[slot][0][0] = ldb(r, c.bc0);
-    st.pin[slot][0][1] = ldb(r, c.bc1);
-    st.pin[slot][1][0] = ldb(w, c.bc0);
-    st.pin[slot][1][1] = ldb(w, c.bc1);
-    const float* kx = frow(A.kx, c.g, yr);
-    const float* ky = frow(A.ky, c.g, yk);
-    st.pk[slot][0][0] = ldb(kx, c.hf0);
-    st.pk[slot][0][1] = ldb(kx, c.hf1);
-    st.pk[slot][1][0] = ldb(ky, c.hc0);
-    st.pk[slot][1][1] = ldb(ky, c.hc1);
-    const int yo = min(max(y - L, c.Y0), c.yhi);
-    const double* ro = drow(A.r_in, c.g, yo);
-    const double* wo = drow(A.w, c.g, yo);
-    const double* uo = drow(A.u, c.g, yo);
-    const double* p = drow(A.p, c.g, yo);
-    st.po[slot][0][0] = ldb(ro, c.bc0);
-    st.po[slot][0][1] = ldb(ro, c.bc1);
-    st.po[slot][1][0] = ldb(wo, c.bc0);
-    st.po[slot][1][1] = ldb(wo, c.bc1);
-    st.po[slot][2][0] = ldb(uo, c.bc0);
-    st.po[slot][2][1] = ldb(uo, c.bc1);
-    st.po[slot][3][0] = ldb(p, c.bc0);
-    st.po[slot][3][1] = ldb(p, c.bc1);
+    // byte offsets of the rows (the host keeps every field under 4 GiB);
+    // each load is the array's base plus a per-lane 32-bit offset
+    const unsigned oy = (unsigned)(c.g.base + (long)y * c.g.sx);
+    const unsigned oyr = YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 1) * c.g.sx) : oy;
+    const unsigned oyk = YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 2) * c.g.sx) : oy;  // ky also on the top face
+    const unsigned oyo = YE ? (unsigned)(c.g.base + (long)min(max(y - L, c.Y0), c.yhi) * c.g.sx)
+                            : oy - (unsigned)(L * c.g.sx);
+    st.pin[slot][0][0] = ldb(A.r_in, 8u * oyr + c.bc0);
+    st.pin[slot][0][1] = ldb(A.r_in, 8u * oyr + c.bc1);
+    st.pin[slot][1][0] = ldb(A.w, 8u * oyr + c.bc0);
+    st.pin[slot][1][1] = ldb(A.w, 8u * oyr + c.bc1);
+    st.pk[slot][0][0] = ldb(A.kx, 4u * oyr + c.hf0);
+    st.pk[slot][0][1] = ldb(A.kx, 4u * oyr + c.hf1);
+    st.pk[slot][1][0] = ldb(A.ky, 4u * oyk + c.hc0);
+    st.pk[slot][1][1] = ldb(A.ky, 4u * oyk + c.hc1);
+    st.po[slot][0][0] = ldb(A.u, 8u * oyo + c.bc0);
+    st.po[slot][0][1] = ldb(A.u, 8u * oyo + c.bc1);
+    st.po[slot][1][0] = ldb(A.p, 8u * oyo + c.bc0);
+    st.po[slot][1][1] = ldb(A.p, 8u * oyo + c.bc1);
 }
 
 // One round: level 0 at row y, level s at row y - s (s = 1..L); level L's
@@ -180,12 +165,14 @@ __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXStat
     const int ry = ring(y, L + 1);
     float kyn0, kyn1;
     if (!YE || y <= ny + 1) {
+        double rp[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const double rp = st.pin[P][0][q] - c.alpha * st.pin[P][1][q];
-            st.rw[0][R0][q] = (float)rp;
-            st.sdw[0][S0][q] = (float)(rp * c.inv_theta);
+            rp[q] = st.pin[P][0][q] - c.alpha * st.pin[P][1][q];
+            st.rw[0][R0][q] = (float)rp[q];
+            st.sdw[0][S0][q] = (float)(rp[q] * c.inv_theta);
         }
+        c.rpr[ry * WAVE + lane] = make_double2(rp[0], rp[1]);
         const float kx0 = st.pk[P][0][0], kx1 = st.pk[P][0][1];
         c.kxr[ry * WAVE + lane] = make_float4(kx0, kx1, f_from_east(kx0), 0.0f);
         c.kyr[ry * WAVE + lane] = make_float2(st.pk[P][1][0], st.pk[P][1][1]);
@@ -203,14 +190,16 @@ __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXStat
             st.rw[0][R0][q] = st.rw[0][R1][q];
         }
     }
-    // the output row of this round, loaded two rounds ago
-    double po[4][2];
+    // the output row of this round: u, p loaded two rounds ago, r' from
+    // the ring (row y - L shares no slot with row y)
+    double po[2][2];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
         po[k][0] = st.po[P][k][0];
         po[k][1] = st.po[P][k][1];
     }
-    mx_load<L>(A, c, y + 2, P, st);
+    const double2 rpo = c.rpr[ring(y - L, L + 1) * WAVE + lane];
+    mx_load<L, YE>(A, c, y + 2, P, st);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
@@ -281,17 +270,15 @@ __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXStat
             const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
             const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
-            const int zs = min(max(z, 0), ny + 2 * HALO - 1);
-            double* ro = const_cast<double*>(drow(A.r_out, c.g, zs));
-            double* uo = const_cast<double*>(drow(A.u, c.g, zs));
-            const double rp0 = po[0][0] - c.alpha * po[1][0], rp1 = po[0][1] - c.alpha * po[1][1];
-            const double up0 = po[2][0] + c.alpha * po[3][0], up1 = po[2][1] + c.alpha * po[3][1];
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)(YE ? min(max(z, 0), ny + 2 * HALO - 1) : z) * c.g.sx);
+            const double rp0 = rpo.x, rp1 = rpo.y;
+            const double up0 = po[0][0] + c.alpha * po[1][0], up1 = po[0][1] + c.alpha * po[1][1];
             const double rr0 = rp0 + (double)(t0 - (float)rp0);
             const double rr1 = rp1 + (double)(t1 - (float)rp1);
-            stb(ro, b0, rr0);
-            stb(uo, b0, up0 + (double)d0);
-            stb(ro, b1, rr1);
-            stb(uo, b1, up1 + (double)d1);
+            stb(A.r_out, oz + b0, rr0);
+            stb(A.u, oz + b0, up0 + (double)d0);
+            stb(A.r_out, oz + b1, rr1);
+            stb(A.u, oz + b1, up1 + (double)d1);
             st.acc += o0 ? rr0 * rr0 : 0.0;
             st.acc += o1 ? rr1 * rr1 : 0.0;
         }
@@ -312,8 +299,8 @@ __device__ __forceinline__ void mx_wave(const MXArgs& A, const MXCtx& c)
         }
     }
     st.acc = 0.0;
-    mx_load<L>(A, c, c.ylo, 0, st);
-    mx_load<L>(A, c, c.ylo + 1, 1, st);
+    mx_load<L, YE>(A, c, c.ylo, 0, st);
+    mx_load<L, YE>(A, c, c.ylo + 1, 1, st);
     const int yend = c.yhi + L;
     for (int y = c.ylo; y <= yend; y += 6) {
         mx_round<L, XE, YE, 0>(A, c, st, y);
@@ -333,6 +320,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVE
 {
     __shared__ float4 kx_ring[(L + 1) * WAVE];
     __shared__ float2 ky_ring[(L + 1) * WAVE];
+    __shared__ double2 rp_ring[(L + 1) * WAVE];
     __shared__ float coef[2 * L];
     const int lane = threadIdx.x;
     if (lane < 2 * L) coef[lane] = A.coef[lane];
@@ -376,9 +364,12 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVE
     c.inv_theta = A.inv_theta;
     c.kxr = kx_ring;
     c.kyr = ky_ring;
+    c.rpr = rp_ring;
     c.coef = coef;
     const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
-    const bool yedge = c.ylo == HALO || c.yhi + L >= c.ny + 1;
+    // waves near a y face clamp their rows; the others read rows ylo + 2 - L
+    // .. yhi + L + 7 (prefetch and the six-round unroll) unclamped
+    const bool yedge = c.ylo == HALO || c.Y0 < 2 * L + HALO || c.yhi + L + 8 >= c.ny + 1;
     if (xedge) {
         if (yedge) mx_wave<L, true, true>(A, c);
         else mx_wave<L, true, false>(A, c);
@@ -406,7 +397,9 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, lo
 
 bool ppcg_mixed_supported(const Chunk& c, int inner_steps)
 {
-    return !c.preconditioner && mx_levels_built(inner_steps) && c.g.nx >= 2 && c.g.ny >= 2;
+    // the kernel addresses a field with 32-bit byte offsets
+    const bool fits = (double)c.g.alloc * sizeof(double) < 4294967296.0;
+    return !c.preconditioner && mx_levels_built(inner_steps) && c.g.nx >= 2 && c.g.ny >= 2 && fits;
 }
 
 void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int inner_steps, double theta)

@@ -33,7 +33,9 @@ int tb_rows(int ny, int tiles_x, int L, int simds)
 
 bool ppcg_tb_supported(const Chunk& c, int inner_steps)
 {
-    return TL_DIM == 2 && !c.preconditioner && inner_steps >= 1 && c.g.nx >= 2 && c.g.ny >= 2;
+    // the kernel addresses a field with 32-bit byte offsets
+    const bool fits = (double)c.g.alloc * sizeof(double) < 4294967296.0;
+    return TL_DIM == 2 && !c.preconditioner && inner_steps >= 1 && c.g.nx >= 2 && c.g.ny >= 2 && fits;
 }
 
 std::vector<int> tb_passes(int inner_steps)

@@ -100,17 +100,6 @@ struct TBCtx {
     const double* coef;  // LDS copy of a[0..L), b[0..L)
 };
 
-// Loads of row y into a prefetch slot.  Every load is issued on every
-// round, with the row clamped into the mesh (rows past the top repeat the
-// last one; u and p are only used on the wave's output rows), so that the
-// number of memory operations in flight is the same on every path and the
-// compiler can wait for exactly the oldest slot instead of draining all.
-// Row y of an array, as a wave-uniform pointer.
-__device__ __forceinline__ const double* row_of(const double* a, const Geom& g, int y)
-{
-    return a + g.base + (long)y * g.sx;
-}
-
 __device__ __forceinline__ double ld(const double* row, unsigned boff)
 {
     return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row) + boff);
@@ -122,37 +111,35 @@ __device__ __forceinline__ void st(double* row, unsigned boff, double v)
 }
 
 // Loads of row y into a prefetch slot.  Every load is issued on every
-// round, with the row clamped into the mesh (rows past the top repeat the
-// last one; u and p are only used on the wave's output rows), so that the
-// number of memory operations in flight is the same on every path and the
-// compiler can wait for exactly the oldest slot instead of draining all.
-// Addresses are a uniform row pointer plus a per-lane 32-bit byte offset.
-template <bool FIRST>
+// round, so that the number of memory operations in flight is the same on
+// every path and the compiler can wait for exactly the oldest slot instead
+// of draining all.  Waves that reach a y face (YE) clamp the rows into the
+// mesh (rows past the top repeat the last one; u and p are only used on the
+// wave's output rows); the others read row y as it is (their placement keeps
+// it inside the mesh).  An address is the array's base plus a per-lane
+// 32-bit byte offset of row and column (every field is under 4 GiB).
+template <bool FIRST, bool YE>
 __device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
 {
-    const int yr = min(y, c.ny + 1);
-    const int yk = min(y, c.ny + 2);  // ky also on the top face
-    const int yo = min(max(y, c.Y0), c.yhi);
-    const double* r = row_of(A.r_in, c.g, yr);
-    v[0][0] = ld(r, c.bc0);
-    v[0][1] = ld(r, c.bc1);
-    const double* b = row_of(FIRST ? A.w : A.sd_in, c.g, yr);
-    v[1][0] = ld(b, c.bc0);
-    v[1][1] = ld(b, c.bc1);
-    const double* u = row_of(A.u, c.g, yo);
-    v[2][0] = ld(u, c.bc0);
-    v[2][1] = ld(u, c.bc1);
+    const unsigned oy = (unsigned)(c.g.base + (long)y * c.g.sx);
+    const unsigned oyr = 8u * (YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 1) * c.g.sx) : oy);
+    const unsigned oyk = 8u * (YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 2) * c.g.sx) : oy);  // ky: top face
+    const unsigned oyo = 8u * (YE ? (unsigned)(c.g.base + (long)min(max(y, c.Y0), c.yhi) * c.g.sx) : oy);
+    v[0][0] = ld(A.r_in, oyr + c.bc0);
+    v[0][1] = ld(A.r_in, oyr + c.bc1);
+    const double* b = FIRST ? A.w : A.sd_in;
+    v[1][0] = ld(b, oyr + c.bc0);
+    v[1][1] = ld(b, oyr + c.bc1);
+    v[2][0] = ld(A.u, oyo + c.bc0);
+    v[2][1] = ld(A.u, oyo + c.bc1);
     if (FIRST) {
-        const double* p = row_of(A.p, c.g, yo);
-        v[3][0] = ld(p, c.bc0);
-        v[3][1] = ld(p, c.bc1);
+        v[3][0] = ld(A.p, oyo + c.bc0);
+        v[3][1] = ld(A.p, oyo + c.bc1);
     }
-    const double* kx = row_of(A.kx, c.g, yr);
-    v[4][0] = ld(kx, c.bf0);
-    v[4][1] = ld(kx, c.bf1);
-    const double* ky = row_of(A.ky, c.g, yk);
-    v[5][0] = ld(ky, c.bc0);
-    v[5][1] = ld(ky, c.bc1);
+    v[4][0] = ld(A.kx, oyr + c.bf0);
+    v[4][1] = ld(A.kx, oyr + c.bf1);
+    v[5][0] = ld(A.ky, oyk + c.bc0);
+    v[5][1] = ld(A.ky, oyk + c.bc1);
 }
 
 __device__ __forceinline__ int ring(int row, int n)
@@ -223,7 +210,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         }
     }
     // prefetch row y + 2 into the slot just consumed
-    tb_load<FIRST>(A, c, y + 2, v);
+    tb_load<FIRST, YE>(A, c, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
@@ -295,18 +282,15 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
             const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
             const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
-            const int zs = min(max(z, 0), ny + 2 * HALO - 1);
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)(YE ? min(max(z, 0), ny + 2 * HALO - 1) : z) * c.g.sx);
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
-            double* ro = const_cast<double*>(row_of(A.r_out, c.g, zs));
-            double* uo = const_cast<double*>(row_of(A.u, c.g, zs));
-            tb::st(ro, b0, rn0);
-            tb::st(uo, b0, u.x);
-            tb::st(ro, b1, rn1);
-            tb::st(uo, b1, u.y);
+            tb::st(A.r_out, oz + b0, rn0);
+            tb::st(A.u, oz + b0, u.x);
+            tb::st(A.r_out, oz + b1, rn1);
+            tb::st(A.u, oz + b1, u.y);
             if (!LAST) {
-                double* so = const_cast<double*>(row_of(A.sd_out, c.g, zs));
-                tb::st(so, b0, a * c0 + b * rn0);
-                tb::st(so, b1, a * c1 + b * rn1);
+                tb::st(A.sd_out, oz + b0, a * c0 + b * rn0);
+                tb::st(A.sd_out, oz + b1, a * c1 + b * rn1);
             }
             if (LAST) {
                 st.acc += o0 ? rn0 * rn0 : 0.0;
@@ -333,8 +317,8 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
         st.pf[0][k][0] = st.pf[0][k][1] = st.pf[1][k][0] = st.pf[1][k][1] = 0.0;
     }
     st.acc = 0.0;
-    tb_load<FIRST>(A, c, c.ylo, st.pf[0]);
-    tb_load<FIRST>(A, c, c.ylo + 1, st.pf[1]);
+    tb_load<FIRST, YE>(A, c, c.ylo, st.pf[0]);
+    tb_load<FIRST, YE>(A, c, c.ylo + 1, st.pf[1]);
     // rounds run in blocks of six; rounds past the last output row only
     // touch windows, never memory outside the mesh
     const int yend = c.yhi + L;
@@ -404,7 +388,10 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.coef = coef;
     // waves whose columns or rows reach a face of the mesh need the clamps
     const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
-    const bool yedge = c.ylo == HALO || c.yhi + L >= c.ny + 1;
+    // waves near a y face clamp their rows; the others read rows up to
+    // yhi + L + 7 (prefetch and the six-round unroll) and store rows from
+    // Y0 - 2L unclamped
+    const bool yedge = c.ylo == HALO || c.Y0 < 2 * L + HALO || c.yhi + L + 8 >= c.ny + 1;
     if (xedge) {
         if (yedge) tb_wave<L, FIRST, LAST, true, true>(A, c);
         else tb_wave<L, FIRST, LAST, true, false>(A, c);
