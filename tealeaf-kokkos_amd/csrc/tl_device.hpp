@@ -19,6 +19,10 @@ constexpr int HALO = 2;
 // begins XOFF doubles before the first halo column.
 constexpr int XOFF = 30;
 constexpr int ROW_ALIGN = 32;  // doubles (256 B)
+// Zeroed rows before the first and after the last row of every field: the
+// temporal-blocked kernels read and store rows up to 2L + 8 past the mesh
+// (L <= 12 inner steps) without clamping the row index.
+constexpr int SLACK_ROWS = 32;
 
 constexpr int WAVE = 64;
 constexpr int WAVES_PER_BLOCK = 4;

@@ -1000,8 +1000,8 @@ static Geom make_geom(int nx, int ny, int nz)
 #endif
     g.sx = ((XOFF + g.X + ROW_ALIGN - 1) / ROW_ALIGN) * ROW_ALIGN;
     g.sy = g.sx * g.Y;
-    g.base = XOFF;
-    g.alloc = g.base + g.sy * g.Z + ROW_ALIGN;
+    g.base = XOFF + (long)SLACK_ROWS * g.sx;
+    g.alloc = g.base + g.sy * g.Z + (long)SLACK_ROWS * g.sx + ROW_ALIGN;
     g.tiles_x = (nx + WAVE - 1) / WAVE;
 #if TL_DIM == 3
     g.tiles_y = (ny + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;

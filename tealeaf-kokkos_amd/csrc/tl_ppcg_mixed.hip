@@ -39,6 +39,11 @@ namespace tl {
 namespace {
 
 constexpr int MX_MAX_LEVELS = 12;
+constexpr int VMCNT0 = 0x0F70;  // s_waitcnt immediate: vmcnt(0), expcnt and lgkmcnt unwaited
+#ifndef TL_MX_PF
+#define TL_MX_PF 3
+#endif
+constexpr int MX_PF = TL_MX_PF;  // rounds between a row's loads and their use (divides 6)
 
 // Inner-step counts with a compiled kernel (each costs build time); other
 // counts run the FP64 kernels.
@@ -77,27 +82,28 @@ __device__ __forceinline__ float f_from_east(float v)
 
 template <int L>
 struct MXState {
-    float sdw[L][3][2];   // sd of level s, slot (round) % 3, column
+    float sdw[L][2][2];   // sd of level s, slot (round) % 2, column
+    float dnw[L + 1][2];  // level s: sd(z) - sd(z - 1) of level s - 1 at its last row z
     float rw[L][2][2];    // r32 of level s, slot (round) % 2
     float duw[L][2][2];   // du after level s (s >= 1), slot (round) % 2
-    double pin[2][2][2];  // prefetched level-0 rows: r, w
-    float pk[2][2][2];    // prefetched level-0 rows: kx, ky
-    double po[2][2][2];   // prefetched output rows: u, p
+    double pin[MX_PF][2][2];  // prefetched level-0 rows: r, w
+    float pk[MX_PF][2][2];    // prefetched level-0 rows: kx, ky
+    double po[MX_PF][2][2];   // prefetched output rows: u, p
     double acc;
+    float ca[L], cb[L];   // step coefficients (wave-uniform)
 };
 
 struct MXCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
-    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;  // byte offsets of doubles in a row
-    unsigned hc0, hc1, hf0, hf1;                  // byte offsets of floats in a row
-    bool wcl0, wcl1, ecl0, ecl1;
+    unsigned bc0, bc1, bs0, bs1, bpad;  // byte offsets of doubles in a row (stores)
+    unsigned bp, hp;                    // the lane's column pair, clamped into the mesh: doubles, floats
+    unsigned bop;                       // u, p: lanes without output read an output pair
+    bool kw0, kw1;  // kx at c0, c1 is not a boundary face
     bool out0, out1;
     double alpha, inv_theta;
-    float4* kxr;   // LDS ring [L+1][64]: kx at c0, c1, c1 + 1
-    float2* kyr;   // LDS ring [L+1][64]: ky at c0, c1
-    double2* rpr;  // LDS ring [L+1][64]: r' at c0, c1
-    const float* coef;
+    float4* kr;    // LDS ring [16][64]: kx at c0, c1, ky at c0, c1
+    double2* rpr;  // LDS ring [16][64]: r' at c0, c1
 };
 
 template <class T>
@@ -111,166 +117,146 @@ __device__ __forceinline__ void stb(double* row, unsigned boff, double v)
     *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
 }
 
-__device__ __forceinline__ int ring(int row, int n)
-{
-    const int m = row % n;
-    return m < 0 ? m + n : m;
-}
+// Ring slot of a row: 16 rows, more than the L + 1 a wave keeps.
+constexpr int NRING = 16;
+__device__ __forceinline__ int ring(int row) { return row & (NRING - 1); }
 
-// Prefetch for round y + 2: level-0 inputs of row y + 2 and the output row
-// y + 2 - L.  Every load is issued on every round, so the count of loads in
-// flight never depends on a branch.  Waves that reach a y face (YE) clamp
-// the rows; the others read rows y + 2 and y + 2 - L as they are (the wave
-// placement keeps them inside the mesh), one row offset for every array.
-template <int L, bool YE>
+// Prefetch for round y: level-0 inputs of row y and the output row y - L,
+// issued MX_PF rounds ahead.  Every load is issued on every round, so the count of loads in
+// flight never depends on a branch.  Rows are not clamped: a strip reaches
+// at most 2L rows below and L + 5 + MX_PF rows above the mesh, which fall in the
+// halo and the zeroed slack rows of the allocation.  Each load is the
+// array's base plus a per-lane 32-bit byte offset (the host keeps every
+// field under 4 GiB).
+template <int L>
 __device__ __forceinline__ void mx_load(const MXArgs& A, const MXCtx& c, int y, int slot, MXState<L>& st)
 {
-    // byte offsets of the rows (the host keeps every field under 4 GiB);
-    // each load is the array's base plus a per-lane 32-bit offset
-    const unsigned oy = (unsigned)(c.g.base + (long)y * c.g.sx);
-    const unsigned oyr = YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 1) * c.g.sx) : oy;
-    const unsigned oyk = YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 2) * c.g.sx) : oy;  // ky also on the top face
-    const unsigned oyo = YE ? (unsigned)(c.g.base + (long)min(max(y - L, c.Y0), c.yhi) * c.g.sx)
-                            : oy - (unsigned)(L * c.g.sx);
-    st.pin[slot][0][0] = ldb(A.r_in, 8u * oyr + c.bc0);
-    st.pin[slot][0][1] = ldb(A.r_in, 8u * oyr + c.bc1);
-    st.pin[slot][1][0] = ldb(A.w, 8u * oyr + c.bc0);
-    st.pin[slot][1][1] = ldb(A.w, 8u * oyr + c.bc1);
-    st.pk[slot][0][0] = ldb(A.kx, 4u * oyr + c.hf0);
-    st.pk[slot][0][1] = ldb(A.kx, 4u * oyr + c.hf1);
-    st.pk[slot][1][0] = ldb(A.ky, 4u * oyk + c.hc0);
-    st.pk[slot][1][1] = ldb(A.ky, 4u * oyk + c.hc1);
-    st.po[slot][0][0] = ldb(A.u, 8u * oyo + c.bc0);
-    st.po[slot][0][1] = ldb(A.u, 8u * oyo + c.bc1);
-    st.po[slot][1][0] = ldb(A.p, 8u * oyo + c.bc0);
-    st.po[slot][1][1] = ldb(A.p, 8u * oyo + c.bc1);
+    const unsigned oyr = (unsigned)(c.g.base + (long)y * c.g.sx);
+    const unsigned oyo = oyr - (unsigned)(L * c.g.sx);
+    // a lane's two columns are one 16-byte (8-byte for floats) load
+    const double2 r = ldb(reinterpret_cast<const double2*>(A.r_in), 8u * oyr + c.bp);
+    const double2 w = ldb(reinterpret_cast<const double2*>(A.w), 8u * oyr + c.bp);
+    const float2 kx = ldb(reinterpret_cast<const float2*>(A.kx), 4u * oyr + c.hp);
+    const float2 ky = ldb(reinterpret_cast<const float2*>(A.ky), 4u * oyr + c.hp);
+    const double2 u = ldb(reinterpret_cast<const double2*>(A.u), 8u * oyo + c.bop);
+    const double2 p = ldb(reinterpret_cast<const double2*>(A.p), 8u * oyo + c.bop);
+    st.pin[slot][0][0] = r.x;
+    st.pin[slot][0][1] = r.y;
+    st.pin[slot][1][0] = w.x;
+    st.pin[slot][1][1] = w.y;
+    st.pk[slot][0][0] = kx.x;
+    st.pk[slot][0][1] = kx.y;
+    st.pk[slot][1][0] = ky.x;
+    st.pk[slot][1][1] = ky.y;
+    st.po[slot][0][0] = u.x;
+    st.po[slot][0][1] = u.y;
+    st.po[slot][1][0] = p.x;
+    st.po[slot][1][1] = p.y;
 }
 
 // One round: level 0 at row y, level s at row y - s (s = 1..L); level L's
 // row is the output row.
-template <int L, bool XE, bool YE, int PH>
+//
+// The reflective faces are folded into the coefficients: a mirrored
+// neighbour equals the cell, so its term k_f (sd_f - sd) is zero, and the
+// rings hold k_f = 0 for the faces on the mesh boundary (west of column 2,
+// east of column nx + 1, south of row 2, north of row ny + 1).  Every level
+// then runs the same arithmetic on every row and column, with the same
+// result as substituting the mirrored value; rows and columns outside the
+// mesh compute finite values (halo and slack rows hold finite values,
+// columns are clamped, the rings start zeroed) that reach no cell of the
+// mesh.
+template <int L, int PH>
 __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXState<L>& st, int y)
 {
-    constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
-    constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
-    constexpr int P = PH % 2;                                          // prefetch slot of this round
+    constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;  // rounds y, y-1
+    constexpr int P = PH % MX_PF;                  // prefetch slot of this round
     const int lane = threadIdx.x;
-    const int ny = c.ny;
 
-    int rn = ring(y - 1, L + 1);
-    float4 nkx = c.kxr[rn * WAVE + lane];
-    float2 nky = c.kyr[rn * WAVE + lane];
+    float4 nk = c.kr[ring(y - 1) * WAVE + lane];
 
     // level 0 at row y: r' = r - alpha w in FP64, then its FP32 rounding
-    const int ry = ring(y, L + 1);
+    const int ry = ring(y);
     float kyn0, kyn1;
-    if (!YE || y <= ny + 1) {
+    {
         double rp[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             rp[q] = st.pin[P][0][q] - c.alpha * st.pin[P][1][q];
             st.rw[0][R0][q] = (float)rp[q];
-            st.sdw[0][S0][q] = (float)(rp[q] * c.inv_theta);
+            st.sdw[0][R0][q] = (float)(rp[q] * c.inv_theta);
         }
         c.rpr[ry * WAVE + lane] = make_double2(rp[0], rp[1]);
-        const float kx0 = st.pk[P][0][0], kx1 = st.pk[P][0][1];
-        c.kxr[ry * WAVE + lane] = make_float4(kx0, kx1, f_from_east(kx0), 0.0f);
-        c.kyr[ry * WAVE + lane] = make_float2(st.pk[P][1][0], st.pk[P][1][1]);
         kyn0 = st.pk[P][1][0];
         kyn1 = st.pk[P][1][1];
-    } else {
-        // beyond the top face: the reflective mirror repeats the last row
-        const bool face = y == ny + 2;
-        kyn0 = face ? st.pk[P][1][0] : 0.0f;
-        kyn1 = face ? st.pk[P][1][1] : 0.0f;
-        c.kyr[ry * WAVE + lane] = make_float2(kyn0, kyn1);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            st.sdw[0][S0][q] = st.sdw[0][S1][q];
-            st.rw[0][R0][q] = st.rw[0][R1][q];
-        }
+        if (y == HALO || y > c.ny + 1) kyn0 = kyn1 = 0.0f;
+        c.kr[ry * WAVE + lane] = make_float4(c.kw0 ? st.pk[P][0][0] : 0.0f, c.kw1 ? st.pk[P][0][1] : 0.0f, kyn0, kyn1);
     }
-    // the output row of this round: u, p loaded two rounds ago, r' from
-    // the ring (row y - L shares no slot with row y)
+    // the output row of this round: u, p loaded MX_PF rounds ago, r' from
+    // the ring
     double po[2][2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         po[k][0] = st.po[P][k][0];
         po[k][1] = st.po[P][k][1];
     }
-    const double2 rpo = c.rpr[ring(y - L, L + 1) * WAVE + lane];
-    mx_load<L, YE>(A, c, y + 2, P, st);
+    const double2 rpo = c.rpr[ring(y - L) * WAVE + lane];
+    mx_load<L>(A, c, y + MX_PF, P, st);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
         const int z = y - s;
-        const float4 kx = nkx;
-        const float2 ky = nky;
-        const float a = c.coef[s - 1], b = c.coef[L + s - 1];
+        const float4 k = nk;  // kx at c0, c1 and ky at c0, c1 of row z
+        const float a = st.ca[s - 1], b = st.cb[s - 1];
+        if (s < L) nk = c.kr[ring(z - 1) * WAVE + lane];
+        // level s - 1 at rows z (c) and z + 1 (n); its row z - 1 enters
+        // through the difference carried from the previous round.  The
+        // differences are the ones the direct form takes (sd_f - sd), so
+        // the sums below round exactly as they would:
+        //   e0 - c0 = c1 - c0,     w1 - c1 = -(c1 - c0),
+        //   e1 - c1 = c0(+1) - c1, w0 - c0 = -(e1 - c1 of the lane to the west),
+        //   s - c = -(c - s), the north difference of row z - 1.
+        const float c0 = st.sdw[s - 1][R1][0], c1 = st.sdw[s - 1][R1][1];
+        const float n0 = st.sdw[s - 1][R0][0], n1 = st.sdw[s - 1][R0][1];
+        const float d01 = c1 - c0;
+        const float de1 = f_from_east(c0) - c1;
+        const float dw0 = f_from_west(de1);
+        const float dn0 = n0 - c0, dn1 = n1 - c1;
+        const float kx2 = f_from_east(k.x);  // kx at c1 + 1
+        // r - A sd = (r - sd) + sum_f k_f (sd_f - sd), the north face
+        // last: only that term needs the row the level below produced in
+        // this round, so the rest of every level is off the chain that
+        // runs through the L levels of a round
+        float t0 = st.rw[s - 1][R1][0] - c0;
+        t0 = __builtin_fmaf(k.y, d01, t0);
+        t0 = __builtin_fmaf(k.x, -dw0, t0);
+        t0 = __builtin_fmaf(k.z, -st.dnw[s][0], t0);
+        t0 = __builtin_fmaf(kyn0, dn0, t0);
+        float t1 = st.rw[s - 1][R1][1] - c1;
+        t1 = __builtin_fmaf(kx2, de1, t1);
+        t1 = __builtin_fmaf(k.y, -d01, t1);
+        t1 = __builtin_fmaf(k.w, -st.dnw[s][1], t1);
+        t1 = __builtin_fmaf(kyn1, dn1, t1);
+        st.dnw[s][0] = dn0;
+        st.dnw[s][1] = dn1;
+        kyn0 = k.z;
+        kyn1 = k.w;
+        const float d0 = s == 1 ? c0 : st.duw[s - 1][R1][0] + c0;
+        const float d1 = s == 1 ? c1 : st.duw[s - 1][R1][1] + c1;
         if (s < L) {
-            rn = ring(z - 1, L + 1);
-            nkx = c.kxr[rn * WAVE + lane];
-            nky = c.kyr[rn * WAVE + lane];
-        }
-        float t0 = 0.0f, t1 = 0.0f, d0 = 0.0f, d1 = 0.0f;
-        if (!YE || z <= ny + 1) {
-            const float c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
-            float s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
-            float n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
-            if (YE) {
-                if (z == HALO) { s0 = c0; s1 = c1; }
-                if (z == ny + 1) { n0 = c0; n1 = c1; }
-            }
-            float w0 = f_from_west(c1), e1 = f_from_east(c0);
-            float e0 = c1, w1 = c0;
-            if (XE) {
-                if (c.wcl0) w0 = c0;
-                if (c.wcl1) w1 = c1;
-                if (c.ecl0) e0 = c0;
-                if (c.ecl1) e1 = c1;
-            }
-            // r - A sd = (r - sd) + sum_f k_f (sd_f - sd)
-            t0 = st.rw[s - 1][R1][0] - c0;
-            t0 = __builtin_fmaf(kx.y, e0 - c0, t0);
-            t0 = __builtin_fmaf(kx.x, w0 - c0, t0);
-            t0 = __builtin_fmaf(kyn0, n0 - c0, t0);
-            t0 = __builtin_fmaf(ky.x, s0 - c0, t0);
-            t1 = st.rw[s - 1][R1][1] - c1;
-            t1 = __builtin_fmaf(kx.z, e1 - c1, t1);
-            t1 = __builtin_fmaf(kx.y, w1 - c1, t1);
-            t1 = __builtin_fmaf(kyn1, n1 - c1, t1);
-            t1 = __builtin_fmaf(ky.y, s1 - c1, t1);
-            kyn0 = ky.x;
-            kyn1 = ky.y;
-            d0 = s == 1 ? c0 : st.duw[s - 1][R1][0] + c0;
-            d1 = s == 1 ? c1 : st.duw[s - 1][R1][1] + c1;
-            if (s < L) {
-                st.sdw[s][S0][0] = __builtin_fmaf(a, c0, b * t0);
-                st.sdw[s][S0][1] = __builtin_fmaf(a, c1, b * t1);
-                st.rw[s][R0][0] = t0;
-                st.rw[s][R0][1] = t1;
-                st.duw[s][R0][0] = d0;
-                st.duw[s][R0][1] = d1;
-            }
+            st.sdw[s][R0][0] = __builtin_fmaf(a, c0, b * t0);
+            st.sdw[s][R0][1] = __builtin_fmaf(a, c1, b * t1);
+            st.rw[s][R0][0] = t0;
+            st.rw[s][R0][1] = t1;
+            st.duw[s][R0][0] = d0;
+            st.duw[s][R0][1] = d1;
         } else {
-            kyn0 = ky.x;
-            kyn1 = ky.y;
-            if (s < L) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    st.sdw[s][S0][q] = st.sdw[s][S1][q];
-                    st.rw[s][R0][q] = st.rw[s][R1][q];
-                    st.duw[s][R0][q] = st.duw[s][R1][q];
-                }
-            }
-        }
-        if (s == L) {
             // u_out = u' + du, r_out = r' + (r32_L - r32_0), in FP64; every
             // lane stores, lanes and rows that own no cell into a pad column
-            const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
+            const bool zin = z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
             const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
-            const unsigned oz = 8u * (unsigned)(c.g.base + (long)(YE ? min(max(z, 0), ny + 2 * HALO - 1) : z) * c.g.sx);
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
             const double rp0 = rpo.x, rp1 = rpo.y;
             const double up0 = po[0][0] + c.alpha * po[1][0], up1 = po[0][1] + c.alpha * po[1][1];
             const double rr0 = rp0 + (double)(t0 - (float)rp0);
@@ -285,7 +271,7 @@ __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXStat
     }
 }
 
-template <int L, bool XE, bool YE>
+template <int L>
 __device__ __forceinline__ void mx_wave(const MXArgs& A, const MXCtx& c)
 {
     MXState<L> st;
@@ -293,22 +279,41 @@ __device__ __forceinline__ void mx_wave(const MXArgs& A, const MXCtx& c)
     for (int s = 0; s < L; ++s) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            st.sdw[s][0][q] = st.sdw[s][1][q] = st.sdw[s][2][q] = 0.0f;
+            st.sdw[s][0][q] = st.sdw[s][1][q] = 0.0f;
+            st.dnw[s + 1][q] = 0.0f;
             st.rw[s][0][q] = st.rw[s][1][q] = 0.0f;
             st.duw[s][0][q] = st.duw[s][1][q] = 0.0f;
         }
     }
     st.acc = 0.0;
-    mx_load<L, YE>(A, c, c.ylo, 0, st);
-    mx_load<L, YE>(A, c, c.ylo + 1, 1, st);
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        st.ca[s] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(A.coef[s])));
+        st.cb[s] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(A.coef[L + s])));
+    }
+#pragma unroll
+    for (int k = 0; k < MX_PF; ++k) mx_load<L>(A, c, c.ylo + k, k, st);
+    // wait for the first rows here: with loads still in flight on entry
+    // the compiler's wait at the loop head has to cover the entry path too,
+    // and it then drains all loads at the head of every block of rounds
+    __builtin_amdgcn_s_waitcnt(VMCNT0);
     const int yend = c.yhi + L;
+    // the scheduler may not move work between rounds: interleaving them
+    // raises the register count past 256, and the copies it then makes of
+    // registers with loads in flight wait for those loads
     for (int y = c.ylo; y <= yend; y += 6) {
-        mx_round<L, XE, YE, 0>(A, c, st, y);
-        mx_round<L, XE, YE, 1>(A, c, st, y + 1);
-        mx_round<L, XE, YE, 2>(A, c, st, y + 2);
-        mx_round<L, XE, YE, 3>(A, c, st, y + 3);
-        mx_round<L, XE, YE, 4>(A, c, st, y + 4);
-        mx_round<L, XE, YE, 5>(A, c, st, y + 5);
+        mx_round<L, 0>(A, c, st, y);
+        __builtin_amdgcn_sched_barrier(0);
+        mx_round<L, 1>(A, c, st, y + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mx_round<L, 2>(A, c, st, y + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mx_round<L, 3>(A, c, st, y + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        mx_round<L, 4>(A, c, st, y + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        mx_round<L, 5>(A, c, st, y + 5);
+        __builtin_amdgcn_sched_barrier(0);
     }
     double v[1] = {st.acc};
     block_partial<1, 1>(v, Reduce{A.partials});
@@ -318,39 +323,45 @@ template <int L>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVES_PER_SIMD))) void k_ppcg_mixed(
     Geom g, MXArgs A)
 {
-    __shared__ float4 kx_ring[(L + 1) * WAVE];
-    __shared__ float2 ky_ring[(L + 1) * WAVE];
-    __shared__ double2 rp_ring[(L + 1) * WAVE];
-    __shared__ float coef[2 * L];
+    static_assert(L + 1 <= NRING, "ring too short");
+    __shared__ float4 k_ring[NRING * WAVE];
+    __shared__ double2 rp_ring[NRING * WAVE];
     const int lane = threadIdx.x;
-    if (lane < 2 * L) coef[lane] = A.coef[lane];
+    // rows below the strip read ring slots before any row is written there
+    for (int k = 0; k < NRING; ++k) k_ring[k * WAVE + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     __syncthreads();
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
-    constexpr int OW = 2 * WAVE - 2 * L;
+    // the wave's first column is even, so a lane's column pair is 16-byte
+    // aligned: LP = L rounded up to even columns of margin on each side
+    constexpr int LP = L + (L & 1);
+    constexpr int OW = 2 * WAVE - 2 * LP;
     MXCtx c;
     c.g = g;
     c.nx = g.nx;
     c.ny = g.ny;
     const int X0 = HALO + tx * OW;
     c.Y0 = HALO + ty * A.rows;
-    const int k0 = X0 - L + 2 * lane, k1 = k0 + 1;
-    const int kc0 = min(max(k0, HALO), c.nx + 1), kc1 = min(max(k1, HALO), c.nx + 1);
-    const int kf0 = min(max(k0, HALO), c.nx + 2), kf1 = min(max(k1, HALO), c.nx + 2);
-    c.bc0 = 8u * (unsigned)kc0;
-    c.bc1 = 8u * (unsigned)kc1;
-    c.bf0 = 8u * (unsigned)kf0;
-    c.bf1 = 8u * (unsigned)kf1;
-    c.hc0 = 4u * (unsigned)kc0;
-    c.hc1 = 4u * (unsigned)kc1;
-    c.hf0 = 4u * (unsigned)kf0;
-    c.hf1 = 4u * (unsigned)kf1;
-    c.wcl0 = k0 == HALO;
-    c.wcl1 = k1 == HALO;
-    c.ecl0 = k0 == c.nx + 1;
-    c.ecl1 = k1 == c.nx + 1;
+    const int k0 = X0 - LP + 2 * lane, k1 = k0 + 1;
+    c.bc0 = 8u * (unsigned)k0;
+    c.bc1 = 8u * (unsigned)k1;
+    {
+        // pairs start on even columns: the last one may hold the halo
+        // column nx + 2, which no cell of the mesh reads (its face is zero)
+        const int kp = min(max(k0, HALO), (c.nx + 1) & ~1);
+        c.bp = 8u * (unsigned)kp;
+        c.hp = 4u * (unsigned)kp;
+    }
+    // faces HALO (west of column 2) and nx + 2 (east of column nx + 1)
+    c.kw0 = k0 != HALO && k0 != c.nx + 2;
+    c.kw1 = k1 != HALO && k1 != c.nx + 2;
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
+    {
+        // u and p of lanes without output come from the same cache lines
+        const int xlast = min(X0 + OW - 1, c.nx + 1);
+        c.bop = 8u * (unsigned)min(max(k0, X0), xlast & ~1);
+    }
     {
         // pad columns X .. sx-1 of every row are never read
         const int npad = (int)(g.sx - g.X);
@@ -358,25 +369,13 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVE
         c.bs0 = c.out0 ? c.bc0 : c.bpad;
         c.bs1 = c.out1 ? c.bc1 : c.bpad;
     }
-    c.ylo = max(c.Y0 - L, HALO);
+    c.ylo = c.Y0 - L;
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
     c.alpha = A.ds->alpha;
     c.inv_theta = A.inv_theta;
-    c.kxr = kx_ring;
-    c.kyr = ky_ring;
+    c.kr = k_ring;
     c.rpr = rp_ring;
-    c.coef = coef;
-    const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
-    // waves near a y face clamp their rows; the others read rows ylo + 2 - L
-    // .. yhi + L + 7 (prefetch and the six-round unroll) unclamped
-    const bool yedge = c.ylo == HALO || c.Y0 < 2 * L + HALO || c.yhi + L + 8 >= c.ny + 1;
-    if (xedge) {
-        if (yedge) mx_wave<L, true, true>(A, c);
-        else mx_wave<L, true, false>(A, c);
-    } else {
-        if (yedge) mx_wave<L, false, true>(A, c);
-        else mx_wave<L, false, false>(A, c);
-    }
+    mx_wave<L>(A, c);
 }
 
 template <int L>
@@ -444,7 +443,7 @@ long op_ppcg_outer_mixed(Chunk& c, int inner_steps)
     a.coef = c.d_coef32;
     a.partials = c.partials;
     const int L = inner_steps;
-    const int ow = 2 * WAVE - 2 * L;
+    const int ow = 2 * WAVE - 2 * (L + (L & 1));
     a.tiles_x = (c.g.nx + ow - 1) / ow;
     a.rows = tb_rows(c.g.ny, a.tiles_x, L, MX_WAVES_PER_SIMD * c.simds);
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);

@@ -31,6 +31,9 @@ namespace tl {
 #if TL_DIM == 2
 namespace tb {
 
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left unwaited
+constexpr int VMCNT0 = 0x0F70;
+
 struct TBArgs {
     double* u;
     const double* p;
@@ -319,16 +322,26 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
     st.acc = 0.0;
     tb_load<FIRST, YE>(A, c, c.ylo, st.pf[0]);
     tb_load<FIRST, YE>(A, c, c.ylo + 1, st.pf[1]);
+    // wait for the first two rows here: with loads still in flight on entry
+    // the compiler's wait at the loop head has to cover the entry path too,
+    // and it then drains all loads at the head of every block of rounds
+    __builtin_amdgcn_s_waitcnt(VMCNT0);
     // rounds run in blocks of six; rounds past the last output row only
     // touch windows, never memory outside the mesh
     const int yend = c.yhi + L;
     for (int y = c.ylo; y <= yend; y += 6) {
         tb_round<L, FIRST, LAST, XE, YE, 0>(A, c, st, y);
+        __builtin_amdgcn_sched_barrier(0);
         tb_round<L, FIRST, LAST, XE, YE, 1>(A, c, st, y + 1);
+        __builtin_amdgcn_sched_barrier(0);
         tb_round<L, FIRST, LAST, XE, YE, 2>(A, c, st, y + 2);
+        __builtin_amdgcn_sched_barrier(0);
         tb_round<L, FIRST, LAST, XE, YE, 3>(A, c, st, y + 3);
+        __builtin_amdgcn_sched_barrier(0);
         tb_round<L, FIRST, LAST, XE, YE, 4>(A, c, st, y + 4);
+        __builtin_amdgcn_sched_barrier(0);
         tb_round<L, FIRST, LAST, XE, YE, 5>(A, c, st, y + 5);
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (LAST) {
         double v[1] = {st.acc};
