@@ -54,9 +54,7 @@ struct TBArgs {
 };
 
 // Lane i receives lane i-1 (x-neighbour at kk-1); lane 0 receives 0.
-// Lane 0 and lane 63 either sit on a face of the mesh, where the clamps
-// replace the shifted value, or own no output cell, so the fill never
-// reaches a result.
+// Lanes 0 and 63 own no output cell, so the fill never reaches a result.
 __device__ __forceinline__ double from_west(double v)
 {
     const int lo = __double2loint(v), hi = __double2hiint(v);
@@ -89,12 +87,12 @@ struct TBState {
 struct TBCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
-    int kc0, kc1, kf0, kf1;  // clamped columns (cells; kx faces up to nx + 2)
-    // byte offsets within a row: cells, kx faces, store targets (the cell
-    // for lanes that own one, else a pad column past the halo that nothing
-    // reads) and the pad column alone (rows that own no output)
+    bool small;  // ny too small for one reflection to reach every row a strip reads (SMALL)
+    // byte offsets within a row: the lane's cells and kx faces reflected
+    // into the mesh, store targets (the cell for lanes that own one, else a
+    // pad column past the halo that nothing reads) and the pad column alone
+    // (rows that own no output)
     unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
-    bool wcl0, wcl1, ecl0, ecl1;
     bool out0, out1, out_any;
     double alpha, theta;
     double* ur;  // LDS rings [L+1][64][2]
@@ -113,36 +111,70 @@ __device__ __forceinline__ void st(double* row, unsigned boff, double v)
     *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
 }
 
+// The reflective boundary as data: a cell index outside [HALO, n + 1]
+// reads the cell it mirrors, a face index outside [HALO, n + 2] the face it
+// mirrors (reflections at both faces, repeated for meshes narrower than the
+// reach).  A wave computing cells past a face then holds there the mirror
+// image of the cells inside, level by level: the mirrored cell's stencil
+// has the same operands with its east and west (north and south) terms
+// exchanged, and the two-term sums it forms are commutative in IEEE
+// arithmetic, so the image is bitwise equal.  The cell on the face thus
+// reads its own value as its outside neighbour, as after the reference's
+// reflective halo update, with no per-lane or per-row selection.
+__host__ __device__ inline int fold_cell(int i, int n)
+{
+    const int P = 2 * n;
+    int t = (i - HALO) % P;
+    if (t < 0) t += P;
+    if (t >= n) t = P - 1 - t;
+    return HALO + t;
+}
+
+__host__ __device__ inline int fold_face(int f, int n)
+{
+    const int P = 2 * n;
+    int t = (f - HALO) % P;
+    if (t < 0) t += P;
+    if (t > n) t = P - t;
+    return HALO + t;
+}
+
 // Loads of row y into a prefetch slot.  Every load is issued on every
 // round, so that the number of memory operations in flight is the same on
 // every path and the compiler can wait for exactly the oldest slot instead
-// of draining all.  Waves that reach a y face (YE) clamp the rows into the
-// mesh (rows past the top repeat the last one; u and p are only used on the
-// wave's output rows); the others read row y as it is (their placement keeps
-// it inside the mesh).  An address is the array's base plus a per-lane
-// 32-bit byte offset of row and column (every field is under 4 GiB).
-template <bool FIRST, bool YE>
+// of draining all.  Rows outside the mesh read the rows they mirror (cells
+// for r, w|sd, u, p, kx; faces for ky).  An address is the array's base
+// plus a per-lane 32-bit byte offset of row and column (every field is under
+// 4 GiB).
+template <bool FIRST, bool SMALL>
 __device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
 {
-    const unsigned oy = (unsigned)(c.g.base + (long)y * c.g.sx);
-    const unsigned oyr = 8u * (YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 1) * c.g.sx) : oy);
-    const unsigned oyk = 8u * (YE ? (unsigned)(c.g.base + (long)min(y, c.ny + 2) * c.g.sx) : oy);  // ky: top face
-    const unsigned oyo = 8u * (YE ? (unsigned)(c.g.base + (long)min(max(y, c.Y0), c.yhi) * c.g.sx) : oy);
-    v[0][0] = ld(A.r_in, oyr + c.bc0);
-    v[0][1] = ld(A.r_in, oyr + c.bc1);
-    const double* b = FIRST ? A.w : A.sd_in;
-    v[1][0] = ld(b, oyr + c.bc0);
-    v[1][1] = ld(b, oyr + c.bc1);
-    v[2][0] = ld(A.u, oyo + c.bc0);
-    v[2][1] = ld(A.u, oyo + c.bc1);
-    if (FIRST) {
-        v[3][0] = ld(A.p, oyo + c.bc0);
-        v[3][1] = ld(A.p, oyo + c.bc1);
+    int yc, yf;
+    if (SMALL) {
+        yc = fold_cell(y, c.ny);
+        yf = fold_face(y, c.ny);
+    } else {
+        // one reflection reaches every row a strip reads: HALO - L .. ny + L + 8
+        yc = y < HALO ? 2 * HALO - 1 - y : (y > c.ny + 1 ? 2 * c.ny + 3 - y : y);
+        yf = y < HALO ? 2 * HALO - y : (y > c.ny + 2 ? 2 * c.ny + 4 - y : y);
     }
-    v[4][0] = ld(A.kx, oyr + c.bf0);
-    v[4][1] = ld(A.kx, oyr + c.bf1);
-    v[5][0] = ld(A.ky, oyk + c.bc0);
-    v[5][1] = ld(A.ky, oyk + c.bc1);
+    const unsigned oy = 8u * (unsigned)(c.g.base + (long)yc * c.g.sx);
+    const unsigned of = 8u * (unsigned)(c.g.base + (long)yf * c.g.sx);
+    v[0][0] = ld(A.r_in, oy + c.bc0);
+    v[0][1] = ld(A.r_in, oy + c.bc1);
+    const double* b = FIRST ? A.w : A.sd_in;
+    v[1][0] = ld(b, oy + c.bc0);
+    v[1][1] = ld(b, oy + c.bc1);
+    v[2][0] = ld(A.u, oy + c.bc0);
+    v[2][1] = ld(A.u, oy + c.bc1);
+    if (FIRST) {
+        v[3][0] = ld(A.p, oy + c.bc0);
+        v[3][1] = ld(A.p, oy + c.bc1);
+    }
+    v[4][0] = ld(A.kx, oy + c.bf0);
+    v[4][1] = ld(A.kx, oy + c.bf1);
+    v[5][0] = ld(A.ky, of + c.bc0);
+    v[5][1] = ld(A.ky, of + c.bc1);
 }
 
 __device__ __forceinline__ int ring(int row, int n)
@@ -151,18 +183,15 @@ __device__ __forceinline__ int ring(int row, int n)
     return m < 0 ? m + n : m;
 }
 
-// One round: level 0 at row y, then level s at row y - s for s = 1..L.
-// XE: the wave reaches an x face (per-lane clamps); YE: it reaches the
-// bottom or top face (row clamps and rows past the top).  The LDS rings of
+// One round: level 0 at row y, then level s at row y - s for s = 1..L.  The LDS rings of
 // level s + 1 are read before level s computes, so their latency overlaps
 // level s's arithmetic (the rows differ, so no write of level s is missed).
-template <int L, bool FIRST, bool LAST, bool XE, bool YE, int PH>
+template <int L, bool FIRST, bool LAST, bool SMALL, int PH>
 __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y)
 {
     constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
     constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
     const int lane = threadIdx.x;
-    const int ny = c.ny;
     double(&v)[6][2] = st.pf[PH % 2];
     double2* ur = reinterpret_cast<double2*>(c.ur);
     double2* kxr = reinterpret_cast<double2*>(c.kxr);
@@ -176,7 +205,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
     const int ry = ring(y, L + 1);
     double kyn0, kyn1;
-    if (!YE || y <= ny + 1) {
+    {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             double r0, sd0;
@@ -200,20 +229,9 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         kyr[ry * WAVE + lane] = make_double2(v[5][0], v[5][1]);
         kyn0 = v[5][0];
         kyn1 = v[5][1];
-    } else {
-        // beyond the top face: the reflective mirror repeats the last row
-        const bool face = y == ny + 2;
-        kyn0 = face ? v[5][0] : 0.0;
-        kyn1 = face ? v[5][1] : 0.0;
-        kyr[ry * WAVE + lane] = make_double2(kyn0, kyn1);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            st.sdw[0][S0][q] = st.sdw[0][S1][q];
-            st.rw[0][R0][q] = st.rw[0][R1][q];
-        }
     }
     // prefetch row y + 2 into the slot just consumed
-    tb_load<FIRST, YE>(A, c, y + 2, v);
+    tb_load<FIRST, SMALL>(A, c, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
@@ -230,25 +248,14 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
             na = c.coef[s];
             nb = c.coef[L + s];
         }
-        // level L's results (zero where the level is beyond the top face)
-        double rn0 = 0.0, rn1 = 0.0;
-        if (!YE || z <= ny + 1) {
+        double rn0, rn1;
+        {
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
-            double s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
-            double n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
-            if (YE) {
-                if (z == HALO) { s0 = c0; s1 = c1; }
-                if (z == ny + 1) { n0 = c0; n1 = c1; }
-            }
+            const double s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
+            const double n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
             // x-neighbours: c0's west is the previous lane's c1, c1's east the next lane's c0
-            double w0 = from_west(c1), e1 = from_east(c0);
-            double e0 = c1, w1 = c0;
-            if (XE) {
-                if (c.wcl0) w0 = c0;
-                if (c.wcl1) w1 = c1;
-                if (c.ecl0) e0 = c0;
-                if (c.ecl1) e1 = c1;
-            }
+            const double w0 = from_west(c1), e1 = from_east(c0);
+            const double e0 = c1, w1 = c0;
             const double kxe1 = from_east(kx.x);
             // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z)
             const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
@@ -268,24 +275,14 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                 st.rw[s][R0][1] = rn1;
                 ur[rz * WAVE + lane] = u;
             }
-        } else {
-            kyn0 = ky.x;
-            kyn1 = ky.y;
-            if (s < L) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    st.sdw[s][S0][q] = st.sdw[s][S1][q];
-                    st.rw[s][R0][q] = st.rw[s][R1][q];
-                }
-            }
         }
         if (s == L) {
             // output row: every lane stores, lanes and rows that own no
             // cell into a pad column, so the store count never depends on a branch
-            const bool zin = (!YE || z <= ny + 1) && z >= c.Y0 && z <= c.yhi;
+            const bool zin = z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
             const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
-            const unsigned oz = 8u * (unsigned)(c.g.base + (long)(YE ? min(max(z, 0), ny + 2 * HALO - 1) : z) * c.g.sx);
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
             tb::st(A.r_out, oz + b0, rn0);
             tb::st(A.u, oz + b0, u.x);
@@ -303,7 +300,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     }
 }
 
-template <int L, bool FIRST, bool LAST, bool XE, bool YE>
+template <int L, bool FIRST, bool LAST, bool SMALL>
 __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
 {
     TBState<L> st;
@@ -320,27 +317,28 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
         st.pf[0][k][0] = st.pf[0][k][1] = st.pf[1][k][0] = st.pf[1][k][1] = 0.0;
     }
     st.acc = 0.0;
-    tb_load<FIRST, YE>(A, c, c.ylo, st.pf[0]);
-    tb_load<FIRST, YE>(A, c, c.ylo + 1, st.pf[1]);
+    tb_load<FIRST, SMALL>(A, c, c.ylo, st.pf[0]);
+    tb_load<FIRST, SMALL>(A, c, c.ylo + 1, st.pf[1]);
     // wait for the first two rows here: with loads still in flight on entry
     // the compiler's wait at the loop head has to cover the entry path too,
     // and it then drains all loads at the head of every block of rounds
     __builtin_amdgcn_s_waitcnt(VMCNT0);
-    // rounds run in blocks of six; rounds past the last output row only
-    // touch windows, never memory outside the mesh
+    // rounds run in blocks of six; the scheduler may not move work between
+    // rounds (interleaving them raises the register count, and the copies
+    // it then makes of registers with loads in flight wait for those loads)
     const int yend = c.yhi + L;
     for (int y = c.ylo; y <= yend; y += 6) {
-        tb_round<L, FIRST, LAST, XE, YE, 0>(A, c, st, y);
+        tb_round<L, FIRST, LAST, SMALL, 0>(A, c, st, y);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, XE, YE, 1>(A, c, st, y + 1);
+        tb_round<L, FIRST, LAST, SMALL, 1>(A, c, st, y + 1);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, XE, YE, 2>(A, c, st, y + 2);
+        tb_round<L, FIRST, LAST, SMALL, 2>(A, c, st, y + 2);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, XE, YE, 3>(A, c, st, y + 3);
+        tb_round<L, FIRST, LAST, SMALL, 3>(A, c, st, y + 3);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, XE, YE, 4>(A, c, st, y + 4);
+        tb_round<L, FIRST, LAST, SMALL, 4>(A, c, st, y + 4);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, XE, YE, 5>(A, c, st, y + 5);
+        tb_round<L, FIRST, LAST, SMALL, 5>(A, c, st, y + 5);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (LAST) {
@@ -369,18 +367,11 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     const int X0 = HALO + tx * OW;
     c.Y0 = HALO + ty * A.rows;
     const int k0 = X0 - L + 2 * lane, k1 = k0 + 1;
-    c.kc0 = min(max(k0, HALO), c.nx + 1);
-    c.kc1 = min(max(k1, HALO), c.nx + 1);
-    c.kf0 = min(max(k0, HALO), c.nx + 2);
-    c.kf1 = min(max(k1, HALO), c.nx + 2);
-    c.bc0 = 8u * (unsigned)c.kc0;
-    c.bc1 = 8u * (unsigned)c.kc1;
-    c.bf0 = 8u * (unsigned)c.kf0;
-    c.bf1 = 8u * (unsigned)c.kf1;
-    c.wcl0 = k0 == HALO;
-    c.wcl1 = k1 == HALO;
-    c.ecl0 = k0 == c.nx + 1;
-    c.ecl1 = k1 == c.nx + 1;
+    c.bc0 = 8u * (unsigned)fold_cell(k0, c.nx);
+    c.bc1 = 8u * (unsigned)fold_cell(k1, c.nx);
+    c.bf0 = 8u * (unsigned)fold_face(k0, c.nx);
+    c.bf1 = 8u * (unsigned)fold_face(k1, c.nx);
+    c.small = c.ny < L + 10;
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
     c.out_any = c.out0 || c.out1;
@@ -391,7 +382,7 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
         c.bs0 = c.out0 ? c.bc0 : c.bpad;
         c.bs1 = c.out1 ? c.bc1 : c.bpad;
     }
-    c.ylo = max(c.Y0 - L, HALO);
+    c.ylo = c.Y0 - L;
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
     c.alpha = FIRST ? A.ds->alpha : 0.0;
     c.theta = A.theta;
@@ -399,19 +390,8 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
     c.coef = coef;
-    // waves whose columns or rows reach a face of the mesh need the clamps
-    const bool xedge = X0 - L <= HALO || X0 - L + 2 * WAVE - 1 >= c.nx + 1;
-    // waves near a y face clamp their rows; the others read rows up to
-    // yhi + L + 7 (prefetch and the six-round unroll) and store rows from
-    // Y0 - 2L unclamped
-    const bool yedge = c.ylo == HALO || c.Y0 < 2 * L + HALO || c.yhi + L + 8 >= c.ny + 1;
-    if (xedge) {
-        if (yedge) tb_wave<L, FIRST, LAST, true, true>(A, c);
-        else tb_wave<L, FIRST, LAST, true, false>(A, c);
-    } else {
-        if (yedge) tb_wave<L, FIRST, LAST, false, true>(A, c);
-        else tb_wave<L, FIRST, LAST, false, false>(A, c);
-    }
+    if (c.small) tb_wave<L, FIRST, LAST, true>(A, c);
+    else tb_wave<L, FIRST, LAST, false>(A, c);
 }
 
 template <int L, bool FIRST, bool LAST>
