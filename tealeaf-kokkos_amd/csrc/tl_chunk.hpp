@@ -108,8 +108,7 @@ struct Chunk {
 
     // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
     // elements) and of the inner-step coefficients, made at the switch.
-    float* kx32 = nullptr;
-    float* ky32 = nullptr;
+    float* kxy32 = nullptr;  // FP32 kx, ky interleaved: element i at [2i], [2i + 1]
     float* d_coef32 = nullptr;  // a[0..n) then b[0..n)
     int mixed_steps = 0;
     double inv_theta = 0.0;

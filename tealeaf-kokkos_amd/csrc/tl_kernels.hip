@@ -1079,8 +1079,7 @@ Chunk::~Chunk()
     (void)hipFree(sd_alt);
     (void)hipFree(p_alt);
     (void)hipFree(r_alt);
-    (void)hipFree(kx32);
-    (void)hipFree(ky32);
+    (void)hipFree(kxy32);
     (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);

@@ -59,8 +59,7 @@ struct MXArgs {
     const double* r_in;
     const double* w;
     double* r_out;
-    const float* kx;  // FP32 copies, same element index as the FP64 fields
-    const float* ky;
+    const float* kxy;  // FP32 kx, ky interleaved: element i of the FP64 fields at [2i], [2i + 1]
     const DevScalars* ds;
     double inv_theta;
     const float* coef;  // a[0..L) then b[0..L)
@@ -96,9 +95,10 @@ struct MXState {
 struct MXCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
-    unsigned bc0, bc1, bs0, bs1, bpad;  // byte offsets of doubles in a row (stores)
-    unsigned bp, hp;                    // the lane's column pair, clamped into the mesh: doubles, floats
-    unsigned bop;                       // u, p: lanes without output read an output pair
+    unsigned bs;    // store target of the lane's column pair (a pad pair for lanes without output)
+    unsigned bpad;  // a pad pair past the halo that nothing reads (rows without output)
+    unsigned bp;    // the lane's column pair, clamped into the mesh
+    unsigned bop;   // u, p: lanes without output read an output pair
     bool kw0, kw1;  // kx at c0, c1 is not a boundary face
     bool out0, out1;
     double alpha, inv_theta;
@@ -112,9 +112,9 @@ __device__ __forceinline__ T ldb(const T* row, unsigned boff)
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(row) + boff);
 }
 
-__device__ __forceinline__ void stb(double* row, unsigned boff, double v)
+__device__ __forceinline__ void stb(double* row, unsigned boff, double2 v)
 {
-    *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
+    *reinterpret_cast<double2*>(reinterpret_cast<char*>(row) + boff) = v;
 }
 
 // Ring slot of a row: 16 rows, more than the L + 1 a wave keeps.
@@ -136,18 +136,17 @@ __device__ __forceinline__ void mx_load(const MXArgs& A, const MXCtx& c, int y, 
     // a lane's two columns are one 16-byte (8-byte for floats) load
     const double2 r = ldb(reinterpret_cast<const double2*>(A.r_in), 8u * oyr + c.bp);
     const double2 w = ldb(reinterpret_cast<const double2*>(A.w), 8u * oyr + c.bp);
-    const float2 kx = ldb(reinterpret_cast<const float2*>(A.kx), 4u * oyr + c.hp);
-    const float2 ky = ldb(reinterpret_cast<const float2*>(A.ky), 4u * oyr + c.hp);
+    const float4 k = ldb(reinterpret_cast<const float4*>(A.kxy), 8u * oyr + c.bp);  // kx0 ky0 kx1 ky1
     const double2 u = ldb(reinterpret_cast<const double2*>(A.u), 8u * oyo + c.bop);
     const double2 p = ldb(reinterpret_cast<const double2*>(A.p), 8u * oyo + c.bop);
     st.pin[slot][0][0] = r.x;
     st.pin[slot][0][1] = r.y;
     st.pin[slot][1][0] = w.x;
     st.pin[slot][1][1] = w.y;
-    st.pk[slot][0][0] = kx.x;
-    st.pk[slot][0][1] = kx.y;
-    st.pk[slot][1][0] = ky.x;
-    st.pk[slot][1][1] = ky.y;
+    st.pk[slot][0][0] = k.x;
+    st.pk[slot][0][1] = k.z;
+    st.pk[slot][1][0] = k.y;
+    st.pk[slot][1][1] = k.w;
     st.po[slot][0][0] = u.x;
     st.po[slot][0][1] = u.y;
     st.po[slot][1][0] = p.x;
@@ -255,16 +254,14 @@ __device__ __forceinline__ void mx_round(const MXArgs& A, const MXCtx& c, MXStat
             // lane stores, lanes and rows that own no cell into a pad column
             const bool zin = z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
-            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            const unsigned bz = zin ? c.bs : c.bpad;
             const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
             const double rp0 = rpo.x, rp1 = rpo.y;
             const double up0 = po[0][0] + c.alpha * po[1][0], up1 = po[0][1] + c.alpha * po[1][1];
             const double rr0 = rp0 + (double)(t0 - (float)rp0);
             const double rr1 = rp1 + (double)(t1 - (float)rp1);
-            stb(A.r_out, oz + b0, rr0);
-            stb(A.u, oz + b0, up0 + (double)d0);
-            stb(A.r_out, oz + b1, rr1);
-            stb(A.u, oz + b1, up1 + (double)d1);
+            stb(A.r_out, oz + bz, make_double2(rr0, rr1));
+            stb(A.u, oz + bz, make_double2(up0 + (double)d0, up1 + (double)d1));
             st.acc += o0 ? rr0 * rr0 : 0.0;
             st.acc += o1 ? rr1 * rr1 : 0.0;
         }
@@ -343,14 +340,11 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVE
     const int X0 = HALO + tx * OW;
     c.Y0 = HALO + ty * A.rows;
     const int k0 = X0 - LP + 2 * lane, k1 = k0 + 1;
-    c.bc0 = 8u * (unsigned)k0;
-    c.bc1 = 8u * (unsigned)k1;
     {
         // pairs start on even columns: the last one may hold the halo
         // column nx + 2, which no cell of the mesh reads (its face is zero)
         const int kp = min(max(k0, HALO), (c.nx + 1) & ~1);
         c.bp = 8u * (unsigned)kp;
-        c.hp = 4u * (unsigned)kp;
     }
     // faces HALO (west of column 2) and nx + 2 (east of column nx + 1)
     c.kw0 = k0 != HALO && k0 != c.nx + 2;
@@ -363,11 +357,14 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(TL_MX_WAVE
         c.bop = 8u * (unsigned)min(max(k0, X0), xlast & ~1);
     }
     {
-        // pad columns X .. sx-1 of every row are never read
-        const int npad = (int)(g.sx - g.X);
-        c.bpad = 8u * (unsigned)(g.X + lane % npad);
-        c.bs0 = c.out0 ? c.bc0 : c.bpad;
-        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+        // pad columns X .. sx-1 of every row are never read (at least 30
+        // of them); a lane whose first column is the last cell of an odd
+        // nx also writes the halo column nx + 2 past it, which no kernel
+        // reads either (faces are read as mirrored values)
+        const int xe = (int)g.X + ((int)g.X & 1);
+        const int npair = (int)(g.sx - xe) / 2;
+        c.bpad = 8u * (unsigned)(xe + 2 * (lane % npair));
+        c.bs = (c.out0 || c.out1) ? 8u * (unsigned)k0 : c.bpad;
     }
     c.ylo = c.Y0 - L;
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
@@ -386,10 +383,11 @@ void launch(Chunk& c, MXArgs& a)
     if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
 }
 
-__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long n)
+__global__ void k_to_f32(const double* __restrict__ kx, const double* __restrict__ ky, float* __restrict__ kxy,
+                         long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        b[i] = (float)a[i];
+        reinterpret_cast<float2*>(kxy)[i] = make_float2((float)kx[i], (float)ky[i]);
 }
 
 }  // namespace
@@ -406,12 +404,10 @@ void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int i
     if (!ppcg_mixed_supported(c, inner_steps))
         throw Error("mixed-precision PPCG needs no preconditioner and 1, 3 or 10 inner steps");
     const size_t n = (size_t)c.g.alloc;
-    if (!c.kx32) TL_HIP(hipMalloc(&c.kx32, sizeof(float) * n));
-    if (!c.ky32) TL_HIP(hipMalloc(&c.ky32, sizeof(float) * n));
+    if (!c.kxy32) TL_HIP(hipMalloc(&c.kxy32, sizeof(float) * 2 * n));
     if (!c.d_coef32) TL_HIP(hipMalloc(&c.d_coef32, sizeof(float) * 2 * MX_MAX_LEVELS));
     const unsigned blocks = (unsigned)std::min<long>(((long)n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_to_f32, dim3(blocks), dim3(256), 0, c.stream, c.f[F_KX], c.kx32, (long)n);
-    hipLaunchKernelGGL(k_to_f32, dim3(blocks), dim3(256), 0, c.stream, c.f[F_KY], c.ky32, (long)n);
+    hipLaunchKernelGGL(k_to_f32, dim3(blocks), dim3(256), 0, c.stream, c.f[F_KX], c.f[F_KY], c.kxy32, (long)n);
     TL_HIP(hipGetLastError());
     float h[2 * MX_MAX_LEVELS] = {};
     for (int k = 0; k < inner_steps; ++k) {
@@ -429,15 +425,14 @@ void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int i
 // reduction's first phase left in Chunk::partials (returns its block count).
 long op_ppcg_outer_mixed(Chunk& c, int inner_steps)
 {
-    if (c.mixed_steps != inner_steps || !c.kx32) throw Error("mixed-precision PPCG used before op_mixed_prepare");
+    if (c.mixed_steps != inner_steps || !c.kxy32) throw Error("mixed-precision PPCG used before op_mixed_prepare");
     MXArgs a{};
     a.u = c.f[F_U];
     a.p = c.f[F_P];
     a.r_in = c.f[F_R];
     a.w = c.f[F_W];
     a.r_out = c.r_alt;
-    a.kx = c.kx32;
-    a.ky = c.ky32;
+    a.kxy = c.kxy32;
     a.ds = c.ds;
     a.inv_theta = c.inv_theta;
     a.coef = c.d_coef32;
