@@ -429,7 +429,12 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
         const int nx = g.nx, ny = g.ny;
         const int c0 = HALO + tx * 2 * WAVE + 2 * lane, c1 = c0 + 1;
         const bool v0 = c0 <= nx + 1, v1 = c1 <= nx + 1;
-        const int k0 = min(c0, nx + 1), k1 = min(c1, nx + 1);
+        // a lane's two columns are one 16-byte load (c0 is even): cells from
+        // the pair clamped to start at or before nx + 1, kx faces at or
+        // before nx + 2; for an odd nx the last pair holds the halo column
+        // nx + 2, which the face select below never reads
+        const int k0 = min(c0, (nx + 1) & ~1);
+        const int f0 = min(c0, (nx + 2) & ~1);
         // the column beyond the wave's edge (west of lane 0, east of lane 63)
         const int ce = lane == 0 ? max(c0 - 1, HALO) : min(c1 + 1, nx + 1);
         const bool edge = lane == 0 || lane == WAVE - 1;
@@ -439,36 +444,49 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             const long i = gidx(g, col, row, 0);
             return beta * p[i] + q[i];
         };
+        auto ld2 = [&](const double* a, long i) { return *reinterpret_cast<const double2*>(a + i); };
+        auto pnew2 = [&](int row) {
+            const long i = gidx(g, k0, row, 0);
+            const double2 pp = ld2(p, i), qq = ld2(q, i);
+            return make_double2(beta * pp.x + qq.x, beta * pp.y + qq.y);
+        };
         // window: rows j-1 (s), j (c), j+1 (n) of the new p, two columns
         const int js = max(j0 - 1, HALO);
-        double s0 = pnew(k0, js), s1 = pnew(k1, js);
-        double c0v = pnew(k0, j0), c1v = pnew(k1, j0);
+        double2 t2 = pnew2(js);
+        double s0 = t2.x, s1 = t2.y;
+        t2 = pnew2(j0);
+        double c0v = t2.x, c1v = t2.y;
         double ec = edge ? pnew(ce, j0) : 0.0;
         if (j0 == HALO) {  // bottom face: the reflective mirror
             s0 = c0v;
             s1 = c1v;
         }
-        double kyc0 = k.ky[gidx(g, k0, j0, 0)], kyc1 = k.ky[gidx(g, k1, j0, 0)];
+        t2 = ld2(k.ky, gidx(g, k0, j0, 0));
+        double kyc0 = t2.x, kyc1 = t2.y;
         // Raw inputs are loaded one row ahead of their use, so each row's
         // loads are in flight while the previous row is computed: for row
         // j + 1 the new p (n, en) and ky; for row j the x-face coefficients.
         const int jn = min(j0 + 1, ny + 1);
-        double n0 = pnew(k0, jn), n1 = pnew(k1, jn);
+        t2 = pnew2(jn);
+        double n0 = t2.x, n1 = t2.y;
         double en = edge ? pnew(ce, jn) : 0.0;
-        double kyn0 = k.ky[gidx(g, k0, j0 + 1, 0)], kyn1 = k.ky[gidx(g, k1, j0 + 1, 0)];  // rows <= ny + 2
-        double kx0 = k.kx[gidx(g, min(c0, nx + 2), j0, 0)], kx1 = k.kx[gidx(g, min(c1, nx + 2), j0, 0)];
+        t2 = ld2(k.ky, gidx(g, k0, j0 + 1, 0));  // rows <= ny + 2
+        double kyn0 = t2.x, kyn1 = t2.y;
+        t2 = ld2(k.kx, gidx(g, f0, j0, 0));
+        double kx0 = t2.x, kx1 = t2.y;
         double kxl = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j0, 0)] : 0.0;
         for (int j = j0; j < j1; ++j) {
             // next row's raw inputs (row j + 2 for p, q, ky; row j + 1 for kx)
             const int j2 = min(j + 2, ny + 1), j1k = min(j + 1, ny + 1);
-            const long a0 = gidx(g, k0, j2, 0), a1 = gidx(g, k1, j2, 0), ae = gidx(g, ce, j2, 0);
-            const double pp0 = p[a0], pp1 = p[a1], qq0 = q[a0], qq1 = q[a1];
+            const long a0 = gidx(g, k0, j2, 0), ae = gidx(g, ce, j2, 0);
+            const double2 pp = ld2(p, a0), qq = ld2(q, a0);
+            const double pp0 = pp.x, pp1 = pp.y, qq0 = qq.x, qq1 = qq.y;
             const double ppe = edge ? p[ae] : 0.0, qqe = edge ? q[ae] : 0.0;
             // ky of row j + 2's south face = the north face of row j + 1
-            const double kyq0 = k.ky[gidx(g, k0, min(j + 2, ny + 2), 0)];
-            const double kyq1 = k.ky[gidx(g, k1, min(j + 2, ny + 2), 0)];
-            const double kxq0 = k.kx[gidx(g, min(c0, nx + 2), j1k, 0)];
-            const double kxq1 = k.kx[gidx(g, min(c1, nx + 2), j1k, 0)];
+            const double2 kyq = ld2(k.ky, gidx(g, k0, min(j + 2, ny + 2), 0));
+            const double kyq0 = kyq.x, kyq1 = kyq.y;
+            const double2 kxq = ld2(k.kx, gidx(g, f0, j1k, 0));
+            const double kxq0 = kxq.x, kxq1 = kxq.y;
             const double kxql = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j1k, 0)] : 0.0;
 
             double nn0 = n0, nn1 = n1;
@@ -476,7 +494,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
                 nn0 = c0v;
                 nn1 = c1v;
             }
-            const long i0 = gidx(g, k0, j, 0), i1 = gidx(g, k1, j, 0);
+            const long i0 = gidx(g, c0, j, 0);
             // kx lives on faces up to nx + 2: the lane past the last column
             // must hold kx of the right face for its west neighbour's DPP
             double kxe1 = dpp_from_east(kx0);
@@ -490,15 +508,15 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             if (c1 == nx + 1) e1 = c1v;
             const double m0 = (1.0 + (kx1 + kx0) + (kyn0 + kyc0)) * c0v - (kx1 * e0 + kx0 * w0) - (kyn0 * nn0 + kyc0 * s0);
             const double m1 = (1.0 + (kxe1 + kx1) + (kyn1 + kyc1)) * c1v - (kxe1 * e1 + kx1 * w1) - (kyn1 * nn1 + kyc1 * s1);
-            if (v0) {
+            if (v1) {
+                *reinterpret_cast<double2*>(p_out + i0) = make_double2(c0v, c1v);
+                *reinterpret_cast<double2*>(w + i0) = make_double2(m0, m1);
+                acc += m0 * c0v;
+                acc += m1 * c1v;
+            } else if (v0) {
                 p_out[i0] = c0v;
                 w[i0] = m0;
                 acc += m0 * c0v;
-            }
-            if (v1) {
-                p_out[i1] = c1v;
-                w[i1] = m1;
-                acc += m1 * c1v;
             }
             s0 = c0v;
             s1 = c1v;
@@ -1300,8 +1318,10 @@ void op_cg_calc_w_dev(Chunk& c, bool fold)
 #if TL_DIM == 2
         PWStrip t;
         t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
+        // strips of 64 rows (fewer strip starts, each re-reading two rows of
+        // p and r), shorter only while there would be under two waves per SIMD
         t.rows = 64;
-        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 8192) t.rows /= 2;
+        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
         t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         TIMED(c, K_CG_WP, {
