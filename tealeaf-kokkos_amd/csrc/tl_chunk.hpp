@@ -109,6 +109,7 @@ struct Chunk {
     // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
     // elements) and of the inner-step coefficients, made at the switch.
     float* kxy32 = nullptr;  // FP32 kx, ky interleaved: element i at [2i], [2i + 1]
+    unsigned* d_rowtab = nullptr;  // reflected row byte offsets of the temporal-blocked kernel (cell, face)
     float* d_coef32 = nullptr;  // a[0..n) then b[0..n)
     int mixed_steps = 0;
     double inv_theta = 0.0;

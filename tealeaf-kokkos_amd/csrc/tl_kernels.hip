@@ -1098,6 +1098,7 @@ Chunk::~Chunk()
     (void)hipFree(p_alt);
     (void)hipFree(r_alt);
     (void)hipFree(kxy32);
+    (void)hipFree(d_rowtab);
     (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);

@@ -47,11 +47,23 @@ struct TBArgs {
     const DevScalars* ds;
     double theta;
     const double* coef;  // a[0..L) then b[0..L) of this pass (device)
+    // byte offsets of the rows a strip reads (cell row, then face row, both
+    // reflected into the mesh) for rows row0 .. row0 + count - 1
+    const uint2* rowtab;
+    int row0;
     double* partials;
     int tiles_x;
     int rows;  // output rows per wave
     long nblocks;
 };
+
+// A wave-uniform double as scalar registers.
+__device__ __forceinline__ double uniform(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
 
 // Lane i receives lane i-1 (x-neighbour at kk-1); lane 0 receives 0.
 // Lanes 0 and 63 own no output cell, so the fill never reaches a result.
@@ -82,12 +94,12 @@ struct TBState {
     double rw[L][2][2];   // r of level s, slot (round - ylo) % 2, column
     double pf[2][6][2];   // prefetched rows: r, w|sd, u, p, kx, ky
     double acc;
+    double ca[L], cb[L];  // step coefficients (wave-uniform)
 };
 
 struct TBCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
-    bool small;  // ny too small for one reflection to reach every row a strip reads (SMALL)
     // byte offsets within a row: the lane's cells and kx faces reflected
     // into the mesh, store targets (the cell for lanes that own one, else a
     // pad column past the halo that nothing reads) and the pad column alone
@@ -98,7 +110,7 @@ struct TBCtx {
     double* ur;  // LDS rings [L+1][64][2]
     double* kxr;
     double* kyr;
-    const double* coef;  // LDS copy of a[0..L), b[0..L)
+    double* kxer;  // LDS ring [L+1][64]: kx at c1 + 1 (the east lane's c0)
 };
 
 __device__ __forceinline__ double ld(const double* row, unsigned boff)
@@ -143,23 +155,16 @@ __host__ __device__ inline int fold_face(int f, int n)
 // round, so that the number of memory operations in flight is the same on
 // every path and the compiler can wait for exactly the oldest slot instead
 // of draining all.  Rows outside the mesh read the rows they mirror (cells
-// for r, w|sd, u, p, kx; faces for ky).  An address is the array's base
-// plus a per-lane 32-bit byte offset of row and column (every field is under
-// 4 GiB).
-template <bool FIRST, bool SMALL>
+// for r, w|sd, u, p, kx; faces for ky), from a table of reflected row
+// offsets (one scalar load per round).  An address is the array's base plus
+// a per-lane 32-bit byte offset of row and column (every field is under 4 GiB).
+template <bool FIRST>
 __device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
 {
-    int yc, yf;
-    if (SMALL) {
-        yc = fold_cell(y, c.ny);
-        yf = fold_face(y, c.ny);
-    } else {
-        // one reflection reaches every row a strip reads: HALO - L .. ny + L + 8
-        yc = y < HALO ? 2 * HALO - 1 - y : (y > c.ny + 1 ? 2 * c.ny + 3 - y : y);
-        yf = y < HALO ? 2 * HALO - y : (y > c.ny + 2 ? 2 * c.ny + 4 - y : y);
-    }
-    const unsigned oy = 8u * (unsigned)(c.g.base + (long)yc * c.g.sx);
-    const unsigned of = 8u * (unsigned)(c.g.base + (long)yf * c.g.sx);
+    // a uniform read through the constant address space: a scalar load
+    typedef const __attribute__((address_space(4))) unsigned ConstU;
+    ConstU* t = (ConstU*)(const unsigned*)A.rowtab + 2 * (y - A.row0);
+    const unsigned oy = t[0], of = t[1];
     v[0][0] = ld(A.r_in, oy + c.bc0);
     v[0][1] = ld(A.r_in, oy + c.bc1);
     const double* b = FIRST ? A.w : A.sd_in;
@@ -186,7 +191,7 @@ __device__ __forceinline__ int ring(int row, int n)
 // One round: level 0 at row y, then level s at row y - s for s = 1..L.  The LDS rings of
 // level s + 1 are read before level s computes, so their latency overlaps
 // level s's arithmetic (the rows differ, so no write of level s is missed).
-template <int L, bool FIRST, bool LAST, bool SMALL, int PH>
+template <int L, bool FIRST, bool LAST, int PH>
 __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y)
 {
     constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
@@ -200,7 +205,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     // inputs of level 1 (row y - 1)
     int rn = ring(y - 1, L + 1);
     double2 nkx = kxr[rn * WAVE + lane], nky = kyr[rn * WAVE + lane], nu = ur[rn * WAVE + lane];
-    double na = c.coef[0], nb = c.coef[L];
+    double nkxe = c.kxer[rn * WAVE + lane];
 
     // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
     const int ry = ring(y, L + 1);
@@ -226,27 +231,28 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         }
         ur[ry * WAVE + lane] = u0;
         kxr[ry * WAVE + lane] = make_double2(v[4][0], v[4][1]);
+        c.kxer[ry * WAVE + lane] = from_east(v[4][0]);
         kyr[ry * WAVE + lane] = make_double2(v[5][0], v[5][1]);
         kyn0 = v[5][0];
         kyn1 = v[5][1];
     }
     // prefetch row y + 2 into the slot just consumed
-    tb_load<FIRST, SMALL>(A, c, y + 2, v);
+    tb_load<FIRST>(A, c, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
         const int z = y - s;
         const int rz = rn;
         const double2 kx = nkx, ky = nky;
+        const double kxe1 = nkxe;
         double2 u = nu;
-        const double a = na, b = nb;
+        const double a = st.ca[s - 1], b = st.cb[s - 1];
         if (s < L) {
             rn = ring(z - 1, L + 1);
             nkx = kxr[rn * WAVE + lane];
             nky = kyr[rn * WAVE + lane];
             nu = ur[rn * WAVE + lane];
-            na = c.coef[s];
-            nb = c.coef[L + s];
+            nkxe = c.kxer[rn * WAVE + lane];
         }
         double rn0, rn1;
         {
@@ -256,7 +262,6 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
             // x-neighbours: c0's west is the previous lane's c1, c1's east the next lane's c0
             const double w0 = from_west(c1), e1 = from_east(c0);
             const double e0 = c1, w1 = c0;
-            const double kxe1 = from_east(kx.x);
             // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z)
             const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
             const double m1 = (1.0 + (kxe1 + kx.y) + (kyn1 + ky.y)) * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
@@ -300,7 +305,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     }
 }
 
-template <int L, bool FIRST, bool LAST, bool SMALL>
+template <int L, bool FIRST, bool LAST>
 __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
 {
     TBState<L> st;
@@ -317,8 +322,13 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
         st.pf[0][k][0] = st.pf[0][k][1] = st.pf[1][k][0] = st.pf[1][k][1] = 0.0;
     }
     st.acc = 0.0;
-    tb_load<FIRST, SMALL>(A, c, c.ylo, st.pf[0]);
-    tb_load<FIRST, SMALL>(A, c, c.ylo + 1, st.pf[1]);
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        st.ca[s] = uniform(A.coef[s]);
+        st.cb[s] = uniform(A.coef[L + s]);
+    }
+    tb_load<FIRST>(A, c, c.ylo, st.pf[0]);
+    tb_load<FIRST>(A, c, c.ylo + 1, st.pf[1]);
     // wait for the first two rows here: with loads still in flight on entry
     // the compiler's wait at the loop head has to cover the entry path too,
     // and it then drains all loads at the head of every block of rounds
@@ -328,17 +338,17 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
     // it then makes of registers with loads in flight wait for those loads)
     const int yend = c.yhi + L;
     for (int y = c.ylo; y <= yend; y += 6) {
-        tb_round<L, FIRST, LAST, SMALL, 0>(A, c, st, y);
+        tb_round<L, FIRST, LAST, 0>(A, c, st, y);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, SMALL, 1>(A, c, st, y + 1);
+        tb_round<L, FIRST, LAST, 1>(A, c, st, y + 1);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, SMALL, 2>(A, c, st, y + 2);
+        tb_round<L, FIRST, LAST, 2>(A, c, st, y + 2);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, SMALL, 3>(A, c, st, y + 3);
+        tb_round<L, FIRST, LAST, 3>(A, c, st, y + 3);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, SMALL, 4>(A, c, st, y + 4);
+        tb_round<L, FIRST, LAST, 4>(A, c, st, y + 4);
         __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, SMALL, 5>(A, c, st, y + 5);
+        tb_round<L, FIRST, LAST, 5>(A, c, st, y + 5);
         __builtin_amdgcn_sched_barrier(0);
     }
     if (LAST) {
@@ -353,10 +363,8 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     __shared__ double2 u_ring[(L + 1) * WAVE];
     __shared__ double2 kx_ring[(L + 1) * WAVE];
     __shared__ double2 ky_ring[(L + 1) * WAVE];
-    __shared__ double coef[2 * L];
+    __shared__ double kxe_ring[(L + 1) * WAVE];
     const int lane = threadIdx.x;
-    if (lane < 2 * L) coef[lane] = A.coef[lane];
-    __syncthreads();
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
     constexpr int OW = 2 * WAVE - 2 * L;
@@ -371,7 +379,6 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.bc1 = 8u * (unsigned)fold_cell(k1, c.nx);
     c.bf0 = 8u * (unsigned)fold_face(k0, c.nx);
     c.bf1 = 8u * (unsigned)fold_face(k1, c.nx);
-    c.small = c.ny < L + 10;
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
     c.out_any = c.out0 || c.out1;
@@ -389,9 +396,8 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.ur = reinterpret_cast<double*>(u_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
-    c.coef = coef;
-    if (c.small) tb_wave<L, FIRST, LAST, true>(A, c);
-    else tb_wave<L, FIRST, LAST, false>(A, c);
+    c.kxer = kxe_ring;
+    tb_wave<L, FIRST, LAST>(A, c);
 }
 
 template <int L, bool FIRST, bool LAST>
