@@ -288,7 +288,9 @@ def test_random_stencil_inputs(hip, oracle, dim):
 
 
 @pytest.mark.parametrize("shape,nsteps", [((37, 23), 10), ((150, 140), 10), ((150, 140), 3), ((64, 200), 1),
-                                          ((130, 70), 12), ((90, 100), 15), ((45, 66), 25)])
+                                          ((130, 70), 12), ((90, 100), 15), ((45, 66), 25),
+                                          # meshes narrower than the levels' reach: reflections repeat
+                                          ((5, 4), 10), ((9, 3), 12), ((2, 17), 5), ((131, 6), 10)])
 def test_ppcg_outer_fused(hip, oracle, shape, nsteps):
     """The driver's temporal-blocked PPCG outer iteration (calc_ur, init_sd,
     all inner steps and ||r||^2 in one sweep) against the reference's

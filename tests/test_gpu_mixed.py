@@ -47,7 +47,7 @@ def apply_a(x, kx, ky):
 
 
 @pytest.mark.parametrize("shape,nsteps", [((37, 23), 10), ((150, 140), 10), ((150, 140), 3), ((64, 200), 1),
-                                          ((300, 257), 10)])
+                                          ((300, 257), 10), ((5, 4), 10), ((3, 9), 3)])
 def test_ppcg_outer_mixed(hip, oracle, shape, nsteps):
     nx, ny = shape
     print(f"mixed outer {shape} nsteps={nsteps}", flush=True)
