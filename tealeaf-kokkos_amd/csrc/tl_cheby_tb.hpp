@@ -1,0 +1,282 @@
+// Temporal-blocked Chebyshev iterations for the 2D 5-point stencil: the
+// device code, instantiated per pass size in tl_cheby_tb_l*.hip.
+//
+// A Chebyshev iteration of the reference (ChebyIterate + ChebyCalcU,
+// ext_cheby_kernel.hpp:75-115) is w = A u, r = u0 - w, p = a_s p + b_s r,
+// u += p over the whole mesh; the driver checks ||r||^2 only every tenth
+// iteration past its estimate (tea_solve.f90:260), so the K iterations up
+// to the next check need no global result and one kernel runs them in a
+// single pass over HBM, with the wavefront of the temporal-blocked PPCG
+// kernel (tl_ppcg_tb.hpp): level s (iteration s of the pass) at row y - s is
+// computed as soon as level s - 1 has row y - s + 1, from a three-row
+// register window of u and a two-row window of p per level; u0, kx, ky wait
+// in LDS rings for the levels that reach their row.  Faces are mirrored
+// loads, as there.  Each cell's value is the reference's expressions in its
+// order, so u, p and r are bitwise those of K separate iterations; HBM
+// traffic per K iterations is 40 + 24 B/cell (read u, p, u0, kx, ky; write
+// u, p, r) instead of K x 64.
+#pragma once
+
+#include "tl_ppcg_tb.hpp"
+
+namespace tl {
+
+#if TL_DIM == 2
+namespace ctb {
+
+using tb::fold_cell;
+using tb::fold_face;
+using tb::from_east;
+using tb::from_west;
+using tb::ld;
+using tb::ring;
+using tb::uniform;
+
+struct CTBArgs {
+    const double* u_in;
+    double* u_out;  // != u_in: neighbouring tiles read u_in across their halo
+    const double* p_in;
+    double* p_out;  // != p_in, for the same reason
+    double* r;
+    const double* u0;
+    const double* kx;
+    const double* ky;
+    const double* ca;  // a[0..K) and b[0..K) of this pass (device)
+    const double* cb;
+    const uint2* rowtab;  // reflected row byte offsets (cell row, face row)
+    int row0;
+    double* partials;
+    int tiles_x;
+    int rows;
+    long nblocks;
+};
+
+template <int K>
+struct CTBState {
+    double uw[K][3][2];  // u after iteration s, slot (round) % 3, column
+    double pw[K][2][2];  // p after iteration s, slot (round) % 2
+    double pf[2][5][2];  // prefetched rows: u, p, u0, kx, ky
+    double acc;
+    double ca[K], cb[K];
+};
+
+struct CTBCtx {
+    Geom g;
+    int ny, ylo, Y0, yhi;
+    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
+    bool out0, out1;
+    double2* u0r;  // LDS rings [K+1][64]
+    double2* kxr;
+    double2* kyr;
+    double* kxer;  // kx at c1 + 1 (the east lane's c0)
+};
+
+__device__ __forceinline__ void stv(double* a, unsigned boff, double v)
+{
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + boff) = v;
+}
+
+// Row y into a prefetch slot; rows outside the mesh read the rows they
+// mirror (one scalar load of the row table per round).
+__device__ __forceinline__ void ctb_load(const CTBArgs& A, const CTBCtx& c, int y, double (&v)[5][2])
+{
+    typedef const __attribute__((address_space(4))) unsigned ConstU;
+    ConstU* t = (ConstU*)(const unsigned*)A.rowtab + 2 * (y - A.row0);
+    const unsigned oy = t[0], of = t[1];
+    v[0][0] = ld(A.u_in, oy + c.bc0);
+    v[0][1] = ld(A.u_in, oy + c.bc1);
+    v[1][0] = ld(A.p_in, oy + c.bc0);
+    v[1][1] = ld(A.p_in, oy + c.bc1);
+    v[2][0] = ld(A.u0, oy + c.bc0);
+    v[2][1] = ld(A.u0, oy + c.bc1);
+    v[3][0] = ld(A.kx, oy + c.bf0);
+    v[3][1] = ld(A.kx, oy + c.bf1);
+    v[4][0] = ld(A.ky, of + c.bc0);
+    v[4][1] = ld(A.ky, of + c.bc1);
+}
+
+// One round: the loaded row y enters as iteration 0's u and p, iteration s
+// updates row y - s (s = 1..K); row y - K leaves the pass.
+template <int K, bool LAST, int PH>
+__device__ __forceinline__ void ctb_round(const CTBArgs& A, const CTBCtx& c, CTBState<K>& st, int y)
+{
+    constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
+    constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
+    const int lane = threadIdx.x;
+    double(&v)[5][2] = st.pf[PH % 2];
+
+    int rn = ring(y - 1, K + 1);
+    double2 nkx = c.kxr[rn * WAVE + lane], nky = c.kyr[rn * WAVE + lane], nu0 = c.u0r[rn * WAVE + lane];
+    double nkxe = c.kxer[rn * WAVE + lane];
+
+    const int ry = ring(y, K + 1);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        st.uw[0][S0][q] = v[0][q];
+        st.pw[0][R0][q] = v[1][q];
+    }
+    c.u0r[ry * WAVE + lane] = make_double2(v[2][0], v[2][1]);
+    c.kxr[ry * WAVE + lane] = make_double2(v[3][0], v[3][1]);
+    c.kxer[ry * WAVE + lane] = from_east(v[3][0]);
+    c.kyr[ry * WAVE + lane] = make_double2(v[4][0], v[4][1]);
+    double kyn0 = v[4][0], kyn1 = v[4][1];
+    ctb_load(A, c, y + 2, v);
+
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+        const int z = y - s;
+        const double2 kx = nkx, ky = nky, u0 = nu0;
+        const double kxe1 = nkxe;
+        const double a = st.ca[s - 1], b = st.cb[s - 1];
+        if (s < K) {
+            rn = ring(z - 1, K + 1);
+            nkx = c.kxr[rn * WAVE + lane];
+            nky = c.kyr[rn * WAVE + lane];
+            nu0 = c.u0r[rn * WAVE + lane];
+            nkxe = c.kxer[rn * WAVE + lane];
+        }
+        const double c0 = st.uw[s - 1][S1][0], c1 = st.uw[s - 1][S1][1];
+        const double s0 = st.uw[s - 1][S2][0], s1 = st.uw[s - 1][S2][1];
+        const double n0 = st.uw[s - 1][S0][0], n1 = st.uw[s - 1][S0][1];
+        const double w0 = from_west(c1), e1 = from_east(c0);
+        const double e0 = c1, w1 = c0;
+        // ChebyIterate: w = A u (SMVP, ext_shared.hpp:37-41), r = u0 - w,
+        // p = a p + b r; ChebyCalcU: u += p
+        const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
+        const double m1 = (1.0 + (kxe1 + kx.y) + (kyn1 + ky.y)) * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
+        kyn0 = ky.x;
+        kyn1 = ky.y;
+        const double r0 = u0.x - m0, r1 = u0.y - m1;
+        const double p0 = a * st.pw[s - 1][R1][0] + b * r0;
+        const double p1 = a * st.pw[s - 1][R1][1] + b * r1;
+        const double un0 = c0 + p0, un1 = c1 + p1;
+        if (s < K) {
+            st.uw[s][S0][0] = un0;
+            st.uw[s][S0][1] = un1;
+            st.pw[s][R0][0] = p0;
+            st.pw[s][R0][1] = p1;
+        } else {
+            // every lane stores, lanes and rows that own no cell into a pad column
+            const bool zin = z >= c.Y0 && z <= c.yhi;
+            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
+            stv(A.u_out, oz + b0, un0);
+            stv(A.p_out, oz + b0, p0);
+            stv(A.u_out, oz + b1, un1);
+            stv(A.p_out, oz + b1, p1);
+            if (LAST) {
+                stv(A.r, oz + b0, r0);
+                stv(A.r, oz + b1, r1);
+                st.acc += zin && c.out0 ? r0 * r0 : 0.0;
+                st.acc += zin && c.out1 ? r1 * r1 : 0.0;
+            }
+        }
+    }
+}
+
+template <int K, bool LAST>
+__device__ __forceinline__ void ctb_wave(const CTBArgs& A, const CTBCtx& c)
+{
+    CTBState<K> st;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            st.uw[s][0][q] = st.uw[s][1][q] = st.uw[s][2][q] = 0.0;
+            st.pw[s][0][q] = st.pw[s][1][q] = 0.0;
+        }
+        st.ca[s] = uniform(A.ca[s]);
+        st.cb[s] = uniform(A.cb[s]);
+    }
+    st.acc = 0.0;
+    ctb_load(A, c, c.ylo, st.pf[0]);
+    ctb_load(A, c, c.ylo + 1, st.pf[1]);
+    // the first rows complete before the loop, so the loop head waits only
+    // for the oldest prefetch slot (see tl_ppcg_tb.hpp)
+    __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
+    const int yend = c.yhi + K;
+    for (int y = c.ylo; y <= yend; y += 6) {
+        ctb_round<K, LAST, 0>(A, c, st, y);
+        __builtin_amdgcn_sched_barrier(0);
+        ctb_round<K, LAST, 1>(A, c, st, y + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        ctb_round<K, LAST, 2>(A, c, st, y + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        ctb_round<K, LAST, 3>(A, c, st, y + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        ctb_round<K, LAST, 4>(A, c, st, y + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        ctb_round<K, LAST, 5>(A, c, st, y + 5);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (LAST) {
+        double v[1] = {st.acc};
+        block_partial<1, 1>(v, Reduce{A.partials});
+    }
+}
+
+template <int K, bool LAST>
+__global__ __launch_bounds__(WAVE) void k_cheby_tb(Geom g, CTBArgs A)
+{
+    __shared__ double2 u0_ring[(K + 1) * WAVE];
+    __shared__ double2 kx_ring[(K + 1) * WAVE];
+    __shared__ double2 ky_ring[(K + 1) * WAVE];
+    __shared__ double kxe_ring[(K + 1) * WAVE];
+    const int lane = threadIdx.x;
+    const long t = xcd_tile(blockIdx.x, A.nblocks);
+    const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
+    constexpr int OW = 2 * WAVE - 2 * K;
+    CTBCtx c;
+    c.g = g;
+    c.ny = g.ny;
+    const int X0 = HALO + tx * OW;
+    c.Y0 = HALO + ty * A.rows;
+    const int k0 = X0 - K + 2 * lane, k1 = k0 + 1;
+    c.bc0 = 8u * (unsigned)fold_cell(k0, g.nx);
+    c.bc1 = 8u * (unsigned)fold_cell(k1, g.nx);
+    c.bf0 = 8u * (unsigned)fold_face(k0, g.nx);
+    c.bf1 = 8u * (unsigned)fold_face(k1, g.nx);
+    c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= g.nx + 1;
+    c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= g.nx + 1;
+    {
+        // pad columns X .. sx-1 of every row are never read
+        const int npad = (int)(g.sx - g.X);
+        c.bpad = 8u * (unsigned)(g.X + lane % npad);
+        c.bs0 = c.out0 ? c.bc0 : c.bpad;
+        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+    }
+    c.ylo = c.Y0 - K;
+    c.yhi = min(c.Y0 + A.rows - 1, g.ny + 1);
+    c.u0r = u0_ring;
+    c.kxr = kx_ring;
+    c.kyr = ky_ring;
+    c.kxer = kxe_ring;
+    ctb_wave<K, LAST>(A, c);
+}
+
+template <int K, bool LAST>
+void launch(Chunk& c, CTBArgs& a)
+{
+    hipLaunchKernelGGL((k_cheby_tb<K, LAST>), dim3((unsigned)a.nblocks), dim3(WAVE), 0, c.stream, c.g, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+// Launchers of the compiled pass sizes (one translation unit each); last =
+// the pass writes r and the first phase of ||r||^2.
+void ctb_launch_1(Chunk& c, CTBArgs& a, bool last);
+void ctb_launch_2(Chunk& c, CTBArgs& a, bool last);
+void ctb_launch_5(Chunk& c, CTBArgs& a, bool last);
+void ctb_launch_10(Chunk& c, CTBArgs& a, bool last);
+
+#define TL_CTB_DEFINE_LAUNCH(N)                               \
+    void ctb_launch_##N(Chunk& c, CTBArgs& a, bool last)      \
+    {                                                         \
+        if (last) launch<N, true>(c, a);                      \
+        else launch<N, false>(c, a);                          \
+    }
+
+}  // namespace ctb
+#endif
+
+}  // namespace tl

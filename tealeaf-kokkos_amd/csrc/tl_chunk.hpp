@@ -51,7 +51,7 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 // Kernels timed by the optional per-launch event timer.
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
-    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_COUNT
+    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -109,7 +109,9 @@ struct Chunk {
     // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
     // elements) and of the inner-step coefficients, made at the switch.
     float* kxy32 = nullptr;  // FP32 kx, ky interleaved: element i at [2i], [2i + 1]
-    unsigned* d_rowtab = nullptr;  // reflected row byte offsets of the temporal-blocked kernel (cell, face)
+    unsigned* d_rowtab = nullptr;  // reflected row byte offsets of the temporal-blocked kernels (cell, face)
+    double* d_cheby = nullptr;     // Chebyshev a[0..n) then b[0..n) on the device
+    int d_cheby_n = 0, d_cheby_cap = 0;
     float* d_coef32 = nullptr;  // a[0..n) then b[0..n)
     int mixed_steps = 0;
     double inv_theta = 0.0;
@@ -198,6 +200,20 @@ int tb_rows(int ny, int tiles_x, int L, int slots);
 std::vector<int> tb_passes(int inner_steps);
 bool ppcg_tb_supported(const Chunk& c, int inner_steps);
 long op_ppcg_outer_tb(Chunk& c, int inner_steps);
+// First row of the reflected-row table of the temporal-blocked kernels
+// (2D): entry y - TB_ROWTAB0 holds the byte offsets of the cell row and the
+// face row that row y reads.
+constexpr int TB_ROWTAB0 = HALO - 14;
+#if TL_DIM == 2
+const uint2* tb_rowtab(Chunk& c);
+#endif
+// Temporal-blocked Chebyshev (2D, no preconditioner): `count` iterations
+// from the 1-based step `step1` in passes of at most 10; with `norm` the
+// last writes r and leaves the first phase of ||r||^2 (returns its block
+// count, closed by op_finish_norm).
+bool cheby_tb_supported(const Chunk& c);
+long op_cheby_tb(Chunk& c, int step1, int count, bool norm);
+double op_finish_norm(Chunk& c, long nblocks);
 
 // Flexible CG (tl_learn.hip): a . b over the interior; p = z + beta p.
 double op_dot(Chunk& c, const double* a, const double* b);

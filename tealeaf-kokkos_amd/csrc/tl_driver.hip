@@ -471,6 +471,17 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     est_itc = (int)std::lround(std::log(it_alpha) / (2.0 * std::log(gamm)));
                     cheby_calc_steps = 1;
                     switch_step = n;
+                } else if (cheby_tb_supported(c)) {
+                    // iterations n .. m in one temporal-blocked sweep, m the
+                    // next that checks ||r||^2 (or the last allowed): nothing
+                    // between them reads a global result
+                    int m = n;
+                    while (m < max_iters && !((m >= est_itc) && (m % 10 == 0))) ++m;
+                    const bool check = (m >= est_itc) && (m % 10 == 0);
+                    const long nb = op_cheby_tb(c, cheby_calc_steps, m - n + 1, check);
+                    if (check) error = op_finish_norm(c, nb);
+                    cheby_calc_steps += m - n;
+                    n = m;
                 } else {
                     op_cheby_iterate(c, cheby_calc_steps, true);
                     if ((n >= est_itc) && (n % 10 == 0)) error = op_calc_2norm(c, true);

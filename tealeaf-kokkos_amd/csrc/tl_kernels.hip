@@ -920,7 +920,7 @@ const char* kernel_name(int k)
     static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_p_w", "cg_calc_ur", "cg_calc_p",
                                          "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
-                                         "field_summary", "update_halo", "ppcg_outer_mixed"};
+                                         "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -943,6 +943,7 @@ double kernel_bytes_per_cell(int k)
     case K_FINALISE: return 24.0;
     case K_SUMMARY: return 32.0;
     case K_PPCG_MIXED: return 48.0 + kb / 2;  // u, p, r, w, FP32 coefficients -> u, r
+    case K_CHEBY_TB: return 48.0 + kb;        // u, p, u0, coefficients -> u', p', r (per pass)
     default: return 0.0;
     }
 }
@@ -1099,6 +1100,7 @@ Chunk::~Chunk()
     (void)hipFree(r_alt);
     (void)hipFree(kxy32);
     (void)hipFree(d_rowtab);
+    (void)hipFree(d_cheby);
     (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);
@@ -1374,6 +1376,12 @@ double op_error(Chunk& c) { return c.result(S_RRN); }
 
 void op_finish_rrn(Chunk& c, long nblocks) { finish<1>(c, nblocks, S_RRN, EP_RRN); }
 
+double op_finish_norm(Chunk& c, long nblocks)
+{
+    finish<1>(c, nblocks, S_NORM, EP_STORE);
+    return c.result(S_NORM);
+}
+
 void op_2norm_replace_dev(Chunk& c)
 {
     TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));
@@ -1434,6 +1442,21 @@ void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, d
     c.preconditioner = preconditioner;
     store_coefs(c, alphas, betas, n);
     const int pre = preconditioner;
+    if (fused && TL_DIM == 2 && n > 0) {
+        // device copy for the temporal-blocked iterations: a[0..n) then b[0..n)
+        if (2 * n > c.d_cheby_cap) {
+            c.sync();
+            (void)hipFree(c.d_cheby);
+            c.d_cheby = nullptr;
+            TL_HIP(hipMalloc(&c.d_cheby, sizeof(double) * 2 * (size_t)n));
+            c.d_cheby_cap = 2 * n;
+        }
+        std::vector<double> h(c.alphas);
+        h.insert(h.end(), c.betas.begin(), c.betas.end());
+        c.sync();
+        TL_HIP(hipMemcpy(c.d_cheby, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        c.d_cheby_n = n;
+    }
     if (fused) {
         STRIP_LAUNCH((k_cheby<true, true>), c, kptr(c), pre, theta, 0.0, 0.0, c.f[F_U], c.f[F_U0], c.f[F_MI],
                      c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt);

@@ -79,18 +79,18 @@ void tb_coefs(Chunk& c, int inner_steps, const std::vector<int>& passes)
     c.d_coef_steps = inner_steps;
 }
 
+}  // namespace
+
 // Rows a strip reads: HALO - 12 .. ny + 20 (12 levels at most, the
 // prefetch and the six-round unroll); one uint2 (cell row, face row byte
 // offsets, reflected into the mesh) per row, built once per chunk.
-constexpr int ROWTAB0 = HALO - 14;
-
 const uint2* tb_rowtab(Chunk& c)
 {
     if (!c.d_rowtab) {
         const int count = c.g.ny + 36;
         std::vector<unsigned> h(2 * (size_t)count);
         for (int k = 0; k < count; ++k) {
-            const int y = ROWTAB0 + k;
+            const int y = TB_ROWTAB0 + k;
             h[2 * k] = 8u * (unsigned)(c.g.base + (long)tb::fold_cell(y, c.g.ny) * c.g.sx);
             h[2 * k + 1] = 8u * (unsigned)(c.g.base + (long)tb::fold_face(y, c.g.ny) * c.g.sx);
         }
@@ -99,8 +99,6 @@ const uint2* tb_rowtab(Chunk& c)
     }
     return reinterpret_cast<const uint2*>(c.d_rowtab);
 }
-
-}  // namespace
 
 // One PPCG outer iteration after CGCalcW: alpha is read from the device
 // scalars, u and r are updated, and the rrn reduction's first phase is left
@@ -132,7 +130,7 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.theta = c.theta;
         a.coef = c.d_coef + 2 * done;
         a.rowtab = tb_rowtab(c);
-        a.row0 = ROWTAB0;
+        a.row0 = TB_ROWTAB0;
         a.partials = c.partials;
         const int ow = 2 * WAVE - 2 * L;
         a.tiles_x = (c.g.nx + ow - 1) / ow;

@@ -57,6 +57,8 @@ CASES_2D = [
     ("cg", 96, 2, {"extra": ("tl_preconditioner_on",)}),
     ("chebyshev", 180, 2, {"extra": ("tl_preconditioner_on",)}),
     ("chebyshev", 160, 2, {"extra": ("tl_ch_cg_errswitch", "tl_ch_cg_epslim=0.001")}),
+    # temporal-blocked passes: a non-square mesh, and a cap that falls between norm checks
+    ("chebyshev", 130, 2, {"ny": 47}), ("chebyshev", 128, 1, {"max_iters": 157}),
     ("cg", 130, 2, {"extra": ("tl_coefficient_inverrse_density", "tl_check_result")}),
 ]
 
