@@ -51,7 +51,8 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 // Kernels timed by the optional per-launch event timer.
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
-    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_COUNT
+    K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_JACOBI_TB,
+    K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -111,6 +112,7 @@ struct Chunk {
     float* kxy32 = nullptr;  // FP32 kx, ky interleaved: element i at [2i], [2i + 1]
     unsigned* d_rowtab = nullptr;  // reflected row byte offsets of the temporal-blocked kernels (cell, face)
     double* d_cheby = nullptr;     // Chebyshev a[0..n) then b[0..n) on the device
+    double* d_levels = nullptr;    // per-iteration error sums of a temporal-blocked Jacobi pass
     int d_cheby_n = 0, d_cheby_cap = 0;
     float* d_coef32 = nullptr;  // a[0..n) then b[0..n)
     int mixed_steps = 0;
@@ -214,6 +216,13 @@ const uint2* tb_rowtab(Chunk& c);
 bool cheby_tb_supported(const Chunk& c);
 long op_cheby_tb(Chunk& c, int step1, int count, bool norm);
 double op_finish_norm(Chunk& c, long nblocks);
+// Temporal-blocked Jacobi (2D): one pass of K (1, 2, 5 or 10) iterations,
+// K = jacobi_tb_pass(iterations left); returns each iteration's error sum;
+// op_jacobi_tb_undo returns u to the pass's input.
+bool jacobi_tb_supported(const Chunk& c);
+int jacobi_tb_pass(int remaining);
+std::vector<double> op_jacobi_tb(Chunk& c, int K);
+void op_jacobi_tb_undo(Chunk& c);
 
 // Flexible CG (tl_learn.hip): a . b over the interior; p = z + beta p.
 double op_dot(Chunk& c, const double* a, const double* b);

@@ -551,6 +551,31 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             op_cg_calc_ur_dev(c, want_history ? n : 0);
             p_pending = true;
             error = op_error(c);
+        } else if (d.use_jacobi && jacobi_tb_supported(c)) {
+            // iterations n .. n + K - 1 in one temporal-blocked pass, ending
+            // at the next multiple of 50 (the residual check) at the latest;
+            // if an iteration inside it meets eps, the pass is run again from
+            // its input up to that iteration
+            const int next50 = (n + 49) / 50 * 50;
+            int K = jacobi_tb_pass(std::min(max_iters, next50) - n + 1);
+            std::vector<double> err = op_jacobi_tb(c, K);
+            int k = 0;
+            while (k < K && !(std::fabs(err[k]) < d.eps)) ++k;
+            if (k < K - 1) {
+                op_jacobi_tb_undo(c);
+                for (int left = k + 1; left > 0;) {
+                    const int m = jacobi_tb_pass(left);
+                    (void)op_jacobi_tb(c, m);
+                    left -= m;
+                }
+                K = k + 1;
+            }
+            error = err[K - 1];
+            n += K - 1;
+            if (n % 50 == 0) {
+                op_calc_residual(c);
+                error = op_calc_2norm(c, true);
+            }
         } else if (d.use_jacobi) {
             error = op_jacobi_solve(c, true);
             if (n % 50 == 0) {

@@ -920,7 +920,8 @@ const char* kernel_name(int k)
     static const char* names[K_COUNT] = {"cg_init", "cg_calc_w", "cg_calc_p_w", "cg_calc_ur", "cg_calc_p",
                                          "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
-                                         "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb"};
+                                         "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
+                                         "jacobi_tb"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -944,6 +945,7 @@ double kernel_bytes_per_cell(int k)
     case K_SUMMARY: return 32.0;
     case K_PPCG_MIXED: return 48.0 + kb / 2;  // u, p, r, w, FP32 coefficients -> u, r
     case K_CHEBY_TB: return 48.0 + kb;        // u, p, u0, coefficients -> u', p', r (per pass)
+    case K_JACOBI_TB: return 32.0 + kb;       // u, u0, coefficients -> u', r (per pass)
     default: return 0.0;
     }
 }
@@ -1101,6 +1103,7 @@ Chunk::~Chunk()
     (void)hipFree(kxy32);
     (void)hipFree(d_rowtab);
     (void)hipFree(d_cheby);
+    (void)hipFree(d_levels);
     (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);

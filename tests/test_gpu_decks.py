@@ -59,6 +59,9 @@ CASES_2D = [
     ("chebyshev", 160, 2, {"extra": ("tl_ch_cg_errswitch", "tl_ch_cg_epslim=0.001")}),
     # temporal-blocked passes: a non-square mesh, and a cap that falls between norm checks
     ("chebyshev", 130, 2, {"ny": 47}), ("chebyshev", 128, 1, {"max_iters": 157}),
+    # Jacobi that converges inside a pass (the pass is run again up to that iteration), and a cap
+    # that is not a multiple of the pass
+    ("jacobi", 40, 2, {"max_iters": 20000}), ("jacobi", 72, 1, {"max_iters": 1237, "ny": 51}),
     ("cg", 130, 2, {"extra": ("tl_coefficient_inverrse_density", "tl_check_result")}),
 ]
 
