@@ -282,43 +282,57 @@ __global__ void k_avgpool_bwd(const u16* dy, u16* dx, int n, int h, int w, int c
 }
 
 // cat[..][0:ca] = nearest 2x upsample of a (h/2 x w/2), cat[..][ca:ca+cb] = skip
+// 8 channels (16 bytes) per thread: ca and cb are multiples of 8
 __global__ void k_up_concat(const u16* a, const u16* skip, u16* cat, int n, int h, int w, int ca, int cb)
 {
-    const int c = ca + cb;
-    const long total = (long)n * h * w * c;
+    const int c = ca + cb, c8 = c / 8, ca8 = ca / 8;
+    const long total = (long)n * h * w * c8;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int ch = (int)(i % c);
-        const long q = i / c;
+        const int j = (int)(i % c8);
+        const long q = i / c8;  // pixel
         const int x = (int)(q % w);
         const long r = q / w;
         const int y = (int)(r % h), b = (int)(r / h);
-        cat[i] = ch < ca ? a[(((long)b * (h / 2) + y / 2) * (w / 2) + x / 2) * ca + ch]
-                         : skip[(((long)b * h + y) * w + x) * cb + (ch - ca)];
+        const uint4 v = j < ca8 ? ld16(a + (((long)b * (h / 2) + y / 2) * (w / 2) + x / 2) * ca + 8 * j)
+                                : ld16(skip + q * cb + 8 * (j - ca8));
+        *reinterpret_cast<uint4*>(cat + q * c + 8 * j) = v;
     }
 }
 
 // backward of k_up_concat: da = 2x2 sum of the first ca channels, dskip = the rest
 __global__ void k_up_concat_bwd(const u16* dcat, u16* da, u16* dskip, int n, int h, int w, int ca, int cb)
 {
-    const int c = ca + cb;
+    const int c = ca + cb, ca8 = ca / 8, cb8 = cb / 8;
     const int ho = h / 2, wo = w / 2;
-    const long ta = (long)n * ho * wo * ca;
+    const long ta = (long)n * ho * wo * ca8;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ta; i += (long)gridDim.x * blockDim.x) {
-        const int ch = (int)(i % ca);
-        const long q = i / ca;
+        const int j = (int)(i % ca8);
+        const long q = i / ca8;  // coarse pixel
         const int xo = (int)(q % wo);
         const long r = q / wo;
         const int yo = (int)(r % ho), b = (int)(r / ho);
-        const long base = (((long)b * h + 2 * yo) * w + 2 * xo) * c + ch;
-        const float s = bf2f(dcat[base]) + bf2f(dcat[base + c]) + bf2f(dcat[base + (long)w * c]) +
-                        bf2f(dcat[base + (long)w * c + c]);
-        da[i] = f2bf(s);
+        const long base = (((long)b * h + 2 * yo) * w + 2 * xo) * c + 8 * j;
+        const uint4 v0 = ld16(dcat + base), v1 = ld16(dcat + base + c);
+        const uint4 v2 = ld16(dcat + base + (long)w * c), v3 = ld16(dcat + base + (long)w * c + c);
+        const uint32_t* p0 = reinterpret_cast<const uint32_t*>(&v0);
+        const uint32_t* p1 = reinterpret_cast<const uint32_t*>(&v1);
+        const uint32_t* p2 = reinterpret_cast<const uint32_t*>(&v2);
+        const uint32_t* p3 = reinterpret_cast<const uint32_t*>(&v3);
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float lo = bf2f((u16)p0[k]) + bf2f((u16)p1[k]) + bf2f((u16)p2[k]) + bf2f((u16)p3[k]);
+            const float hi = bf2f((u16)(p0[k] >> 16)) + bf2f((u16)(p1[k] >> 16)) + bf2f((u16)(p2[k] >> 16)) +
+                             bf2f((u16)(p3[k] >> 16));
+            o[k] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+        }
+        *reinterpret_cast<uint4*>(da + q * ca + 8 * j) = make_uint4(o[0], o[1], o[2], o[3]);
     }
-    const long ts = (long)n * h * w * cb;
+    const long ts = (long)n * h * w * cb8;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ts; i += (long)gridDim.x * blockDim.x) {
-        const int ch = (int)(i % cb);
-        const long p = i / cb;
-        dskip[i] = dcat[p * c + ca + ch];
+        const int j = (int)(i % cb8);
+        const long q = i / cb8;
+        *reinterpret_cast<uint4*>(dskip + q * cb + 8 * j) = ld16(dcat + q * c + ca + 8 * j);
     }
 }
 
@@ -760,15 +774,15 @@ void UNet::forward(int n, int h, int w)
                        64);
     conv_fwd(6, act[P2].d, act[E3A].d, n, h3, w3);
     conv_fwd(7, act[E3A].d, act[E3B].d, n, h3, w3);
-    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p2 * 192)), dim3(256), 0, stream, act[E3B].d, act[E2B].d,
+    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p2 * 192 / 8)), dim3(256), 0, stream, act[E3B].d, act[E2B].d,
                        act[CAT2].d, n, h2, w2, 128, 64);
     conv_fwd(8, act[CAT2].d, act[D2A].d, n, h2, w2);
     conv_fwd(9, act[D2A].d, act[D2B].d, n, h2, w2);
-    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p1 * 96)), dim3(256), 0, stream, act[D2B].d, act[E1B].d,
+    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p1 * 96 / 8)), dim3(256), 0, stream, act[D2B].d, act[E1B].d,
                        act[CAT1].d, n, h1, w1, 64, 32);
     conv_fwd(10, act[CAT1].d, act[D1A].d, n, h1, w1);
     conv_fwd(11, act[D1A].d, act[D1B].d, n, h1, w1);
-    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p0 * 48)), dim3(256), 0, stream, act[D1B].d, act[E0B].d,
+    hipLaunchKernelGGL(k_up_concat, dim3(grid_for(p0 * 48 / 8)), dim3(256), 0, stream, act[D1B].d, act[E0B].d,
                        act[CAT0].d, n, h, w, 32, 16);
     conv_fwd(12, act[CAT0].d, act[D0A].d, n, h, w);
     conv_fwd(13, act[D0A].d, act[D0B].d, n, h, w);
