@@ -195,8 +195,8 @@ double stream_copy_probe(long n, int reps);   // GB/s of a plain copy of n doubl
 // CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep.
 // Returns the number of rrn partials; op_finish_rrn closes the reduction.
 // Output rows per wave of a streaming pass with L levels over tiles_x
-// column tiles, for `slots` resident waves.
-int tb_rows(int ny, int tiles_x, int L, int slots);
+// column tiles, for `slots` resident waves and blocks of `unroll` rounds.
+int tb_rows(int ny, int tiles_x, int L, int slots, int unroll = 6);
 // Level counts of the passes of a temporal-blocked PPCG outer iteration
 // with the given inner steps (each a compiled pass size).
 std::vector<int> tb_passes(int inner_steps);

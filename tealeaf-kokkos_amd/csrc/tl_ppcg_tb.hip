@@ -11,9 +11,9 @@ namespace tl {
 
 // Output rows per wave.  Every wave holds a whole SIMD (one resident wave
 // each), so the pass takes ceil(waves / simds) turns of the slowest SIMD, each
-// of (rows + 2L) rounds rounded up to the six-round unroll; minimise that,
+// of (rows + 2L) rounds rounded up to the unrolled block; minimise that,
 // and among equal costs take the fewest waves (least recomputed overlap).
-int tb_rows(int ny, int tiles_x, int L, int simds)
+int tb_rows(int ny, int tiles_x, int L, int simds, int unroll)
 {
     int best_rows = 16;
     long best = -1;
@@ -21,7 +21,7 @@ int tb_rows(int ny, int tiles_x, int L, int simds)
         const long tiles_y = (ny + rows - 1) / rows;
         if (rows > 16 && (ny + rows - 2) / (rows - 1) == tiles_y) continue;  // same tiling as rows - 1
         const long waves = (long)tiles_x * tiles_y;
-        const long rounds = ((long)std::min(rows, ny) + 2 * L + 5) / 6 * 6;
+        const long rounds = ((long)std::min(rows, ny) + 2 * L + unroll - 1) / unroll * unroll;
         const long cost = (waves + simds - 1) / simds * rounds;
         if (best < 0 || cost < best || (cost == best && tiles_y < (ny + best_rows - 1) / best_rows)) {
             best = cost;
@@ -87,7 +87,7 @@ void tb_coefs(Chunk& c, int inner_steps, const std::vector<int>& passes)
 const uint2* tb_rowtab(Chunk& c)
 {
     if (!c.d_rowtab) {
-        const int count = c.g.ny + 36;
+        const int count = c.g.ny + 48;
         std::vector<unsigned> h(2 * (size_t)count);
         for (int k = 0; k < count; ++k) {
             const int y = TB_ROWTAB0 + k;
@@ -132,9 +132,9 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.rowtab = tb_rowtab(c);
         a.row0 = TB_ROWTAB0;
         a.partials = c.partials;
-        const int ow = 2 * WAVE - 2 * L;
+        const int ow = 2 * WAVE - 2 * tb::tb_margin(L);
         a.tiles_x = (c.g.nx + ow - 1) / ow;
-        a.rows = tb_rows(c.g.ny, a.tiles_x, L, c.simds);
+        a.rows = tb_rows(c.g.ny, a.tiles_x, L, c.simds, tb::TB_UNROLL);
         const long tiles_y = (c.g.ny + a.rows - 1) / a.rows;
         a.nblocks = (long)a.tiles_x * tiles_y;
         if (a.nblocks > c.partials_cap) throw Error("temporal-blocked PPCG: more waves than reduction partials");

@@ -11,7 +11,8 @@
 // window of sd and a two-row window of r; level s at row y - s is updated as
 // soon as level s - 1 has produced row y - s + 1.  x-neighbours come from
 // the adjacent lanes through DPP row shifts; each level therefore consumes
-// one halo column per side, so a 64-lane wave writes 64 - 2L columns.
+// one halo column per side, so a 64-lane wave of two columns per lane
+// writes the 128 - 2L' columns in its middle (L' = L rounded up to even).
 //
 // Every cell value is produced by the reference's expressions in the
 // reference's order (calc_ur, then init_sd, then step after step), so u and
@@ -83,16 +84,26 @@ __device__ __forceinline__ double from_east(double v)
 }
 
 // Per-wave state of the streaming pass.  Each lane owns two adjacent
-// columns (c0 = X0 - L + 2 lane, c1 = c0 + 1), so a wave spans 128 columns
-// and writes the 128 - 2L in its middle.  Register windows rotate by round
-// number instead of moving data: within a block of six rounds the slot of
-// every window is a compile-time constant.  The windows that span L+1 rows
-// (u being accumulated, kx, ky) live in LDS rings indexed by row.
+// columns (c0 = X0 - L' + 2 lane, c1 = c0 + 1), so a wave spans 128 columns
+// and writes the 128 - 2L' in its middle.  Register windows rotate by round
+// number instead of moving data: within an unrolled block of rounds the
+// slot of every window is a compile-time constant.  The windows that span
+// L+1 rows (u being accumulated, kx, ky) live in LDS rings indexed by row.
+// One prefetched row: r, w (first pass) or sd, u, p and ky as the aligned
+// pair that holds the lane's cells, kx at the lane's two faces.
+struct TBRow {
+    double2 r, b, u, p, ky;
+    double kx0, kx1;
+};
+
+// Rows in flight ahead of the row being consumed (divides the unroll).
+constexpr int TB_PF = 2;
+
 template <int L>
 struct TBState {
     double sdw[L][3][2];  // sd of level s, slot (round - ylo) % 3, column
     double rw[L][2][2];   // r of level s, slot (round - ylo) % 2, column
-    double pf[2][6][2];   // prefetched rows: r, w|sd, u, p, kx, ky
+    TBRow pf[TB_PF];      // prefetched rows
     double acc;
     double ca[L], cb[L];  // step coefficients (wave-uniform)
 };
@@ -100,17 +111,19 @@ struct TBState {
 struct TBCtx {
     Geom g;
     int nx, ny, ylo, Y0, yhi;
-    // byte offsets within a row: the lane's cells and kx faces reflected
-    // into the mesh, store targets (the cell for lanes that own one, else a
-    // pad column past the halo that nothing reads) and the pad column alone
+    // byte offsets within a row: the aligned column pair that holds the
+    // lane's two cells reflected into the mesh (hi0, hi1: a cell is the
+    // pair's second element), the lane's kx faces reflected into the mesh,
+    // the store target (the lane's own pair for lanes that own a cell, else
+    // a pad pair past the halo that nothing reads) and the pad pair alone
     // (rows that own no output)
-    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
-    bool out0, out1, out_any;
+    unsigned bcp, bf0, bf1, bsp, bpad;
+    bool hi0, hi1;
+    bool out0, out1;
     double alpha, theta;
-    double* ur;  // LDS rings [L+1][64][2]
+    double* ur;  // LDS rings [ring rows][64][2]
     double* kxr;
     double* kyr;
-    double* kxer;  // LDS ring [L+1][64]: kx at c1 + 1 (the east lane's c0)
 };
 
 __device__ __forceinline__ double ld(const double* row, unsigned boff)
@@ -122,6 +135,18 @@ __device__ __forceinline__ void st(double* row, unsigned boff, double v)
 {
     *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + boff) = v;
 }
+
+__device__ __forceinline__ double2 ldp(const double* row, unsigned boff)
+{
+    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(row) + boff);
+}
+
+__device__ __forceinline__ void stp(double* row, unsigned boff, double2 v)
+{
+    *reinterpret_cast<double2*>(reinterpret_cast<char*>(row) + boff) = v;
+}
+
+__device__ __forceinline__ double pick(double2 v, bool hi) { return hi ? v.y : v.x; }
 
 // The reflective boundary as data: a cell index outside [HALO, n + 1]
 // reads the cell it mirrors, a face index outside [HALO, n + 2] the face it
@@ -151,35 +176,52 @@ __host__ __device__ inline int fold_face(int f, int n)
     return HALO + t;
 }
 
+// The 16-byte aligned column pair that holds the reflections of cells k0
+// and k0 + 1 (k0 even): reflection maps an aligned pair onto an aligned
+// pair, reversed past a face, or onto one cell where an odd n folds
+// (n + 1 and n + 2 both read n + 1).  hi0, hi1: the cell is the pair's
+// second element.
+struct CellPair {
+    int start;
+    bool hi0, hi1;
+};
+
+__host__ __device__ inline CellPair cell_pair(int k0, int n)
+{
+    const int f0 = fold_cell(k0, n), f1 = fold_cell(k0 + 1, n);
+    const int s = (f0 < f1 ? f0 : f1) & ~1;
+    return CellPair{s, f0 != s, f1 != s};
+}
+
+// Column margin of a wave for L levels, rounded up to even so that every
+// lane's first column is even (HALO and the wave's output width are even).
+__host__ __device__ constexpr int tb_margin(int L) { return L + (L & 1); }
+
 // Loads of row y into a prefetch slot.  Every load is issued on every
 // round, so that the number of memory operations in flight is the same on
 // every path and the compiler can wait for exactly the oldest slot instead
 // of draining all.  Rows outside the mesh read the rows they mirror (cells
 // for r, w|sd, u, p, kx; faces for ky), from a table of reflected row
-// offsets (one scalar load per round).  An address is the array's base plus
-// a per-lane 32-bit byte offset of row and column (every field is under 4 GiB).
+// offsets (one scalar load per round).  Cell columns come as the lane's
+// aligned pair (one 16-byte load per array), the two kx faces one by one
+// (reflected faces do not map aligned pairs onto aligned pairs); the pair's
+// elements are assigned to the lane's cells where the row is consumed, so
+// that no load is waited for early.  An address is the array's base plus a
+// per-lane 32-bit byte offset of row and column (every field is under 4 GiB).
+// A row's entry of the reflected-row table, read through the constant
+// address space: a scalar load.
+typedef const __attribute__((address_space(4))) unsigned ConstU;
+
 template <bool FIRST>
-__device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, int y, double (&v)[6][2])
+__device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, unsigned oy, unsigned of, TBRow& v)
 {
-    // a uniform read through the constant address space: a scalar load
-    typedef const __attribute__((address_space(4))) unsigned ConstU;
-    ConstU* t = (ConstU*)(const unsigned*)A.rowtab + 2 * (y - A.row0);
-    const unsigned oy = t[0], of = t[1];
-    v[0][0] = ld(A.r_in, oy + c.bc0);
-    v[0][1] = ld(A.r_in, oy + c.bc1);
-    const double* b = FIRST ? A.w : A.sd_in;
-    v[1][0] = ld(b, oy + c.bc0);
-    v[1][1] = ld(b, oy + c.bc1);
-    v[2][0] = ld(A.u, oy + c.bc0);
-    v[2][1] = ld(A.u, oy + c.bc1);
-    if (FIRST) {
-        v[3][0] = ld(A.p, oy + c.bc0);
-        v[3][1] = ld(A.p, oy + c.bc1);
-    }
-    v[4][0] = ld(A.kx, oy + c.bf0);
-    v[4][1] = ld(A.kx, oy + c.bf1);
-    v[5][0] = ld(A.ky, of + c.bc0);
-    v[5][1] = ld(A.ky, of + c.bc1);
+    v.r = ldp(A.r_in, oy + c.bcp);
+    v.b = ldp(FIRST ? A.w : A.sd_in, oy + c.bcp);
+    v.u = ldp(A.u, oy + c.bcp);
+    if (FIRST) v.p = ldp(A.p, oy + c.bcp);
+    v.kx0 = ld(A.kx, oy + c.bf0);
+    v.kx1 = ld(A.kx, oy + c.bf1);
+    v.ky = ldp(A.ky, of + c.bcp);
 }
 
 __device__ __forceinline__ int ring(int row, int n)
@@ -188,71 +230,86 @@ __device__ __forceinline__ int ring(int row, int n)
     return m < 0 ? m + n : m;
 }
 
-// One round: level 0 at row y, then level s at row y - s for s = 1..L.  The LDS rings of
-// level s + 1 are read before level s computes, so their latency overlaps
-// level s's arithmetic (the rows differ, so no write of level s is missed).
+// Rounds per unrolled block: a multiple of the register windows' periods
+// (3 rows of sd, 2 of r, TB_PF prefetch slots) and of the LDS ring length,
+// so that every window slot and ring row is a compile-time constant.
+constexpr int TB_UNROLL = 12;
+
+// LDS ring length for L levels: rows y - L .. y are live in a round; the
+// smallest divisor of the unroll that holds them.
+template <int L>
+struct TBRing {
+    static constexpr int N = L + 1 <= 2 ? 2 : L + 1 <= 3 ? 3 : L + 1 <= 4 ? 4 : L + 1 <= 6 ? 6 : 12;
+    static_assert(L + 1 <= 12, "pass size beyond the LDS ring");
+    // ring row of the round PH rounds into a block, minus s
+    static constexpr int row(int PH, int s) { return ((PH - s) % N + N) % N; }
+};
+
+// One round: level 0 at row y, then level s at row y - s for s = 1..L.  The
+// LDS rings of level s + 1 are read before level s computes, so their
+// latency overlaps level s's arithmetic (the rows differ, so no write of
+// level s is missed).  `tab` is the row table entry of row y + TB_PF - PH.
 template <int L, bool FIRST, bool LAST, int PH>
-__device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y)
+__device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y, ConstU* tab)
 {
+    using R = TBRing<L>;
     constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
     constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
     const int lane = threadIdx.x;
-    double(&v)[6][2] = st.pf[PH % 2];
-    double2* ur = reinterpret_cast<double2*>(c.ur);
-    double2* kxr = reinterpret_cast<double2*>(c.kxr);
-    double2* kyr = reinterpret_cast<double2*>(c.kyr);
+    TBRow& v = st.pf[PH % TB_PF];
+    double2* ur = reinterpret_cast<double2*>(c.ur) + lane;
+    double2* kxr = reinterpret_cast<double2*>(c.kxr) + lane;
+    double2* kyr = reinterpret_cast<double2*>(c.kyr) + lane;
+    // kx at the east lane's first face: the next element of the ring row
+    const double* kxe = c.kxr + 2 * lane + 2;
 
     // inputs of level 1 (row y - 1)
-    int rn = ring(y - 1, L + 1);
-    double2 nkx = kxr[rn * WAVE + lane], nky = kyr[rn * WAVE + lane], nu = ur[rn * WAVE + lane];
-    double nkxe = c.kxer[rn * WAVE + lane];
+    double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE], nu = ur[R::row(PH, 1) * WAVE];
+    double nkxe = kxe[2 * WAVE * R::row(PH, 1)];
 
-    // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd
-    const int ry = ring(y, L + 1);
+    // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd,
+    // evaluated on the loaded pair (elementwise), then assigned to the cells
     double kyn0, kyn1;
     {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            double r0, sd0;
-            if (FIRST) {
-                r0 = v[0][q] - c.alpha * v[1][q];
-                sd0 = r0 / c.theta;
-            } else {
-                r0 = v[0][q];
-                sd0 = v[1][q];
-            }
-            st.sdw[0][S0][q] = sd0;
-            st.rw[0][R0][q] = r0;
+        double2 r0 = v.r, sd0 = v.b;
+        if (FIRST) {
+            r0.x = v.r.x - c.alpha * v.b.x;
+            r0.y = v.r.y - c.alpha * v.b.y;
+            sd0.x = r0.x / c.theta;
+            sd0.y = r0.y / c.theta;
         }
-        double2 u0 = make_double2(0.0, 0.0);
-        if (y >= c.Y0 && y <= c.yhi) {
-            u0.x = FIRST ? v[2][0] + c.alpha * v[3][0] : v[2][0];
-            u0.y = FIRST ? v[2][1] + c.alpha * v[3][1] : v[2][1];
+        st.sdw[0][S0][0] = pick(sd0, c.hi0);
+        st.sdw[0][S0][1] = pick(sd0, c.hi1);
+        st.rw[0][R0][0] = pick(r0, c.hi0);
+        st.rw[0][R0][1] = pick(r0, c.hi1);
+        double2 uu = v.u;
+        if (FIRST) {
+            uu.x = v.u.x + c.alpha * v.p.x;
+            uu.y = v.u.y + c.alpha * v.p.y;
         }
-        ur[ry * WAVE + lane] = u0;
-        kxr[ry * WAVE + lane] = make_double2(v[4][0], v[4][1]);
-        c.kxer[ry * WAVE + lane] = from_east(v[4][0]);
-        kyr[ry * WAVE + lane] = make_double2(v[5][0], v[5][1]);
-        kyn0 = v[5][0];
-        kyn1 = v[5][1];
+        // rows outside the strip accumulate nothing that is stored
+        const bool yin = y >= c.Y0 && y <= c.yhi;
+        ur[R::row(PH, 0) * WAVE] = make_double2(yin ? pick(uu, c.hi0) : 0.0, yin ? pick(uu, c.hi1) : 0.0);
+        kxr[R::row(PH, 0) * WAVE] = make_double2(v.kx0, v.kx1);
+        kyn0 = pick(v.ky, c.hi0);
+        kyn1 = pick(v.ky, c.hi1);
+        kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
     }
-    // prefetch row y + 2 into the slot just consumed
-    tb_load<FIRST>(A, c, y + 2, v);
+    // prefetch row y + TB_PF into the slot just consumed
+    tb_load<FIRST>(A, c, tab[2 * PH], tab[2 * PH + 1], v);
 
 #pragma unroll
     for (int s = 1; s <= L; ++s) {
         const int z = y - s;
-        const int rz = rn;
         const double2 kx = nkx, ky = nky;
         const double kxe1 = nkxe;
         double2 u = nu;
         const double a = st.ca[s - 1], b = st.cb[s - 1];
         if (s < L) {
-            rn = ring(z - 1, L + 1);
-            nkx = kxr[rn * WAVE + lane];
-            nky = kyr[rn * WAVE + lane];
-            nu = ur[rn * WAVE + lane];
-            nkxe = c.kxer[rn * WAVE + lane];
+            nkx = kxr[R::row(PH, s + 1) * WAVE];
+            nky = kyr[R::row(PH, s + 1) * WAVE];
+            nu = ur[R::row(PH, s + 1) * WAVE];
+            nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
         }
         double rn0, rn1;
         {
@@ -278,30 +335,38 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                 st.sdw[s][S0][1] = a * c1 + b * rn1;
                 st.rw[s][R0][0] = rn0;
                 st.rw[s][R0][1] = rn1;
-                ur[rz * WAVE + lane] = u;
+                ur[R::row(PH, s) * WAVE] = u;
             }
         }
         if (s == L) {
             // output row: every lane stores, lanes and rows that own no
-            // cell into a pad column, so the store count never depends on a branch
+            // cell into a pad pair, so the store count never depends on a branch
             const bool zin = z >= c.Y0 && z <= c.yhi;
             const bool o0 = zin && c.out0, o1 = zin && c.out1;
-            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            const unsigned bz = zin ? c.bsp : c.bpad;
             const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
-            tb::st(A.r_out, oz + b0, rn0);
-            tb::st(A.u, oz + b0, u.x);
-            tb::st(A.r_out, oz + b1, rn1);
-            tb::st(A.u, oz + b1, u.y);
-            if (!LAST) {
-                tb::st(A.sd_out, oz + b0, a * c0 + b * rn0);
-                tb::st(A.sd_out, oz + b1, a * c1 + b * rn1);
-            }
+            stp(A.r_out, oz + bz, make_double2(rn0, rn1));
+            stp(A.u, oz + bz, u);
+            if (!LAST) stp(A.sd_out, oz + bz, make_double2(a * c0 + b * rn0, a * c1 + b * rn1));
             if (LAST) {
                 st.acc += o0 ? rn0 * rn0 : 0.0;
                 st.acc += o1 ? rn1 * rn1 : 0.0;
             }
         }
+    }
+}
+
+template <int L, bool FIRST, bool LAST, int PH>
+__device__ __forceinline__ void tb_rounds(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y, ConstU* tab)
+{
+    if constexpr (PH < TB_UNROLL) {
+        tb_round<L, FIRST, LAST, PH>(A, c, st, y + PH, tab);
+        // the scheduler may not move work between rounds (interleaving them
+        // raises the register count, and the copies it then makes of
+        // registers with loads in flight wait for those loads)
+        __builtin_amdgcn_sched_barrier(0);
+        tb_rounds<L, FIRST, LAST, PH + 1>(A, c, st, y, tab);
     }
 }
 
@@ -318,8 +383,10 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
         }
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        st.pf[0][k][0] = st.pf[0][k][1] = st.pf[1][k][0] = st.pf[1][k][1] = 0.0;
+    for (int k = 0; k < TB_PF; ++k) {
+        const double2 z = make_double2(0.0, 0.0);
+        st.pf[k].r = st.pf[k].b = st.pf[k].u = st.pf[k].p = st.pf[k].ky = z;
+        st.pf[k].kx0 = st.pf[k].kx1 = 0.0;
     }
     st.acc = 0.0;
 #pragma unroll
@@ -327,29 +394,18 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
         st.ca[s] = uniform(A.coef[s]);
         st.cb[s] = uniform(A.coef[L + s]);
     }
-    tb_load<FIRST>(A, c, c.ylo, st.pf[0]);
-    tb_load<FIRST>(A, c, c.ylo + 1, st.pf[1]);
-    // wait for the first two rows here: with loads still in flight on entry
+    ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
+#pragma unroll
+    for (int k = 0; k < TB_PF; ++k) tb_load<FIRST>(A, c, tab[2 * k], tab[2 * k + 1], st.pf[k]);
+    // wait for the first rows here: with loads still in flight on entry
     // the compiler's wait at the loop head has to cover the entry path too,
     // and it then drains all loads at the head of every block of rounds
     __builtin_amdgcn_s_waitcnt(VMCNT0);
-    // rounds run in blocks of six; the scheduler may not move work between
-    // rounds (interleaving them raises the register count, and the copies
-    // it then makes of registers with loads in flight wait for those loads)
+    // rows ylo + 12 k ... start each block, so that the LDS ring rows and
+    // register window slots of a round depend only on its place in the block
     const int yend = c.yhi + L;
-    for (int y = c.ylo; y <= yend; y += 6) {
-        tb_round<L, FIRST, LAST, 0>(A, c, st, y);
-        __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, 1>(A, c, st, y + 1);
-        __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, 2>(A, c, st, y + 2);
-        __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, 3>(A, c, st, y + 3);
-        __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, 4>(A, c, st, y + 4);
-        __builtin_amdgcn_sched_barrier(0);
-        tb_round<L, FIRST, LAST, 5>(A, c, st, y + 5);
-        __builtin_amdgcn_sched_barrier(0);
+    for (int y = c.ylo; y <= yend; y += TB_UNROLL) {
+        tb_rounds<L, FIRST, LAST, 0>(A, c, st, y, tab + 2 * (y - c.ylo + TB_PF));
     }
     if (LAST) {
         double v[1] = {st.acc};
@@ -360,34 +416,46 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
 template <int L, bool FIRST, bool LAST>
 __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
 {
-    __shared__ double2 u_ring[(L + 1) * WAVE];
-    __shared__ double2 kx_ring[(L + 1) * WAVE];
-    __shared__ double2 ky_ring[(L + 1) * WAVE];
-    __shared__ double kxe_ring[(L + 1) * WAVE];
+    using R = TBRing<L>;
+    __shared__ double2 u_ring[R::N * WAVE];
+    // one more element: the last lane's east face reads past the last row
+    // (that lane owns no cell, so the value is never used)
+    __shared__ double2 kx_ring[R::N * WAVE + 1];
+    __shared__ double2 ky_ring[R::N * WAVE];
     const int lane = threadIdx.x;
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
-    constexpr int OW = 2 * WAVE - 2 * L;
+    constexpr int LP = tb_margin(L);
+    constexpr int OW = 2 * WAVE - 2 * LP;
     TBCtx c;
     c.g = g;
     c.nx = g.nx;
     c.ny = g.ny;
     const int X0 = HALO + tx * OW;
     c.Y0 = HALO + ty * A.rows;
-    const int k0 = X0 - L + 2 * lane, k1 = k0 + 1;
-    c.bc0 = 8u * (unsigned)fold_cell(k0, c.nx);
-    c.bc1 = 8u * (unsigned)fold_cell(k1, c.nx);
+    const int k0 = X0 - LP + 2 * lane, k1 = k0 + 1;
+    {
+        const CellPair cp = cell_pair(k0, c.nx);
+        c.bcp = 8u * (unsigned)cp.start;
+        c.hi0 = cp.hi0;
+        c.hi1 = cp.hi1;
+    }
     c.bf0 = 8u * (unsigned)fold_face(k0, c.nx);
     c.bf1 = 8u * (unsigned)fold_face(k1, c.nx);
+    // k0 and the tile's bounds are even, so a lane owns both of its cells
+    // or neither, except where an odd nx ends the mesh at k0
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= c.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= c.nx + 1;
-    c.out_any = c.out0 || c.out1;
     {
-        // pad columns X .. sx-1 of every row are never read
-        const int npad = (int)(g.sx - g.X);
-        c.bpad = 8u * (unsigned)(g.X + lane % npad);
-        c.bs0 = c.out0 ? c.bc0 : c.bpad;
-        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+        // pad columns X .. sx-1 of every row are never read (at least 30 of
+        // them).  A lane whose first cell is the last of an odd nx also
+        // writes the halo column nx + 2 past it, with the mirror image of
+        // that cell: the value a reflective halo update puts there, and
+        // no kernel reads it (neighbours across a face are read mirrored).
+        const int xe = (int)g.X + ((int)g.X & 1);
+        const int npair = (int)(g.sx - xe) / 2;
+        c.bpad = 8u * (unsigned)(xe + 2 * (lane % npair));
+        c.bsp = c.out0 ? 8u * (unsigned)k0 : c.bpad;
     }
     c.ylo = c.Y0 - L;
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
@@ -396,7 +464,6 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.ur = reinterpret_cast<double*>(u_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
-    c.kxer = kxe_ring;
     tb_wave<L, FIRST, LAST>(A, c);
 }
 
