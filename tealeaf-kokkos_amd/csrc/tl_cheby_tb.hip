@@ -45,9 +45,9 @@ long op_cheby_tb(Chunk& c, int step1, int count, bool norm)
         a.rowtab = tb_rowtab(c);
         a.row0 = TB_ROWTAB0;
         a.partials = c.partials;
-        const int ow = 2 * WAVE - 2 * K;
+        const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
         a.tiles_x = (c.g.nx + ow - 1) / ow;
-        a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds);
+        a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds, tb::TB_UNROLL);
         a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
         if (a.nblocks > c.partials_cap) throw Error("temporal-blocked Chebyshev: more waves than reduction partials");
         TL_TIMED(c, K_CHEBY_TB, {

@@ -64,9 +64,9 @@ std::vector<double> op_jacobi_tb(Chunk& c, int K)
     a.rowtab = tb_rowtab(c);
     a.row0 = TB_ROWTAB0;
     a.partials = c.partials;
-    const int ow = 2 * WAVE - 2 * K;
+    const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
     a.tiles_x = (c.g.nx + ow - 1) / ow;
-    a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds);
+    a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds, tb::TB_UNROLL);
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
     if (a.nblocks * K > 4 * c.partials_cap) throw Error("temporal-blocked Jacobi: more waves than reduction partials");
     TL_TIMED(c, K_JACOBI_TB, {

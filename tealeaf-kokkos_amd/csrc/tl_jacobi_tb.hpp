@@ -22,12 +22,14 @@ namespace tl {
 #if TL_DIM == 2
 namespace jtb {
 
-using tb::fold_cell;
+using tb::ConstU;
 using tb::fold_face;
 using tb::from_east;
 using tb::from_west;
 using tb::ld;
-using tb::ring;
+using tb::ldp;
+using tb::pick;
+using tb::stp;
 
 struct JTBArgs {
     const double* u_in;
@@ -44,66 +46,68 @@ struct JTBArgs {
     long nblocks;
 };
 
+// One prefetched row: u, u0 and ky as the aligned pair that holds the
+// lane's cells, kx at the lane's two faces.
+struct JTBRow {
+    double2 u, u0, ky;
+    double kx0, kx1;
+};
+
 template <int K>
 struct JTBState {
     double uw[K][3][2];  // u after iteration s, slot (round) % 3, column
-    double pf[2][4][2];  // prefetched rows: u, u0, kx, ky
+    JTBRow pf[2];        // prefetched rows
     double err[K];       // sum |u_s - u_(s-1)| over the wave's cells
 };
 
 struct JTBCtx {
     Geom g;
     int ylo, Y0, yhi;
-    unsigned bc0, bc1, bf0, bf1, bs0, bs1, bpad;
+    // as in the PPCG kernel: the lane's aligned cell pair (hi0, hi1: a cell
+    // is its second element), kx faces, store pair and pad pair
+    unsigned bcp, bf0, bf1, bsp, bpad;
+    bool hi0, hi1;
     bool out0, out1;
-    double2* u0r;  // LDS rings [K+1][64]
-    double2* kxr;
-    double2* kyr;
-    double* kxer;  // kx at c1 + 1 (the east lane's c0)
+    double* u0r;  // LDS rings [ring rows][64][2]
+    double* kxr;
+    double* kyr;
 };
 
-__device__ __forceinline__ void stv(double* a, unsigned boff, double v)
+__device__ __forceinline__ void jtb_load(const JTBArgs& A, const JTBCtx& c, unsigned oy, unsigned of, JTBRow& v)
 {
-    *reinterpret_cast<double*>(reinterpret_cast<char*>(a) + boff) = v;
-}
-
-__device__ __forceinline__ void jtb_load(const JTBArgs& A, const JTBCtx& c, int y, double (&v)[4][2])
-{
-    typedef const __attribute__((address_space(4))) unsigned ConstU;
-    ConstU* t = (ConstU*)(const unsigned*)A.rowtab + 2 * (y - A.row0);
-    const unsigned oy = t[0], of = t[1];
-    v[0][0] = ld(A.u_in, oy + c.bc0);
-    v[0][1] = ld(A.u_in, oy + c.bc1);
-    v[1][0] = ld(A.u0, oy + c.bc0);
-    v[1][1] = ld(A.u0, oy + c.bc1);
-    v[2][0] = ld(A.kx, oy + c.bf0);
-    v[2][1] = ld(A.kx, oy + c.bf1);
-    v[3][0] = ld(A.ky, of + c.bc0);
-    v[3][1] = ld(A.ky, of + c.bc1);
+    v.u = ldp(A.u_in, oy + c.bcp);
+    v.u0 = ldp(A.u0, oy + c.bcp);
+    v.kx0 = ld(A.kx, oy + c.bf0);
+    v.kx1 = ld(A.kx, oy + c.bf1);
+    v.ky = ldp(A.ky, of + c.bcp);
 }
 
 // One round: the loaded row y enters as iteration 0's u, iteration s updates
-// row y - s (s = 1..K); row y - K leaves the pass.
+// row y - s (s = 1..K); row y - K leaves the pass.  `tab` is the row table
+// entry of row y + 2 - PH.
 template <int K, int PH>
-__device__ __forceinline__ void jtb_round(const JTBArgs& A, const JTBCtx& c, JTBState<K>& st, int y)
+__device__ __forceinline__ void jtb_round(const JTBArgs& A, const JTBCtx& c, JTBState<K>& st, int y, ConstU* tab)
 {
+    using R = tb::TBRing<K>;
     constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
     const int lane = threadIdx.x;
-    double(&v)[4][2] = st.pf[PH % 2];
+    JTBRow& v = st.pf[PH % 2];
+    double2* u0r = reinterpret_cast<double2*>(c.u0r) + lane;
+    double2* kxr = reinterpret_cast<double2*>(c.kxr) + lane;
+    double2* kyr = reinterpret_cast<double2*>(c.kyr) + lane;
+    // kx at the east lane's first face: the next element of the ring row
+    const double* kxe = c.kxr + 2 * lane + 2;
 
-    int rn = ring(y - 1, K + 1);
-    double2 nkx = c.kxr[rn * WAVE + lane], nky = c.kyr[rn * WAVE + lane], nu0 = c.u0r[rn * WAVE + lane];
-    double nkxe = c.kxer[rn * WAVE + lane];
+    double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE], nu0 = u0r[R::row(PH, 1) * WAVE];
+    double nkxe = kxe[2 * WAVE * R::row(PH, 1)];
 
-    const int ry = ring(y, K + 1);
-    st.uw[0][S0][0] = v[0][0];
-    st.uw[0][S0][1] = v[0][1];
-    c.u0r[ry * WAVE + lane] = make_double2(v[1][0], v[1][1]);
-    c.kxr[ry * WAVE + lane] = make_double2(v[2][0], v[2][1]);
-    c.kxer[ry * WAVE + lane] = from_east(v[2][0]);
-    c.kyr[ry * WAVE + lane] = make_double2(v[3][0], v[3][1]);
-    double kyn0 = v[3][0], kyn1 = v[3][1];
-    jtb_load(A, c, y + 2, v);
+    st.uw[0][S0][0] = pick(v.u, c.hi0);
+    st.uw[0][S0][1] = pick(v.u, c.hi1);
+    u0r[R::row(PH, 0) * WAVE] = make_double2(pick(v.u0, c.hi0), pick(v.u0, c.hi1));
+    kxr[R::row(PH, 0) * WAVE] = make_double2(v.kx0, v.kx1);
+    double kyn0 = pick(v.ky, c.hi0), kyn1 = pick(v.ky, c.hi1);
+    kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
+    jtb_load(A, c, tab[2 * PH], tab[2 * PH + 1], v);
 
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
@@ -111,11 +115,10 @@ __device__ __forceinline__ void jtb_round(const JTBArgs& A, const JTBCtx& c, JTB
         const double2 kx = nkx, ky = nky, u0 = nu0;
         const double kxe1 = nkxe;
         if (s < K) {
-            rn = ring(z - 1, K + 1);
-            nkx = c.kxr[rn * WAVE + lane];
-            nky = c.kyr[rn * WAVE + lane];
-            nu0 = c.u0r[rn * WAVE + lane];
-            nkxe = c.kxer[rn * WAVE + lane];
+            nkx = kxr[R::row(PH, s + 1) * WAVE];
+            nky = kyr[R::row(PH, s + 1) * WAVE];
+            nu0 = u0r[R::row(PH, s + 1) * WAVE];
+            nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
         }
         const double c0 = st.uw[s - 1][S1][0], c1 = st.uw[s - 1][S1][1];
         const double s0 = st.uw[s - 1][S2][0], s1 = st.uw[s - 1][S2][1];
@@ -136,14 +139,22 @@ __device__ __forceinline__ void jtb_round(const JTBArgs& A, const JTBCtx& c, JTB
             st.uw[s][S0][0] = un0;
             st.uw[s][S0][1] = un1;
         } else {
-            // every lane stores, lanes and rows that own no cell into a pad column
-            const unsigned b0 = zin ? c.bs0 : c.bpad, b1 = zin ? c.bs1 : c.bpad;
+            // every lane stores, lanes and rows that own no cell into a pad pair
+            const unsigned bz = zin ? c.bsp : c.bpad;
             const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
-            stv(A.u_out, oz + b0, un0);
-            stv(A.r, oz + b0, c0);
-            stv(A.u_out, oz + b1, un1);
-            stv(A.r, oz + b1, c1);
+            stp(A.u_out, oz + bz, make_double2(un0, un1));
+            stp(A.r, oz + bz, make_double2(c0, c1));
         }
+    }
+}
+
+template <int K, int PH>
+__device__ __forceinline__ void jtb_rounds(const JTBArgs& A, const JTBCtx& c, JTBState<K>& st, int y, ConstU* tab)
+{
+    if constexpr (PH < tb::TB_UNROLL) {
+        jtb_round<K, PH>(A, c, st, y + PH, tab);
+        __builtin_amdgcn_sched_barrier(0);
+        jtb_rounds<K, PH + 1>(A, c, st, y, tab);
     }
 }
 
@@ -157,64 +168,59 @@ __device__ __forceinline__ void jtb_wave(const JTBArgs& A, const JTBCtx& c)
         for (int q = 0; q < 2; ++q) st.uw[s][0][q] = st.uw[s][1][q] = st.uw[s][2][q] = 0.0;
         st.err[s] = 0.0;
     }
-    jtb_load(A, c, c.ylo, st.pf[0]);
-    jtb_load(A, c, c.ylo + 1, st.pf[1]);
+    ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
+    jtb_load(A, c, tab[0], tab[1], st.pf[0]);
+    jtb_load(A, c, tab[2], tab[3], st.pf[1]);
     // the first rows complete before the loop, so the loop head waits only
-    // for the oldest prefetch slot (see tl_ppcg_tb.hpp)
+    // for the oldest prefetch slot (as in the PPCG kernel)
     __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
     const int yend = c.yhi + K;
-    for (int y = c.ylo; y <= yend; y += 6) {
-        jtb_round<K, 0>(A, c, st, y);
-        __builtin_amdgcn_sched_barrier(0);
-        jtb_round<K, 1>(A, c, st, y + 1);
-        __builtin_amdgcn_sched_barrier(0);
-        jtb_round<K, 2>(A, c, st, y + 2);
-        __builtin_amdgcn_sched_barrier(0);
-        jtb_round<K, 3>(A, c, st, y + 3);
-        __builtin_amdgcn_sched_barrier(0);
-        jtb_round<K, 4>(A, c, st, y + 4);
-        __builtin_amdgcn_sched_barrier(0);
-        jtb_round<K, 5>(A, c, st, y + 5);
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL) jtb_rounds<K, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
     block_partial<K, 1>(st.err, Reduce{A.partials});
 }
 
 template <int K>
 __global__ __launch_bounds__(WAVE) void k_jacobi_tb(Geom g, JTBArgs A)
 {
-    __shared__ double2 u0_ring[(K + 1) * WAVE];
-    __shared__ double2 kx_ring[(K + 1) * WAVE];
-    __shared__ double2 ky_ring[(K + 1) * WAVE];
-    __shared__ double kxe_ring[(K + 1) * WAVE];
+    using R = tb::TBRing<K>;
+    __shared__ double2 u0_ring[R::N * WAVE];
+    // one more element: the last lane's east face reads past the last row
+    __shared__ double2 kx_ring[R::N * WAVE + 1];
+    __shared__ double2 ky_ring[R::N * WAVE];
     const int lane = threadIdx.x;
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
-    constexpr int OW = 2 * WAVE - 2 * K;
+    constexpr int KP = tb::tb_margin(K);
+    constexpr int OW = 2 * WAVE - 2 * KP;
     JTBCtx c;
     c.g = g;
     const int X0 = HALO + tx * OW;
     c.Y0 = HALO + ty * A.rows;
-    const int k0 = X0 - K + 2 * lane, k1 = k0 + 1;
-    c.bc0 = 8u * (unsigned)fold_cell(k0, g.nx);
-    c.bc1 = 8u * (unsigned)fold_cell(k1, g.nx);
+    const int k0 = X0 - KP + 2 * lane, k1 = k0 + 1;
+    {
+        const tb::CellPair cp = tb::cell_pair(k0, g.nx);
+        c.bcp = 8u * (unsigned)cp.start;
+        c.hi0 = cp.hi0;
+        c.hi1 = cp.hi1;
+    }
     c.bf0 = 8u * (unsigned)fold_face(k0, g.nx);
     c.bf1 = 8u * (unsigned)fold_face(k1, g.nx);
     c.out0 = k0 >= X0 && k0 < X0 + OW && k0 <= g.nx + 1;
     c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= g.nx + 1;
     {
-        // pad columns X .. sx-1 of every row are never read
-        const int npad = (int)(g.sx - g.X);
-        c.bpad = 8u * (unsigned)(g.X + lane % npad);
-        c.bs0 = c.out0 ? c.bc0 : c.bpad;
-        c.bs1 = c.out1 ? c.bc1 : c.bpad;
+        // pad pairs past the halo are never read; a lane whose first cell
+        // is the last of an odd nx also writes the halo column nx + 2 with
+        // that cell's mirror image (the PPCG kernel explains why)
+        const int xe = (int)g.X + ((int)g.X & 1);
+        const int npair = (int)(g.sx - xe) / 2;
+        c.bpad = 8u * (unsigned)(xe + 2 * (lane % npair));
+        c.bsp = c.out0 ? 8u * (unsigned)k0 : c.bpad;
     }
     c.ylo = c.Y0 - K;
     c.yhi = min(c.Y0 + A.rows - 1, g.ny + 1);
-    c.u0r = u0_ring;
-    c.kxr = kx_ring;
-    c.kyr = ky_ring;
-    c.kxer = kxe_ring;
+    c.u0r = reinterpret_cast<double*>(u0_ring);
+    c.kxr = reinterpret_cast<double*>(kx_ring);
+    c.kyr = reinterpret_cast<double*>(ky_ring);
     jtb_wave<K>(A, c);
 }
 
