@@ -124,6 +124,21 @@ void tl_set_progress(int on);
 int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
                         int nsteps, double* rrn);
 
+/* Chebyshev iterations step1 .. step1 + count - 1 (1-based) as the driver
+ * runs them (no counterpart in the reference; the ext_cheby_solver_iterate_
+ * calls of those steps, ext_core.cpp:501-515, in temporal-blocked passes of
+ * up to 10), on a chunk after ext_cheby_solver_init_: u, p and r as after
+ * those calls, ||r||^2 in *rrn.  2D, no preconditioner; otherwise returns -1
+ * with tl_last_error() set. */
+int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn);
+
+/* count Jacobi iterations as the driver runs them (no counterpart in the
+ * reference; count ext_jacobi_kernel_solve_ calls, ext_core.cpp:362-373, in
+ * temporal-blocked passes of up to 10): u and r as after those calls, the
+ * error of each iteration in errors[0..count).  2D; otherwise returns -1
+ * with tl_last_error() set. */
+int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
+
 /* Mixed-precision form of tl_ppcg_outer_fused (no counterpart in the
  * reference): the nsteps inner steps in FP32 on the FP32 rounding of the
  * residual, u and r updated in FP64 (u += alpha p + sum sd, r = r' + the FP32

@@ -292,6 +292,7 @@ void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double
 // overwritten while its neighbours still need them.
 void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, double theta,
                    bool preconditioner, bool fused);
+void op_cheby_upload(Chunk& c);  // stored coefficients to the device (temporal-blocked passes)
 void op_cheby_iterate(Chunk& c, int step1, bool fused);  // 1-based step
 
 void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, double theta);

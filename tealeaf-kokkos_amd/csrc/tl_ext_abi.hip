@@ -114,6 +114,42 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
     return rc;
 }
 
+// Chebyshev iterations step1 .. step1 + count - 1 on a chunk after
+// ext_cheby_solver_init_, as the driver runs them: temporal-blocked passes
+// of up to 10, the last writing r and the first phase of ||r||^2.  u, p and
+// r end as after the same ext_cheby_solver_iterate_ calls.
+int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = chunk_at(chunk);
+        op_cheby_upload(*c);
+        *rrn = op_finish_norm(*c, op_cheby_tb(*c, step1, count, true));
+        rc = 0;
+    });
+    return rc;
+}
+
+// count Jacobi iterations (JacobiCopyU + JacobiSolve each) as the driver
+// runs them: temporal-blocked passes of up to 10 with one error sum per
+// iteration, written to errors[0..count).  u and r end as after the same
+// ext_jacobi_kernel_solve_ calls.
+int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = chunk_at(chunk);
+        if (count < 1) throw Error("Jacobi needs at least one iteration");
+        int done = 0;
+        while (done < count) {
+            const std::vector<double> e = op_jacobi_tb(*c, jacobi_tb_pass(count - done));
+            for (double v : e) errors[done++] = v;
+        }
+        rc = 0;
+    });
+    return rc;
+}
+
 // The mixed-precision form of the same outer iteration: FP32 inner steps,
 // u and r updated in FP64 (2D only; nsteps with a compiled kernel).
 int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,

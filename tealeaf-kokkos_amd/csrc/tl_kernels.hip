@@ -1325,9 +1325,12 @@ void op_cg_calc_w_dev(Chunk& c, bool fold)
         PWStrip t;
         t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
         // strips of 64 rows (fewer strip starts, each re-reading two rows of
-        // p and r), shorter only while there would be under two waves per SIMD
+        // p and r), shorter only while there would be under two waves per
+        // SIMD, 128 where that still leaves four or more (8192^2: 705 against
+        // 720 us)
         t.rows = 64;
         while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
+        if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
         t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         TIMED(c, K_CG_WP, {
@@ -1435,9 +1438,26 @@ static void store_coefs(Chunk& c, const double* alphas, const double* betas, int
     c.betas.assign(betas, betas + n);
 }
 
-// Device copy for the fused PPCG passes (tl_ppcg_tb.hip): up to 12 steps
-// run as one pass; more run as passes of 10 and a remainder.  The pass that
-// starts at step k finds a[k..k+m) and then b[k..k+m) from offset 2k.
+// Device copy of the stored Chebyshev coefficients for the temporal-blocked
+// iterations: a[0..n) then b[0..n).
+void op_cheby_upload(Chunk& c)
+{
+    const int n = (int)c.alphas.size();
+    if (n == 0) throw Error("no Chebyshev coefficients stored");
+    if (2 * n > c.d_cheby_cap) {
+        c.sync();
+        (void)hipFree(c.d_cheby);
+        c.d_cheby = nullptr;
+        TL_HIP(hipMalloc(&c.d_cheby, sizeof(double) * 2 * (size_t)n));
+        c.d_cheby_cap = 2 * n;
+    }
+    std::vector<double> h(c.alphas);
+    h.insert(h.end(), c.betas.begin(), c.betas.end());
+    c.sync();
+    TL_HIP(hipMemcpy(c.d_cheby, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    c.d_cheby_n = n;
+}
+
 // ext_core.cpp:468-497
 void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, double theta, bool preconditioner,
                    bool fused)
@@ -1445,21 +1465,7 @@ void op_cheby_init(Chunk& c, const double* alphas, const double* betas, int n, d
     c.preconditioner = preconditioner;
     store_coefs(c, alphas, betas, n);
     const int pre = preconditioner;
-    if (fused && TL_DIM == 2 && n > 0) {
-        // device copy for the temporal-blocked iterations: a[0..n) then b[0..n)
-        if (2 * n > c.d_cheby_cap) {
-            c.sync();
-            (void)hipFree(c.d_cheby);
-            c.d_cheby = nullptr;
-            TL_HIP(hipMalloc(&c.d_cheby, sizeof(double) * 2 * (size_t)n));
-            c.d_cheby_cap = 2 * n;
-        }
-        std::vector<double> h(c.alphas);
-        h.insert(h.end(), c.betas.begin(), c.betas.end());
-        c.sync();
-        TL_HIP(hipMemcpy(c.d_cheby, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-        c.d_cheby_n = n;
-    }
+    if (fused && TL_DIM == 2 && n > 0) op_cheby_upload(c);
     if (fused) {
         STRIP_LAUNCH((k_cheby<true, true>), c, kptr(c), pre, theta, 0.0, 0.0, c.f[F_U], c.f[F_U0], c.f[F_MI],
                      c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt);

@@ -321,3 +321,65 @@ def test_ppcg_outer_fused(hip, oracle, shape, nsteps):
     assert_sum(rrn_g.value, rrn_o, rrn_o)
     g.close()
     o.close()
+
+
+# meshes narrower than the passes' reach, odd and even widths, and iteration
+# counts that need passes of 10, 5, 2 and 1
+FUSED_SHAPES = [((37, 23), 10), ((5, 4), 13), ((9, 7), 17), ((131, 6), 10), ((67, 33), 3), ((2, 17), 6),
+                ((130, 41), 8)]
+
+
+@pytest.mark.parametrize("shape,count", FUSED_SHAPES)
+def test_cheby_iterate_fused(hip, oracle, shape, count):
+    """The driver's temporal-blocked Chebyshev passes against the reference's
+    one-iteration ChebyIterate + ChebyCalcU calls (with the depth-1 halo
+    update of u before each, as tea_solve.f90 makes it) on the oracle: u, p
+    and r bitwise, ||r||^2 to 1e-12."""
+    import ctypes
+    nx, ny = shape
+    g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+    o = hip.ExtChunk(oracle.lib(2), 2, nx, ny)
+    rng = np.random.default_rng(nx * 1000 + ny + count)
+    random_fields(g, o, ["u", "u0", "p"], rng, lo=-1.0, hi=1.0)
+    random_fields(g, o, ["kx", "ky"], rng, lo=0.1, hi=40.0)
+    step1 = 2
+    a, b, theta = oracle.calc_ch_coefs(0.7, 150.0, step1 + count + 1)
+    for ch in (g, o):
+        ch.update_halo([FIELD_U], 1)
+        ch.cheby_solver_init(a, b, theta, False)
+        ch.update_halo([FIELD_U], 1)
+    rrn_g = ctypes.c_double()
+    rc = g.L.tl_cheby_iterate_fused(g.c, ctypes.c_int(step1), ctypes.c_int(count), ctypes.byref(rrn_g))
+    assert rc == 0, g.L.tl_last_error()
+    for step in range(step1, step1 + count):
+        o.cheby_solver_iterate(step)
+        o.update_halo([FIELD_U], 1)
+    rrn_o = o.calculate_2norm(True)
+    assert_same(g, o, ["u", "p", "r"], region="interior")
+    assert_sum(rrn_g.value, rrn_o, rrn_o)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("shape,count", FUSED_SHAPES)
+def test_jacobi_iterate_fused(hip, oracle, shape, count):
+    """The driver's temporal-blocked Jacobi passes against the reference's
+    JacobiCopyU + JacobiSolve calls (depth-1 halo update of u before each):
+    u and r bitwise, each iteration's error sum to 1e-12."""
+    import ctypes
+    nx, ny = shape
+    g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+    o = hip.ExtChunk(oracle.lib(2), 2, nx, ny)
+    rng = np.random.default_rng(nx * 1000 + ny + 7 * count)
+    random_fields(g, o, ["u", "u0"], rng, lo=0.1, hi=2.0)
+    random_fields(g, o, ["kx", "ky"], rng, lo=0.1, hi=40.0)
+    errs = (ctypes.c_double * count)()
+    rc = g.L.tl_jacobi_iterate_fused(g.c, ctypes.c_int(count), errs)
+    assert rc == 0, g.L.tl_last_error()
+    for k in range(count):
+        o.update_halo([FIELD_U], 1)
+        e = o.jacobi_solve()
+        assert_sum(errs[k], e, e)
+    assert_same(g, o, ["u", "r"], region="interior")
+    g.close()
+    o.close()
