@@ -52,7 +52,7 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
     K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_JACOBI_TB,
-    K_COUNT
+    K_CG_WPU, K_CG_R, K_CG_U, K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -180,10 +180,15 @@ void op_cg_calc_p(Chunk& c, double beta);
 // Device-resident forms for the driver: the scalars (rro, pw, alpha, rrn,
 // beta, error) stay in Chunk::ds and are updated by the reductions' second
 // phase exactly as tea_solve.f90 updates them; nothing waits for the host.
-// calc_w with `fold` first applies the pending p = beta p + r (or z).
-// calc_ur records alpha/beta of CG iteration `n` (1-based) when n > 0.
-void op_cg_calc_w_dev(Chunk& c, bool fold);
+// calc_w with `fold` first applies the pending p = beta p + r (or z), with
+// `fold_u` (2D) also the pending u += alpha p that calc_r left.
+// calc_ur / calc_r record alpha/beta of CG iteration `n` (1-based) when
+// n > 0; calc_r updates r only and leaves u += alpha p pending, for the
+// next calc_w with fold_u or for calc_u.
+void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u = false);
 void op_cg_calc_ur_dev(Chunk& c, int n);
+void op_cg_calc_r_dev(Chunk& c, int n);
+void op_cg_calc_u_dev(Chunk& c);
 void op_cg_calc_p_dev(Chunk& c);
 void op_ppcg_calc_ur_dev(Chunk& c);  // u += alpha p, r -= alpha w; its r.r is not used
 void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in the PPCG loop

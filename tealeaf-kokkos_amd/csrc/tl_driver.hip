@@ -383,7 +383,8 @@ void halo(Chunk& c, std::initializer_list<int> which, int depth)
 // Fortran recurrences there; the host reads `error` once per iteration for
 // the convergence and switch tests.  CGCalcP is deferred into the next
 // CGCalcW (`p_pending`) and applied on its own only where r changes first
-// (the residual recomputed at the PPCG switch).
+// (the residual recomputed at the PPCG switch); in 2D so is the u half of
+// CGCalcUr (`u_pending`), applied on its own at the switch and at the end.
 StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
 {
     using clock = std::chrono::steady_clock;
@@ -395,6 +396,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
     int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
     bool p_pending = false;
+    // CG iterations leave u += alpha p to the next iteration's p, w strip
+    // (2D): u_pending marks an update not yet applied
+    const bool defer_u = TL_DIM == 2;
+    bool u_pending = false;
     // mixed-precision PPCG state: rrn at the last residual replacement, and
     // true-residual checks that failed without halving the true rrn
     // mixed_form: the solve uses residual replacement and accepts only the
@@ -446,6 +451,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             ch_switch_check = (n >= d.presteps) && (error <= 1.0);
 
         if ((d.use_cheby || d.use_ppcg) && ch_switch_check) {
+            if (u_pending) {
+                op_cg_calc_u_dev(c);
+                u_pending = false;
+            }
             if (cheby_calc_steps == 0) {
                 max_cheby_iters = max_iters - n + 2;
                 op_read_history(c, n - 1, cg_alphas, cg_betas);
@@ -547,8 +556,16 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
             ++cg_calc_steps;
-            op_cg_calc_w_dev(c, p_pending);
-            op_cg_calc_ur_dev(c, want_history ? n : 0);
+            if (defer_u) {
+                // CGCalcUr's u += alpha p rides on the next iteration's p, w
+                // strip, which reads p anyway (bitwise the same update)
+                op_cg_calc_w_dev(c, p_pending, u_pending);
+                op_cg_calc_r_dev(c, want_history ? n : 0);
+                u_pending = true;
+            } else {
+                op_cg_calc_w_dev(c, p_pending);
+                op_cg_calc_ur_dev(c, want_history ? n : 0);
+            }
             p_pending = true;
             error = op_error(c);
         } else if (d.use_jacobi && jacobi_tb_supported(c)) {
@@ -584,6 +601,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             }
         }
         if (std::fabs(error) < d.eps) break;
+    }
+    if (u_pending) {
+        op_cg_calc_u_dev(c);
+        u_pending = false;
     }
     if (solve_hooks().on_end) solve_hooks().on_end(c);
     if (d.check_result) {

@@ -410,15 +410,19 @@ __device__ __forceinline__ double dpp_from_east(double v)
 // of rows j-1, j, j+1 is kept in registers, x-neighbours come from adjacent
 // lanes (DPP) and, at the wave's edges, from one extra load.  Every input
 // is read from HBM once: 48 bytes per cell.  Same expressions as k_cg_calc_w.
+// With U, the previous iteration's CGCalcU (u += alpha p, with that
+// iteration's alpha and p, both still in place) is folded in as well, so
+// CGCalcUr's second sweep only updates r: 16 more bytes here, 24 fewer there.
 struct PWStrip {
     int tiles_x, rows;
     long nwaves;
 };
 
+template <bool U>
 __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k, const double* __restrict__ p,
                                                        const double* __restrict__ q,
                                                        const DevScalars* __restrict__ ds, double* __restrict__ p_out,
-                                                       double* __restrict__ w, Reduce red)
+                                                       double* __restrict__ w, double* __restrict__ u, Reduce red)
 {
     const int lane = threadIdx.x % WAVE;
     const long wv = (long)blockIdx.x * WAVES_PER_BLOCK + threadIdx.x / WAVE;
@@ -440,14 +444,18 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
         const bool edge = lane == 0 || lane == WAVE - 1;
         const int j0 = HALO + ty * t.rows, j1 = min(j0 + t.rows, ny + HALO);  // rows [j0, j1)
         const double beta = ds->beta;
+        const double alpha = U ? ds->alpha : 0.0;
         auto pnew = [&](int col, int row) {
             const long i = gidx(g, col, row, 0);
             return beta * p[i] + q[i];
         };
         auto ld2 = [&](const double* a, long i) { return *reinterpret_cast<const double2*>(a + i); };
+        // the new p of a row's column pair; `raw` keeps the old p (for u)
+        double2 raw;
         auto pnew2 = [&](int row) {
             const long i = gidx(g, k0, row, 0);
             const double2 pp = ld2(p, i), qq = ld2(q, i);
+            raw = pp;
             return make_double2(beta * pp.x + qq.x, beta * pp.y + qq.y);
         };
         // window: rows j-1 (s), j (c), j+1 (n) of the new p, two columns
@@ -456,6 +464,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
         double s0 = t2.x, s1 = t2.y;
         t2 = pnew2(j0);
         double c0v = t2.x, c1v = t2.y;
+        double2 pold_c = raw;  // old p of rows j and j + 1, u of row j
+        double2 uc = U ? ld2(u, gidx(g, k0, j0, 0)) : make_double2(0.0, 0.0);
         double ec = edge ? pnew(ce, j0) : 0.0;
         if (j0 == HALO) {  // bottom face: the reflective mirror
             s0 = c0v;
@@ -469,6 +479,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
         const int jn = min(j0 + 1, ny + 1);
         t2 = pnew2(jn);
         double n0 = t2.x, n1 = t2.y;
+        double2 pold_n = raw;
         double en = edge ? pnew(ce, jn) : 0.0;
         t2 = ld2(k.ky, gidx(g, k0, j0 + 1, 0));  // rows <= ny + 2
         double kyn0 = t2.x, kyn1 = t2.y;
@@ -488,6 +499,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             const double2 kxq = ld2(k.kx, gidx(g, f0, j1k, 0));
             const double kxq0 = kxq.x, kxq1 = kxq.y;
             const double kxql = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j1k, 0)] : 0.0;
+            const double2 un = U ? ld2(u, gidx(g, k0, j1k, 0)) : make_double2(0.0, 0.0);
 
             double nn0 = n0, nn1 = n1;
             if (j == ny + 1) {  // top face
@@ -508,16 +520,23 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             if (c1 == nx + 1) e1 = c1v;
             const double m0 = (1.0 + (kx1 + kx0) + (kyn0 + kyc0)) * c0v - (kx1 * e0 + kx0 * w0) - (kyn0 * nn0 + kyc0 * s0);
             const double m1 = (1.0 + (kxe1 + kx1) + (kyn1 + kyc1)) * c1v - (kxe1 * e1 + kx1 * w1) - (kyn1 * nn1 + kyc1 * s1);
+            // CGCalcUr's u + alpha p of the previous iteration (c0 = k0 where a cell is stored)
+            const double2 uo = make_double2(uc.x + alpha * pold_c.x, uc.y + alpha * pold_c.y);
             if (v1) {
                 *reinterpret_cast<double2*>(p_out + i0) = make_double2(c0v, c1v);
                 *reinterpret_cast<double2*>(w + i0) = make_double2(m0, m1);
+                if (U) *reinterpret_cast<double2*>(u + i0) = uo;
                 acc += m0 * c0v;
                 acc += m1 * c1v;
             } else if (v0) {
                 p_out[i0] = c0v;
                 w[i0] = m0;
+                if (U) u[i0] = uo.x;
                 acc += m0 * c0v;
             }
+            pold_c = pold_n;
+            pold_n = pp;
+            uc = un;
             s0 = c0v;
             s1 = c1v;
             c0v = n0;
@@ -542,6 +561,9 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
 #endif
 
 // CGCalcUr (:244-264); alpha from the device scalars when ds is given.
+// Without U the u update is left to the next k_cg_pw_strip<true> (or
+// k_cg_calc_u), which applies it with the same alpha and p.
+template <bool U>
 __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, const DevScalars* __restrict__ ds,
                                                       int pre, double* __restrict__ u, double* __restrict__ r,
                                                       const double* __restrict__ p, const double* __restrict__ w,
@@ -552,7 +574,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, co
     const double alpha = ds ? ds->alpha : alpha_v;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) {
         const long i = s.i;
-        u[i] = u[i] + alpha * p[i];
+        if (U) u[i] = u[i] + alpha * p[i];
         const double rv = r[i] - alpha * w[i];
         r[i] = rv;
         if (pre) {
@@ -564,6 +586,19 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, co
         }
     }
     block_partial<1>(v, red);
+}
+
+// The u half of CGCalcUr (u += alpha p, alpha from the device scalars): the
+// update a k_cg_calc_ur<false> left pending, where no pw strip follows.
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_u(Geom g, const DevScalars* __restrict__ ds,
+                                                     double* __restrict__ u, const double* __restrict__ p)
+{
+    Strip s = make_strip(g);
+    const double alpha = ds->alpha;
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        u[i] = u[i] + alpha * p[i];
+    }
 }
 
 // CGCalcP (:301-310); beta from the device scalars when ds is given.
@@ -921,7 +956,7 @@ const char* kernel_name(int k)
                                          "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
                                          "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
-                                         "jacobi_tb"};
+                                         "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -946,6 +981,9 @@ double kernel_bytes_per_cell(int k)
     case K_PPCG_MIXED: return 48.0 + kb / 2;  // u, p, r, w, FP32 coefficients -> u, r
     case K_CHEBY_TB: return 48.0 + kb;        // u, p, u0, coefficients -> u', p', r (per pass)
     case K_JACOBI_TB: return 32.0 + kb;       // u, u0, coefficients -> u', r (per pass)
+    case K_CG_WPU: return 48.0 + kb;          // u, p, r, coefficients -> u', p', w
+    case K_CG_R: return 24.0;                 // r, w -> r
+    case K_CG_U: return 24.0;                 // u, p -> u
     default: return 0.0;
     }
 }
@@ -1305,7 +1343,7 @@ double op_cg_calc_w(Chunk& c)
 // ext_core.cpp:433-447
 double op_cg_calc_ur(Chunk& c, double alpha)
 {
-    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, alpha, nullptr, (int)c.preconditioner, c.f[F_U], c.f[F_R],
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, alpha, nullptr, (int)c.preconditioner, c.f[F_U], c.f[F_R],
                                    c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
     finish<1>(c, c.g.nblocks, S_RRN, EP_STORE);
     return c.result(S_RRN);
@@ -1317,9 +1355,10 @@ void op_cg_calc_p(Chunk& c, double beta)
     TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, beta, nullptr, c.f[F_P], c.preconditioner ? c.f[F_Z] : c.f[F_R]));
 }
 
-void op_cg_calc_w_dev(Chunk& c, bool fold)
+void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
 {
     const double* q = c.preconditioner ? c.f[F_Z] : c.f[F_R];
+    if (fold_u && (!fold || TL_DIM != 2)) throw Error("the pending u update folds only into the 2D p, w strip");
     if (fold) {
 #if TL_DIM == 2
         PWStrip t;
@@ -1333,11 +1372,19 @@ void op_cg_calc_w_dev(Chunk& c, bool fold)
         if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
         t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-        TIMED(c, K_CG_WP, {
-            hipLaunchKernelGGL(k_cg_pw_strip, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t, kptr(c), c.f[F_P],
-                               q, c.ds, c.p_alt, c.f[F_W], c.reducer());
-            check_launch();
-        });
+        if (fold_u) {
+            TIMED(c, K_CG_WPU, {
+                hipLaunchKernelGGL(k_cg_pw_strip<true>, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t, kptr(c),
+                                   c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer());
+                check_launch();
+            });
+        } else {
+            TIMED(c, K_CG_WP, {
+                hipLaunchKernelGGL(k_cg_pw_strip<false>, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
+                                   kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer());
+                check_launch();
+            });
+        }
         std::swap(c.f[F_P], c.p_alt);
         finish<1>(c, nb, S_PW, EP_PW);
         return;
@@ -1356,14 +1403,27 @@ void op_cg_calc_w_dev(Chunk& c, bool fold)
 void op_cg_calc_ur_dev(Chunk& c, int n)
 {
     if (n > 0) c.ensure_history(n);
-    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
                                    c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
     finish<1>(c, c.g.nblocks, S_RRN, EP_RRN, n > 0 ? n : -1);
 }
 
+void op_cg_calc_r_dev(Chunk& c, int n)
+{
+    if (n > 0) c.ensure_history(n);
+    TIMED(c, K_CG_R, STRIP_LAUNCH(k_cg_calc_ur<false>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R],
+                                  c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN, n > 0 ? n : -1);
+}
+
+void op_cg_calc_u_dev(Chunk& c)
+{
+    TIMED(c, K_CG_U, STRIP_LAUNCH(k_cg_calc_u, c, c.ds, c.f[F_U], c.f[F_P]));
+}
+
 void op_ppcg_calc_ur_dev(Chunk& c)
 {
-    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
                                    c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
 }
 
