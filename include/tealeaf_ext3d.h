@@ -114,6 +114,11 @@ const char* tl_driver_error(void);
  * timesteps from `from_step` on (0 = off); tl_run_deck's report then has a
  * "kernels" table: launches, summed milliseconds, algorithmic bytes/cell. */
 void tl_set_kernel_timing(int from_step);
+/* Restricts that timing to one kernel, by its name in the report's
+ * "kernels" table (NULL or "" = every kernel): fewer events in the stream,
+ * less perturbed timesteps.  No counterpart in the reference.  Returns -1
+ * for an unknown name. */
+int tl_set_kernel_timing_only(const char* name);
 /* A progress line on stderr after every timestep of tl_run_deck (0 = off). */
 void tl_set_progress(int on);
 

@@ -63,6 +63,7 @@ double kernel_bytes_per_cell(int k);
 // pairs are resolved at the next host synchronisation.
 struct KernelTimer {
     bool on = false;
+    int only = -1;  // time only this KernelId (-1: all)
     double ms[K_COUNT] = {};
     long count[K_COUNT] = {};
     struct Pending {
@@ -77,14 +78,18 @@ struct KernelTimer {
     void flush();
     ~KernelTimer();
 };
-// Timesteps from which the driver turns each chunk's timer on (0 = never).
+// Timesteps from which the driver turns each chunk's timer on (0 = never),
+// and the one kernel it then times (-1: all).
 void set_kernel_timing_from(int step);
 int kernel_timing_from();
+void set_kernel_timing_only(int kid);
+int kernel_timing_only();
+int kernel_id(const char* name);  // -1 for an unknown name
 
 // Runs STMT between two timer events when the chunk's timer is on.
 #define TL_TIMED(c, kid, STMT)                               \
     do {                                                     \
-        if ((c).timer.on) {                                  \
+        if ((c).timer.on && ((c).timer.only < 0 || (c).timer.only == (kid))) { \
             ::tl::KernelTimer::Pending p_;                   \
             (c).timer.begin((kid), (c).stream, p_);          \
             STMT;                                            \

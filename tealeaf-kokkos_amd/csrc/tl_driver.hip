@@ -690,6 +690,7 @@ std::string run_deck(const std::string& text, int max_steps)
     for (;;) {
         ++step;
         c->timer.on = kernel_timing_from() > 0 && step >= kernel_timing_from();
+        c->timer.only = kernel_timing_only();
         if (d.dtinit <= dt) dt = d.dtinit;  // timestep.f90 / calc_dt.f90:35
         const StepInfo si = tea_leaf(*c, d, dt);
         total += si.solve_s;
@@ -771,6 +772,22 @@ const char* tl_driver_error(void) { return tl::g_driver_error.c_str(); }
 // "kernels" table (launch count, summed milliseconds, algorithmic bytes per
 // cell) to its report.
 void tl_set_kernel_timing(int from_step) { tl::set_kernel_timing_from(from_step); }
+
+// Restricts that timing to the kernel of the given report name (NULL or ""
+// times every kernel again): two events per launch of one kernel perturb
+// the timestep less than two around every launch.  Returns -1 for an
+// unknown name.
+int tl_set_kernel_timing_only(const char* name)
+{
+    if (!name || !*name) {
+        tl::set_kernel_timing_only(-1);
+        return 0;
+    }
+    const int k = tl::kernel_id(name);
+    if (k < 0) return -1;
+    tl::set_kernel_timing_only(k);
+    return 0;
+}
 
 // A line on stderr after every timestep of tl_run_deck (0 turns it off).
 void tl_set_progress(int on) { tl::g_progress = on != 0; }

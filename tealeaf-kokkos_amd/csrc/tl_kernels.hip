@@ -947,8 +947,18 @@ void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
 // Kernel timer
 
 static int g_timing_from = 0;
+static int g_timing_only = -1;
 void set_kernel_timing_from(int step) { g_timing_from = step < 0 ? 0 : step; }
 int kernel_timing_from() { return g_timing_from; }
+void set_kernel_timing_only(int kid) { g_timing_only = kid; }
+int kernel_timing_only() { return g_timing_only; }
+
+int kernel_id(const char* name)
+{
+    for (int k = 0; name && k < K_COUNT; ++k)
+        if (std::strcmp(kernel_name(k), name) == 0) return k;
+    return -1;
+}
 
 const char* kernel_name(int k)
 {
