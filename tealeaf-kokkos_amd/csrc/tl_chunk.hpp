@@ -145,6 +145,12 @@ struct Chunk {
     double* d_result = nullptr;           // [S_COUNT * 4] device
     volatile double* h_result = nullptr;  // [S_COUNT * 4] host-mapped
     double* h_result_dev = nullptr;       // device alias of h_result
+    // per slot, the number of the last reduction launched and the number
+    // its second phase writes (host-mapped) after its result: the host
+    // reads a result as soon as the number arrives, without a stream sync
+    unsigned fin_seq[S_COUNT] = {};
+    volatile unsigned* h_seq = nullptr;
+    unsigned* h_seq_dev = nullptr;
 
     hipStream_t stream = nullptr;
     int simds = 1024;            // SIMDs of the device (CUs x 4): one resident wave each
@@ -157,7 +163,7 @@ struct Chunk {
     Chunk& operator=(const Chunk&) = delete;
 
     Reduce reducer() const { return Reduce{partials}; }
-    double result(Slot s, int k = 0) const;  // waits for the stream
+    double result(Slot s, int k = 0) const;  // waits for the slot's last reduction
     void ensure_history(int n);
     void sync() const;
 

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 
 namespace tl {
@@ -622,7 +624,8 @@ template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
                                                   DevScalars* __restrict__ ds, double* __restrict__ dout,
                                                   double* __restrict__ hout, double* __restrict__ hist_a,
-                                                  double* __restrict__ hist_b, int hist_n)
+                                                  double* __restrict__ hist_b, int hist_n, unsigned* hseq,
+                                                  unsigned seq)
 {
     double v[NV];
 #pragma unroll
@@ -678,6 +681,9 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
         dout[k] = out[k];
         hout[k] = out[k];
     }
+    // the result reaches host memory before the number the host waits for
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned*>(hseq) = seq;
 }
 
 // ----------------------------------------------------------------------------
@@ -936,8 +942,10 @@ __global__ __launch_bounds__(COPY_BLOCK) void k_stream_copy(const double2* __res
 template <int NV>
 void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
 {
+    const unsigned seq = ++c.fin_seq[slot];
     hipLaunchKernelGGL(k_finish<NV>, dim3(1), dim3(BLOCK), 0, c.stream, c.partials, nblocks, ep, c.ds,
-                       c.d_result + 4 * slot, c.h_result_dev + 4 * slot, c.hist_a, c.hist_b, hist_n);
+                       c.d_result + 4 * slot, c.h_result_dev + 4 * slot, c.hist_a, c.hist_b, hist_n,
+                       c.h_seq_dev + slot, seq);
     check_launch();
 }
 
@@ -1137,6 +1145,11 @@ Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
     std::memset(h, 0, sizeof(double) * 4 * S_COUNT);
     h_result = h;
     TL_HIP(hipHostGetDevicePointer((void**)&h_result_dev, h, 0));
+    unsigned* hs = nullptr;
+    TL_HIP(hipHostMalloc(&hs, sizeof(unsigned) * S_COUNT, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(hs, 0, sizeof(unsigned) * S_COUNT);
+    h_seq = hs;
+    TL_HIP(hipHostGetDevicePointer((void**)&h_seq_dev, hs, 0));
     TL_HIP(hipStreamSynchronize(stream));
 }
 
@@ -1166,6 +1179,7 @@ Chunk::~Chunk()
     (void)hipFree(partials);
     (void)hipFree(d_result);
     if (h_result) (void)hipHostFree((void*)h_result);
+    if (h_seq) (void)hipHostFree((void*)h_seq);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1189,9 +1203,23 @@ void Chunk::sync() const
     if (timer.on) timer.flush();
 }
 
+// A slot's result once its reduction's second phase has written it: the
+// host spins on the number that phase writes after the result (the device
+// fences the result first), so the next launches need not wait for a
+// stream synchronisation.  A wait longer than 50 ms, or more than 1024
+// pending timer events, falls back to synchronising the stream (which also
+// reports a failed launch).
 double Chunk::result(Slot s, int k) const
 {
-    sync();
+    const unsigned want = fin_seq[s];
+    if (timer.pending.size() < 1024) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (long i = 1; h_seq[s] != want; ++i)
+            if ((i & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+    }
+    if (h_seq[s] != want) sync();
+    if (h_seq[s] != want) throw Error("reduction result of slot " + std::to_string((int)s) + " not delivered");
+    std::atomic_thread_fence(std::memory_order_acquire);
     return h_result[4 * s + k];
 }
 
