@@ -224,12 +224,6 @@ __device__ __forceinline__ void tb_load(const TBArgs& A, const TBCtx& c, unsigne
     v.ky = ldp(A.ky, of + c.bcp);
 }
 
-__device__ __forceinline__ int ring(int row, int n)
-{
-    const int m = row % n;
-    return m < 0 ? m + n : m;
-}
-
 // Rounds per unrolled block: a multiple of the register windows' periods
 // (3 rows of sd, 2 of r, TB_PF prefetch slots) and of the LDS ring length,
 // so that every window slot and ring row is a compile-time constant.
