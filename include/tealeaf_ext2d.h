@@ -54,8 +54,13 @@ void ext_field_summary_kernel_(const int* chunk, double* vol, double* mass, doub
 /* ext_core.cpp:305-336 (reflective copy on faces whose neighbour is -1;
  * fields[6] flags density, energy0, energy1, u, p, sd; depth 1 or 2) */
 void ext_update_halo_kernel_(const int* chunk, const int* chunkNeighbours, const int* fields, const int* depth);
-/* ext_core.cpp:151-245.  Multi-chunk face packing: not supported (one chunk
- * per GPU); calling it is an error. */
+/* ext_core.cpp:151-244, ext_pack_kernel.hpp:7-97.  Halo message of one face
+ * (face 1-4 = left, right, bottom, top; depth 1 or 2) for a neighbouring
+ * chunk: *pack != 0 copies each flagged field's mesh strip next to the face
+ * into buffer + offsets[field], *pack == 0 copies buffer + offsets[field]
+ * into the halo strip beyond the face.  Message lengths: ny*depth (left,
+ * right), nx*depth (top, bottom), in the reference's element order (its
+ * strips start at row 0 / column 0, and so do these). */
 void ext_pack_message_(const int* chunk, const int* fields, const int* offsets, const int* depth, const int* face,
                        double* buffer, bool* pack);
 

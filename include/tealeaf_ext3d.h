@@ -55,8 +55,13 @@ void ext_field_summary_kernel_(const int* chunk, double* vol, double* mass, doub
 /* ext_core.cpp:334-367 (reflective copy of the face interiors whose
  * neighbour is -1; chunkNeighbours[6]; fields[6] flags density, energy0, energy1, u, p, sd; depth 1 or 2) */
 void ext_update_halo_kernel_(const int* chunk, const int* chunkNeighbours, const int* fields, const int* depth);
-/* ext_core.cpp:168-272.  Multi-chunk face packing: not supported (one chunk
- * per GPU); calling it is an error. */
+/* ext_core.cpp:168-270, ext_pack_kernel.hpp:8-196.  Halo message of one
+ * face (face 1-6 = left, right, bottom, top, back, front; depth 1 or 2) for
+ * a neighbouring chunk: *pack != 0 copies each flagged field's mesh strip
+ * next to the face into buffer + offsets[field], *pack == 0 copies buffer +
+ * offsets[field] into the halo strip beyond the face (fieldType is not
+ * read).  Message lengths: ny*nz*depth (left, right), nx*nz*depth (bottom,
+ * top), nx*ny*depth (back, front), in the reference's element order. */
 void ext_pack_message_(const int* chunk, const int* fields, const int* offsets, const int* depth, const int* face,
                        const int* fieldType, double* buffer, bool* pack);
 

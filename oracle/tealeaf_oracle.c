@@ -548,11 +548,104 @@ void ext_update_halo_kernel_(const int *chunk, const int *chunkNeighbours, const
     }
 }
 
-/* Multi-chunk message packing (ext_core.cpp:151-245) only runs when a face
- * has a neighbour chunk, which the single-chunk configurations never have. */
-void ext_pack_message_(void)
+/* Halo messages for a neighbouring chunk: ext_core.cpp:151-244 (2D),
+ * :168-270 (3D), element b of a face's message at the flat index the
+ * packers of ext_pack_kernel.hpp give (2D :10-39 top/bottom, :58-85
+ * left/right; 3D :21-61 top/bottom, :83-122 left/right, :146-185
+ * back/front), restated index for index. */
+static long pack_index(const Chunk *c, int face, int depth, int pack, long b)
 {
-    set_error("ext_pack_message_: multi-chunk halo exchange is not part of this build");
+    const long X = c->X, Y = c->Y;
+#if TL_DIM == 2
+    if (face == CHUNK_TOP || face == CHUNK_BOTTOM) {
+        const long width = X - 2 * HALO, lines = b / width;
+        long off;
+        if (face == CHUNK_TOP && pack) off = X * (Y - HALO - depth) + lines * 2 * HALO;
+        else if (face == CHUNK_BOTTOM && pack) off = X * HALO + lines * 2 * HALO;
+        else if (face == CHUNK_TOP) off = X * (Y - HALO) + lines * 2 * HALO;
+        else off = X * (HALO - depth) + lines * 2 * HALO;
+        return off + b;
+    } else {
+        const long lines = b / depth;
+        long off;
+        if (face == CHUNK_LEFT && pack) off = HALO + lines * (X - depth);
+        else if (face == CHUNK_RIGHT && pack) off = X - HALO - depth + lines * (X - depth);
+        else if (face == CHUNK_LEFT) off = HALO - depth + lines * (X - depth);
+        else off = X - HALO + lines * (X - depth);
+        return off + b;
+    }
+#else
+    const long Z = c->Z, XY = X * Y, HP2 = 2 * HALO;
+    if (face == CHUNK_TOP || face == CHUNK_BOTTOM) {
+        const long lines = b / (X - HP2);
+        long row;
+        if (face == CHUNK_TOP && pack) row = Y - HALO - depth;
+        else if (face == CHUNK_BOTTOM && pack) row = HALO;
+        else if (face == CHUNK_TOP) row = Y - HALO;
+        else row = HALO - depth;
+        return XY * HALO + X * row + HALO + lines * HP2 + X * (Y - depth) * (lines / depth) + b;
+    } else if (face == CHUNK_LEFT || face == CHUNK_RIGHT) {
+        const long height = Y - HP2, lines = b / depth;
+        long col;
+        if (face == CHUNK_LEFT && pack) col = HALO;
+        else if (face == CHUNK_RIGHT && pack) col = X - HALO - depth;
+        else if (face == CHUNK_LEFT) col = HALO - depth;
+        else col = X - HALO;
+        return XY * HALO + X * HALO + col + lines * (X - depth) + (lines / height) * X * HP2 + b;
+    } else {
+        const long height = Y - HP2, lines = b / (X - HP2);
+        long plane;
+        if (face == CHUNK_FRONT && pack) plane = Z - HALO - depth;
+        else if (face == CHUNK_BACK && pack) plane = HALO;
+        else if (face == CHUNK_FRONT) plane = Z - HALO;
+        else plane = HALO - depth;
+        return XY * plane + X * HALO + HALO + lines * HP2 + (lines / height) * X * HP2 + b;
+    }
+#endif
+}
+
+#if TL_DIM == 3
+void ext_pack_message_(const int *chunk, const int *fields, const int *offsets, const int *depth, const int *face,
+                       const int *fieldType, double *buffer, bool *pack)
+#else
+void ext_pack_message_(const int *chunk, const int *fields, const int *offsets, const int *depth, const int *face,
+                       double *buffer, bool *pack)
+#endif
+{
+#if TL_DIM == 3
+    (void)fieldType;
+#endif
+    CHECK_CHUNK(c, *chunk);
+    const int pk = *(const unsigned char *)pack != 0;
+    long len;
+    /* lrLineLen = yMax (*zMax), tbLineLen = xMax (*zMax), fbLineLen = xMax*yMax
+       (ext_chunk.cpp:78-79; 3D :97-99) */
+    switch (*face) {
+    case CHUNK_LEFT: case CHUNK_RIGHT: len = (long)c->ny * c->nz * *depth; break;
+    case CHUNK_TOP: case CHUNK_BOTTOM: len = (long)c->nx * c->nz * *depth; break;
+#if TL_DIM == 3
+    case CHUNK_BACK: case CHUNK_FRONT: len = (long)c->nx * c->ny * *depth; break;
+#endif
+    default: set_error("Incorrect face provided: %d.", *face); return;
+    }
+    for (int q = 0; q < NUM_FIELDS; q++) {
+        if (!fields[q]) continue;
+        double *f = NULL;
+        switch (q + 1) {
+        case FIELD_DENSITY: f = c->f[TF_DENSITY]; break;
+        case FIELD_ENERGY0: f = c->f[TF_ENERGY0]; break;
+        case FIELD_ENERGY1: f = c->f[TF_ENERGY1]; break;
+        case FIELD_U: f = c->f[TF_U]; break;
+        case FIELD_P: f = c->f[TF_P]; break;
+        case FIELD_SD: f = c->f[TF_SD]; break;
+        }
+        double *ext = buffer + offsets[q];
+        for (long b = 0; b < len; b++) {
+            const long i = pack_index(c, *face, *depth, pk, b);
+            if (pk) ext[b] = f[i];
+            else f[i] = ext[b];
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -115,6 +115,9 @@ struct Chunk {
     // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
     // elements) and of the inner-step coefficients, made at the switch.
     float* kxy32 = nullptr;  // FP32 kx, ky interleaved: element i at [2i], [2i + 1]
+    bool internal_faces = false;  // a face has a neighbour chunk (halo update or message)
+    double* d_pack = nullptr;     // staging of halo messages (ext_pack_message_)
+    long pack_cap = 0;
     unsigned* d_rowtab = nullptr;  // reflected row byte offsets of the temporal-blocked kernels (cell, face)
     double* d_cheby = nullptr;     // Chebyshev a[0..n) then b[0..n) on the device
     double* d_levels = nullptr;    // per-iteration error sums of a temporal-blocked Jacobi pass
@@ -182,6 +185,9 @@ void op_set_field(Chunk& c);
 void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* temp);
 // fields[6] flags in FIELD_* order, neighbours[NUM_FACES] (EXTERNAL_FACE marks a reflective face)
 void op_update_halo(Chunk& c, const int* neighbours, const int* fields, int depth);
+// Pack (pack) or unpack one face's halo message of the flagged fields to or
+// from buffer + offsets[field]: ext_pack_message_.
+void op_pack_message(Chunk& c, const int* fields, const int* offsets, int depth, int face, double* buffer, bool pack);
 
 double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt);
 double op_cg_calc_w(Chunk& c);

@@ -43,6 +43,19 @@ Chunk* chunk_at(const int* id)
     return g_chunks[*id - 1].get();
 }
 
+// A chunk for a stencil entry point.  The stencil kernels read a face's
+// outside neighbours as the mirrored interior (the reflective halo), which
+// is right only where every face is external: a chunk that has exchanged
+// halo messages with neighbour chunks is refused rather than solved wrongly.
+Chunk* stencil_chunk(const int* id)
+{
+    Chunk* c = chunk_at(id);
+    if (c->internal_faces)
+        throw Error("chunk " + std::to_string(*id) +
+                    " has neighbour chunks: this library's stencil kernels support external faces only");
+    return c;
+}
+
 template <class F>
 void guarded(F&& f)
 {
@@ -101,7 +114,7 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
 {
     int rc = -1;
     guarded([&] {
-        Chunk* c = chunk_at(chunk);
+        Chunk* c = stencil_chunk(chunk);
         DevScalars s{};
         s.rro = 1.0;
         s.alpha = alpha;
@@ -122,7 +135,7 @@ int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn)
 {
     int rc = -1;
     guarded([&] {
-        Chunk* c = chunk_at(chunk);
+        Chunk* c = stencil_chunk(chunk);
         op_cheby_upload(*c);
         *rrn = op_finish_norm(*c, op_cheby_tb(*c, step1, count, true));
         rc = 0;
@@ -138,7 +151,7 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors)
 {
     int rc = -1;
     guarded([&] {
-        Chunk* c = chunk_at(chunk);
+        Chunk* c = stencil_chunk(chunk);
         if (count < 1) throw Error("Jacobi needs at least one iteration");
         int done = 0;
         while (done < count) {
@@ -157,7 +170,7 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
 {
     int rc = -1;
     guarded([&] {
-        Chunk* c = chunk_at(chunk);
+        Chunk* c = stencil_chunk(chunk);
         if (!ppcg_mixed_supported(*c, nsteps)) throw Error("mixed-precision PPCG does not apply to this chunk");
         DevScalars s{};
         s.rro = 1.0;
@@ -396,15 +409,24 @@ void ext_field_summary_kernel_(const int* chunk, double* vol, double* mass, doub
     guarded([&] { op_field_summary(*chunk_at(chunk), vol, mass, ie, temp); });
 }
 
-// Multi-chunk face packing (ext_core.cpp:151-245) serves neighbour chunks
-// over MPI; a single chunk on one GPU never has one.
+// Halo message of one face for a neighbouring chunk (ext_core.cpp:151-244;
+// 3D :168-270, whose fieldType is not read): pack copies the flagged fields'
+// mesh strip next to the face into buffer + offsets[field], unpack copies
+// buffer + offsets[field] into the halo strip beyond it.  With one chunk per
+// process and rank, the Fortran driver's MPI exchange (tea.f90:246-407)
+// moves these buffers between processes, one GPU each.
 #if TL_DIM == 3
-void ext_pack_message_(const int*, const int*, const int*, const int*, const int*, const int*, double*, bool*)
+void ext_pack_message_(const int* chunk, const int* fields, const int* offsets, const int* depth, const int* face,
+                       const int* /*fieldType*/, double* buffer, bool* pack)
 #else
-void ext_pack_message_(const int*, const int*, const int*, const int*, const int*, double*, bool*)
+void ext_pack_message_(const int* chunk, const int* fields, const int* offsets, const int* depth, const int* face,
+                       double* buffer, bool* pack)
 #endif
 {
-    fail("ext_pack_message_: multi-chunk halo exchange is not supported by this library");
+    guarded([&] {
+        const bool p = *reinterpret_cast<const unsigned char*>(pack) != 0;
+        op_pack_message(*chunk_at(chunk), fields, offsets, *depth, *face, buffer, p);
+    });
 }
 
 void ext_set_field_kernel_(const int* chunk)
@@ -419,7 +441,7 @@ void ext_solver_copy_u_(const int* chunk)
 
 void ext_calculate_residual_(const int* chunk)
 {
-    guarded([&] { op_calc_residual(*chunk_at(chunk)); });
+    guarded([&] { op_calc_residual(*stencil_chunk(chunk)); });
 }
 
 void ext_calculate_2norm_(const int* chunk, double* norm, bool* normArray)
@@ -442,23 +464,23 @@ void ext_update_halo_kernel_(const int* chunk, const int* chunkNeighbours, const
 
 void ext_jacobi_kernel_init_(const int* chunk, const int* coefficient, const double* dt)
 {
-    guarded([&] { op_jacobi_init(*chunk_at(chunk), *coefficient, *dt); });
+    guarded([&] { op_jacobi_init(*stencil_chunk(chunk), *coefficient, *dt); });
 }
 
 void ext_jacobi_kernel_solve_(const int* chunk, double* error)
 {
-    guarded([&] { *error = op_jacobi_solve(*chunk_at(chunk), false); });
+    guarded([&] { *error = op_jacobi_solve(*stencil_chunk(chunk), false); });
 }
 
 void ext_cg_solver_init_(const int* chunk, const int* coefficient, const int* preconditioner, double* dt,
                          double* rro)
 {
-    guarded([&] { *rro = op_cg_init(*chunk_at(chunk), *coefficient, *preconditioner != 0, *dt); });
+    guarded([&] { *rro = op_cg_init(*stencil_chunk(chunk), *coefficient, *preconditioner != 0, *dt); });
 }
 
 void ext_cg_calc_w_(const int* chunk, double* pw)
 {
-    guarded([&] { *pw = op_cg_calc_w(*chunk_at(chunk)); });
+    guarded([&] { *pw = op_cg_calc_w(*stencil_chunk(chunk)); });
 }
 
 void ext_cg_calc_ur_(const int* chunk, const double* alpha, double* rrn)
@@ -475,13 +497,13 @@ void ext_cheby_solver_init_(const int* chunk, double* alphas, double* betas, con
                             const int* preconditioner, const int* maxChebyIters)
 {
     guarded([&] {
-        op_cheby_init(*chunk_at(chunk), alphas, betas, *maxChebyIters, *theta, *preconditioner != 0, false);
+        op_cheby_init(*stencil_chunk(chunk), alphas, betas, *maxChebyIters, *theta, *preconditioner != 0, false);
     });
 }
 
 void ext_cheby_solver_iterate_(const int* chunk, const int* chebyCalcStep)
 {
-    guarded([&] { op_cheby_iterate(*chunk_at(chunk), *chebyCalcStep, false); });
+    guarded([&] { op_cheby_iterate(*stencil_chunk(chunk), *chebyCalcStep, false); });
 }
 
 void ext_ppcg_init_(const int* chunk, double* alphas, double* betas, const double* theta, const int* maxChebyIters)
@@ -496,7 +518,7 @@ void ext_ppcg_init_sd_(const int* chunk)
 
 void ext_ppcg_inner_(const int* chunk, int* currentStep)
 {
-    guarded([&] { op_ppcg_inner(*chunk_at(chunk), *currentStep, false); });
+    guarded([&] { op_ppcg_inner(*stencil_chunk(chunk), *currentStep, false); });
 }
 
 // timer_c.cpp:30 (wall-clock seconds for the Fortran timer)

@@ -1163,6 +1163,7 @@ Chunk::~Chunk()
     (void)hipFree(r_alt);
     (void)hipFree(kxy32);
     (void)hipFree(d_rowtab);
+    (void)hipFree(d_pack);
     (void)hipFree(d_cheby);
     (void)hipFree(d_levels);
     (void)hipFree(d_coef32);
@@ -1319,12 +1320,102 @@ void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* t
     *temp = c.h_result[4 * S_SUMMARY + 3];
 }
 
+// ---------------------------------------------------------------------------
+// Halo messages between chunks (ext_pack_message_, ext_core.cpp:151-244 2D,
+// :168-270 3D; index order of ext_pack_kernel.hpp).  Buffer element b of a
+// face's message maps to one cell: the strip `depth` deep along the face,
+// inside the mesh to pack, in the halo to unpack.  The order is the
+// reference's: across the strip fastest for left/right, along the face
+// fastest for the others.  In 2D the reference's strips start at row 0
+// (left/right) and column 0 (top/bottom), not at the first mesh row or
+// column; that offset is kept, so that a message means the same to both.
+
+struct PackMap {
+    int face, depth, pack;
+    int lead;  // first column (left/right), row (top/bottom) or plane (back/front) of the strip
+};
+
+__device__ __forceinline__ long pack_cell(const Geom& g, const PackMap& m, long b)
+{
+#if TL_DIM == 2
+    if (m.face == CHUNK_LEFT || m.face == CHUNK_RIGHT)
+        return gidx(g, m.lead + (int)(b % m.depth), (int)(b / m.depth), 0);
+    return gidx(g, (int)(b % g.nx), m.lead + (int)(b / g.nx), 0);
+#else
+    if (m.face == CHUNK_LEFT || m.face == CHUNK_RIGHT) {
+        const long line = b / m.depth;
+        return gidx(g, m.lead + (int)(b % m.depth), HALO + (int)(line % g.ny), HALO + (int)(line / g.ny));
+    }
+    const long line = b / g.nx;
+    if (m.face == CHUNK_BOTTOM || m.face == CHUNK_TOP)
+        return gidx(g, HALO + (int)(b % g.nx), m.lead + (int)(line % m.depth), HALO + (int)(line / m.depth));
+    return gidx(g, HALO + (int)(b % g.nx), HALO + (int)(line % g.ny), m.lead + (int)(line / g.ny));
+#endif
+}
+
+__global__ void k_pack(Geom g, PackMap m, double* __restrict__ field, double* __restrict__ buf, long n)
+{
+    for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += (long)gridDim.x * blockDim.x) {
+        const long i = pack_cell(g, m, b);
+        if (m.pack) buf[b] = field[i];
+        else field[i] = buf[b];
+    }
+}
+
+void op_pack_message(Chunk& c, const int* fields, const int* offsets, int depth, int face, double* buffer, bool pack)
+{
+    const Geom& g = c.g;
+    if (depth < 1 || depth > HALO) throw Error("halo message depth must be 1 or 2");
+    c.internal_faces = true;
+    PackMap m{face, depth, pack ? 1 : 0, 0};
+    long n = 0;
+    const bool lo = face == CHUNK_LEFT || face == CHUNK_BOTTOM || face == CHUNK_BACK;
+    const int ext = face == CHUNK_LEFT || face == CHUNK_RIGHT ? g.X : face == CHUNK_BOTTOM || face == CHUNK_TOP ? g.Y : g.Z;
+    switch (face) {
+    case CHUNK_LEFT:
+    case CHUNK_RIGHT: n = (long)g.ny * g.nz * depth; break;
+    case CHUNK_BOTTOM:
+    case CHUNK_TOP: n = (long)g.nx * g.nz * depth; break;
+    case CHUNK_BACK:
+    case CHUNK_FRONT:
+        if (TL_DIM == 3) {
+            n = (long)g.nx * g.ny * depth;
+            break;
+        }
+        [[fallthrough]];
+    default: throw Error("halo message: incorrect face " + std::to_string(face));
+    }
+    // the mesh strip next to the face (pack) or the halo strip beyond it (unpack)
+    m.lead = pack ? (lo ? HALO : ext - HALO - depth) : (lo ? HALO - depth : ext - HALO);
+    static const int field_of[NUM_FIELDS] = {F_DENSITY, F_ENERGY0, F_ENERGY1, F_U, F_P, F_SD};
+    if (n > c.pack_cap) {
+        c.sync();
+        (void)hipFree(c.d_pack);
+        c.d_pack = nullptr;
+        TL_HIP(hipMalloc(&c.d_pack, sizeof(double) * n));
+        c.pack_cap = n;
+    }
+    for (int q = 0; q < NUM_FIELDS; ++q) {
+        if (!fields[q]) continue;
+        double* host = buffer + offsets[q];
+        if (!pack) TL_HIP(hipMemcpyAsync(c.d_pack, host, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, c.stream, g, m,
+                           c.f[field_of[q]], c.d_pack, n);
+        check_launch();
+        if (pack) TL_HIP(hipMemcpyAsync(host, c.d_pack, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    }
+}
+
 void op_update_halo(Chunk& c, const int* neighbours, const int* fields, int depth)
 {
     if (depth < 1 || depth > HALO) throw Error("halo depth must be 1 or 2");
     int mask = 0;
     for (int f = 0; f < NUM_FACES; ++f) {
-        if (neighbours[f] != EXTERNAL_FACE) continue;
+        if (neighbours[f] != EXTERNAL_FACE) {
+            c.internal_faces = true;
+            continue;
+        }
         switch (f + 1) {
         case CHUNK_LEFT: mask |= 1; break;
         case CHUNK_RIGHT: mask |= 2; break;

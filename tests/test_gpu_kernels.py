@@ -383,3 +383,46 @@ def test_jacobi_iterate_fused(hip, oracle, shape, count):
     assert_same(g, o, ["u", "r"], region="interior")
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("depth", [1, 2])
+def test_pack_message(hip, oracle, dim, depth):
+    """ext_pack_message_: every face's message packed from and unpacked into
+    the flagged fields, against the oracle's restatement of the reference's
+    packers: buffers and fields bitwise."""
+    from test_oracle import pack_message
+    g, o = pair(hip, oracle, dim)
+    rng = np.random.default_rng(100 * dim + depth)
+    names = ["density", "energy0", "energy1", "u", "p", "sd"]
+    for face in range(1, 2 * dim + 1):
+        fields = [int(v) for v in rng.integers(0, 2, size=6)]
+        fields[face % 6] = 1
+        random_fields(g, o, names, rng, lo=-1.0, hi=1.0)
+        n = {1: g.ny * g.nz, 2: g.ny * g.nz, 3: g.nx * g.nz, 4: g.nx * g.nz, 5: g.nx * g.ny,
+             6: g.nx * g.ny}[face] * depth
+        offsets = [q * n + 3 for q in range(6)]
+        bufs = [np.full(6 * n + 3, -7.0) for _ in range(2)]
+        for ch, b in zip((g, o), bufs):
+            pack_message(ch, fields, offsets, depth, face, b, True)
+        np.testing.assert_array_equal(bufs[0], bufs[1])
+        msg = rng.uniform(-1, 1, size=6 * n + 3)
+        for ch in (g, o):
+            pack_message(ch, fields, offsets, depth, face, msg.copy(), False)
+        assert_same(g, o, names)
+    g.close()
+    o.close()
+
+
+def test_stencil_refused_with_neighbour_chunks(hip, oracle):
+    """The stencil kernels read reflective boundaries: once a chunk has a
+    neighbour chunk (a halo update naming one), stencil calls are refused
+    with an error instead of computing with the wrong boundary."""
+    g = hip.ExtChunk(hip.lib(2), 2, 37, 23)
+    mesh(g, 2)
+    g.update_halo([FIELD_ENERGY1, FIELD_DENSITY], 2)
+    g.cg_solver_init(1, False, 0.004)
+    g.update_halo([FIELD_P], 1, neighbours=[-1, 2, -1, -1])
+    with pytest.raises(RuntimeError, match="neighbour chunks"):
+        g.cg_calc_w()
+    g.close()

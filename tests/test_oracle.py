@@ -279,3 +279,76 @@ def test_chebyshev_coefficients_closed_form(oracle, eigmin, eigmax, n):
     for k in range(1, n + 1):
         assert a[k - 1] == pytest.approx(T[k - 1] / T[k + 1], rel=1e-10)
         assert b[k - 1] == pytest.approx(2 * T[k] / (delta * T[k + 1]), rel=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# halo messages (ext_pack_message_)
+
+def pack_regions(shape, dim, face, depth):
+    """The mesh strip a face's message packs and the halo strip it unpacks
+    into, as array slices of the (Z, Y, X) field whose row-major order is
+    the message's element order.  In 2D the reference's strips start at row
+    0 (left/right) and column 0 (top/bottom) rather than at the first mesh
+    row or column (tealeaf_2d/ext_pack_kernel.hpp:22-35, 70-83)."""
+    Z, Y, X = shape
+    d = depth
+    if dim == 2:
+        ny, nx = Y - 4, X - 4
+        rows, cols = slice(0, ny), slice(0, nx)
+        return {1: ((0, rows, slice(2, 2 + d)), (0, rows, slice(2 - d, 2))),
+                2: ((0, rows, slice(X - 2 - d, X - 2)), (0, rows, slice(X - 2, X - 2 + d))),
+                3: ((0, slice(2, 2 + d), cols), (0, slice(2 - d, 2), cols)),
+                4: ((0, slice(Y - 2 - d, Y - 2), cols), (0, slice(Y - 2, Y - 2 + d), cols))}[face]
+    inner = slice(2, -2)
+    return {1: ((inner, inner, slice(2, 2 + d)), (inner, inner, slice(2 - d, 2))),
+            2: ((inner, inner, slice(X - 2 - d, X - 2)), (inner, inner, slice(X - 2, X - 2 + d))),
+            3: ((inner, slice(2, 2 + d), inner), (inner, slice(2 - d, 2), inner)),
+            4: ((inner, slice(Y - 2 - d, Y - 2), inner), (inner, slice(Y - 2, Y - 2 + d), inner)),
+            5: ((slice(2, 2 + d), inner, inner), (slice(2 - d, 2), inner, inner)),
+            6: ((slice(Z - 2 - d, Z - 2), inner, inner), (slice(Z - 2, Z - 2 + d), inner, inner))}[face]
+
+
+def pack_message(ch, fields, offsets, depth, face, buf, pack):
+    import ctypes
+    fl = (ctypes.c_int * 6)(*fields)
+    off = (ctypes.c_int * 6)(*offsets)
+    i = lambda v: ctypes.byref(ctypes.c_int(v))
+    flag = ctypes.byref(ctypes.c_bool(pack))
+    ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    if ch.dim == 3:
+        ch.L.ext_pack_message_(ch.c, fl, off, i(depth), i(face), i(1), ptr, flag)
+    else:
+        ch.L.ext_pack_message_(ch.c, fl, off, i(depth), i(face), ptr, flag)
+    ch._check()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("depth", [1, 2])
+def test_pack_message_regions(hip, oracle, dim, depth):
+    """Each face's message holds the mesh strip next to the face in the
+    reference's order, and unpacking puts it in the halo strip beyond."""
+    n = (13, 9, 1) if dim == 2 else (7, 6, 5)
+    ch = _chunk(hip, oracle, dim, n)
+    rng = np.random.default_rng(dim * 10 + depth)
+    names = ["density", "u", "sd"]
+    fields = [1, 0, 0, 1, 0, 1]
+    for face in range(1, 2 * dim + 1):
+        src, dst = pack_regions(ch.shape, dim, face, depth)
+        vals = {k: rng.uniform(-1, 1, size=ch.shape) for k in names}
+        for k in names:
+            ch.set(k, vals[k])
+        m = vals["u"][src].size
+        offsets = [0, 0, 0, m, 0, 2 * m]
+        buf = np.zeros(3 * m)
+        pack_message(ch, fields, offsets, depth, face, buf, True)
+        for q, k in enumerate(names):
+            np.testing.assert_array_equal(buf[q * m:(q + 1) * m], vals[k][src].reshape(-1))
+        for k in names:
+            ch.set(k, np.zeros(ch.shape))
+        pack_message(ch, fields, offsets, depth, face, buf, False)
+        for k in names:
+            got = ch.get(k)
+            want = np.zeros(ch.shape)
+            want[dst] = vals[k][src]
+            np.testing.assert_array_equal(got, want)
+    ch.close()
