@@ -274,6 +274,8 @@ def learned_workload(args, n: int) -> dict:
     t_train = time.perf_counter() - t0
     fwd_ms, macs = learner.time_forward(1, n, n, reps=3)
     tflops = 2.0 * macs / (fwd_ms * 1e-3) / 1e12
+    fwd_bytes = learner.forward_bytes(1, n, n)
+    fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
 
     steps = args.warmup + args.steps
     per_seed, fcg_s, ref_s, sweep_s = [], [], [], []
@@ -316,7 +318,9 @@ def learned_workload(args, n: int) -> dict:
                          "train_s": round(t_train, 2)},
             "unet_roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                               "frac": round(tflops / 2500.0, 5), "forward_ms": round(fwd_ms, 3),
-                              "macs_per_forward": macs, "dtype": "bf16 (fp32 accumulate)"},
+                              "macs_per_forward": macs, "dtype": "bf16 (fp32 accumulate)",
+                              "bytes_per_forward": fwd_bytes, "hbm_achieved_gbs": round(fwd_gbs, 1),
+                              "hbm_frac": round(fwd_gbs / HBM_PEAK_GBS, 4)},
         },
     }
 

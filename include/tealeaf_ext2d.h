@@ -197,6 +197,10 @@ int tl_learn_apply(void* learner, const float* r, const float* f, float* z, int 
 /* Mean milliseconds of one forward pass on device-resident n x h x w planes
  * over `reps` timed passes (HIP events); *macs = its multiply-accumulates. */
 double tl_learn_time_forward(void* learner, int n, int h, int w, int reps, double* macs);
+/* Bytes one forward pass on n x h x w planes moves, each tensor read or
+ * written once as stored (BF16 activations with padded channels, FP32 planes
+ * in and out): the HBM side of the network's roofline. */
+double tl_learn_forward_bytes(int n, int h, int w);
 
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);

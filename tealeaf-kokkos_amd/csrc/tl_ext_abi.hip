@@ -338,6 +338,10 @@ double tl_learn_time_forward(void* h, int n, int hh, int ww, int reps, double* m
     return ms;
 }
 
+// Bytes one network forward pass on n x h x w planes moves, each tensor read
+// or written once as stored (BF16 activations, FP32 planes in and out).
+double tl_learn_forward_bytes(int n, int hh, int ww) { return unet_forward_bytes(n, hh, ww); }
+
 // Bandwidth of a plain device copy of n doubles repeated reps times (GB/s,
 // read + write): the machine's attainable streaming rate, and the known-size
 // kernel that calibrates HBM byte counters.  Returns a negative value on error.

@@ -917,6 +917,28 @@ double unet_forward_macs(int n, int h, int w)
     }
     return macs;
 }
+double unet_forward_bytes(int n, int h, int w)
+{
+    // every tensor of the forward pass read or written once, as stored
+    // (BF16 activations with their padded channel counts, FP32 planes in and out)
+    static const int level[UNet::NL] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    const double p0 = (double)n * h * w;
+    auto px = [&](int lev) { return p0 / (double)(1 << (2 * lev)); };
+    double b = p0 * (8.0 + 2.0 * kShapes[0][0]);  // r, f (FP32) -> input channels
+    for (int l = 0; l < UNet::NL; ++l)
+        b += 2.0 * px(level[l]) * (kShapes[l][0] + kShapes[l][1]) + 2.0 * 9.0 * kShapes[l][0] * kShapes[l][1];
+    const int pool_c[3] = {16, 32, 64};  // 2x2 average pooling after levels 0, 1, 2
+    for (int lev = 0; lev < 3; ++lev) b += 2.0 * pool_c[lev] * (px(lev) + px(lev + 1));
+    // up-sampling with concatenation into levels 2, 1, 0: (up, skip) channels
+    const int up_c[3][2] = {{128, 64}, {64, 32}, {32, 16}};
+    for (int k = 0; k < 3; ++k) {
+        const int lev = 2 - k;
+        b += 2.0 * (px(lev + 1) * up_c[k][0] + px(lev) * up_c[k][1] + px(lev) * (up_c[k][0] + up_c[k][1]));
+    }
+    b += p0 * (2.0 + 4.0);  // output channel -> FP32 plane
+    return b;
+}
+
 hipStream_t unet_stream(UNet& u) { return u.stream; }
 void unet_set_stream(UNet& u, hipStream_t s) { u.stream = s; }
 long unet_param_count(const UNet& u) { return u.param_count(); }

@@ -86,6 +86,7 @@ def lib(dim: int) -> ctypes.CDLL:
             "tl_learn_loss_grad": (ctypes.c_double, [vp, fp, fp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                      fp]),
             "tl_learn_apply": (ctypes.c_int, [vp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "tl_learn_forward_bytes": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "tl_learn_time_forward": (ctypes.c_double, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                         dp]),
         }
@@ -364,6 +365,10 @@ class Learner:
         if ms < 0:
             raise RuntimeError(self.L.tl_last_error().decode())
         return ms, macs.value
+
+    def forward_bytes(self, n: int, h: int, w: int) -> float:
+        """HBM bytes of one forward pass, each tensor read or written once."""
+        return self.L.tl_learn_forward_bytes(n, h, w)
 
     def close(self):
         if self.h:
