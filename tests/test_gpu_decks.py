@@ -63,6 +63,9 @@ CASES_2D = [
     # that is not a multiple of the pass
     ("jacobi", 40, 2, {"max_iters": 20000}), ("jacobi", 72, 1, {"max_iters": 1237, "ny": 51}),
     ("cg", 130, 2, {"extra": ("tl_coefficient_inverrse_density", "tl_check_result")}),
+    # meshes a few cells wide: every strip and tile is an edge one, reflections repeat
+    ("chebyshev", 3, 2, {"ny": 40, "presteps": 5}), ("ppcg", 2, 2, {"ny": 17, "presteps": 5}),
+    ("chebyshev", 17, 2, {"ny": 2, "presteps": 3}),
 ]
 
 
