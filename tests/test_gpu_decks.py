@@ -1,7 +1,7 @@
 """End-to-end parity: decks run by the C++ driver on the GPU against the
 oracle's restated Fortran driver on the CPU.
 
-Bar (from the north-star tolerance): every per-step field-summary total
+Bar: every per-step field-summary total
 (vol, mass, ie, temp) within 1e-11 relative and the iteration count of every
 timestep within +-1.  The QA decks additionally meet the reference's own
 test: final temperature within 0.001 % of the value the reference ships
