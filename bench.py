@@ -221,6 +221,26 @@ def roofline_of(rep: dict, n: int) -> dict | None:
             "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_launch": bytes_per_launch}
 
 
+def fcg_form(ref: dict, fcg: dict) -> dict:
+    """The flexible CG with Jacobi-sweep preconditioning (the learned form's
+    solver with the network off) against the FP64 PPCG solve of the same
+    deck: time per timestep, ratio, per-step field-summary difference from
+    the FP64 solve, iterations, true residual, fallbacks."""
+    per_step = []
+    for a, b in zip(fcg["timed"], ref["timed"]):
+        per_step.append({
+            "step": a["step"], "iters": a["fcg_iters"], "fp64_iters": b["iters"],
+            "fallback": a["fcg_fallback"], "true_rrn": a["exact_error"], "fp64_true_rrn": b["exact_error"],
+            "max_summary_rel_err": max(abs(a[k] - b[k]) / abs(b[k]) for k in ("vol", "mass", "ie", "temp")),
+        })
+    return {
+        "s_per_step": round(fcg["s_per_step"], 6),
+        "speedup_vs_fp64": round(ref["s_per_step"] / fcg["s_per_step"], 3),
+        "max_summary_rel_err": max(p["max_summary_rel_err"] for p in per_step),
+        "per_step": per_step,
+    }
+
+
 def mixed_form(ref: dict, mix: dict) -> dict:
     """The mixed-precision PPCG solve of the same deck against the FP64 one:
     time per timestep and its ratio, and per timestep the relative
@@ -344,6 +364,7 @@ def main() -> int:
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
     ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision PPCG solve")
+    ap.add_argument("--no-fcg", action="store_true", help="skip the flexible CG (Jacobi sweeps) solve")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -421,6 +442,13 @@ def main() -> int:
         mix = run_solve(mdeck, dim, args.warmup, only="ppcg_outer_mixed")
         mix["kernels_first_step"] = mfirst["kernels"]
         forms["mixed_ppcg"] = mixed_form(ref, mix)
+    if dim == 2 and args.solver == "ppcg" and not args.no_fcg:
+        # the flexible CG of the learned form with its network switched off:
+        # z = 4 weighted Jacobi sweeps on r, convergence on the true residual
+        print("bench: flexible CG with Jacobi sweeps on the same deck", file=sys.stderr, flush=True)
+        fdeck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network")
+        fcg = run_solve(fdeck, dim, args.warmup, timing=False)
+        forms["fcg_sweeps"] = fcg_form(ref, fcg)
 
     import ctypes
     L = tealeaf_hip.lib(2)

@@ -148,6 +148,13 @@ int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn);
  * with tl_last_error() set. */
 int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
 
+/* The flexible CG's preconditioner (no counterpart in the reference): z =
+ * `sweeps` weighted Jacobi sweeps z += omega (r - A z) / D on the chunk's z
+ * and r fields (from z = 0 with from_zero), as one temporal-blocked pass
+ * (fused = 1; 2D, 4 sweeps) or one launch per sweep (fused = 0).  Returns 0,
+ * or -1 with tl_last_error() set. */
+int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
+
 /* Mixed-precision form of tl_ppcg_outer_fused (no counterpart in the
  * reference): the nsteps inner steps in FP32 on the FP32 rounding of the
  * residual, u and r updated in FP64 (u += alpha p + sum sd, r = r' + the FP32

@@ -275,6 +275,11 @@ struct FcgOptions {
 };
 bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, int max_iters, int& iters,
                        double& error);
+// The flexible CG's preconditioner sweeps (weighted Jacobi on A z = r, into
+// F_Z): one temporal-blocked pass where supported and allowed.
+void fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused = true);
+bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);
+void op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps);
 
 // The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps
 // timesteps when that is positive; returns the JSON report.

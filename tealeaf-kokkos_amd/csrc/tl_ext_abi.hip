@@ -163,6 +163,23 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors)
     return rc;
 }
 
+// The flexible CG's preconditioner sweeps on a chunk built through the ABI:
+// z (the u0 field's neighbour in the chunk, F_Z) = `sweeps` weighted Jacobi
+// sweeps on A z = r from z, or from zero with from_zero; fused = 1 runs them
+// as one temporal-blocked pass, 0 as one launch per sweep.
+int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = stencil_chunk(chunk);
+        if (fused && !fcg_smooth_tb_supported(*c, sweeps)) throw Error("no temporal-blocked pass for these sweeps");
+        fcg_smooth(*c, sweeps, omega, from_zero != 0, fused != 0);
+        c->sync();
+        rc = 0;
+    });
+    return rc;
+}
+
 // The mixed-precision form of the same outer iteration: FP32 inner steps,
 // u and r updated in FP64 (2D only; nsteps with a compiled kernel).
 int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,

@@ -1,0 +1,292 @@
+// The flexible CG's preconditioner sweeps as one temporal-blocked pass (2D).
+//
+// The preconditioner applies K weighted Jacobi sweeps on A z = r,
+// z_(s) = z_(s-1) + omega (r - A z_(s-1)) / D, D the diagonal of A (1 plus
+// the coefficients of the cell's interior faces; a face on the mesh boundary
+// carries no flux), starting from z = 0 or from the network's output.  Sweep
+// s needs sweep s - 1 on the neighbouring rows only, so the wavefront of the
+// temporal-blocked Jacobi kernel (tl_jacobi_tb.hpp) runs all K sweeps in one
+// pass: sweep s at row y - s from a three-row register window of sweep s - 1,
+// r, kx and ky in LDS rings, 24 (32 with a network start) + 8 bytes per cell
+// instead of K x 40 and a clearing pass.
+//
+// Faces on the mesh boundary enter with a zero coefficient: the coefficient
+// rows are masked as they enter the rings, so every wave runs one code path
+// and a cell's sums are those of the one-sweep kernel, term for term in its
+// order (east, west, north, south), with + 0 for a boundary face where that
+// kernel skips the term.  Cells outside the mesh are computed from mirrored
+// loads and feed no cell inside it (their faces to it are boundary faces).
+#include "tl_ppcg_tb.hpp"
+
+#include <algorithm>
+#include <string>
+
+namespace tl {
+
+#if TL_DIM == 2
+namespace ftb {
+
+using tb::ConstU;
+using tb::fold_face;
+using tb::from_east;
+using tb::from_west;
+using tb::ld;
+using tb::ldp;
+using tb::pick;
+using tb::stp;
+
+struct FArgs {
+    const double* z_in;  // start (unread when the pass starts from zero)
+    double* z_out;
+    const double* r;
+    const double* kx;
+    const double* ky;
+    const uint2* rowtab;
+    int row0;
+    int ny;
+    double omega;
+    int tiles_x;
+    int rows;
+    long nblocks;
+};
+
+struct FRow {
+    double2 z, r, ky;
+    double kx0, kx1;
+    bool ky_in;  // the row's south face is inside the mesh
+};
+
+template <int K>
+struct FState {
+    double zw[K][3][2];  // z after sweep s, slot (round) % 3, column
+    FRow pf[2];
+};
+
+struct FCtx {
+    Geom g;
+    int ylo, Y0, yhi;
+    unsigned bcp, bf0, bf1, bsp, bpad;
+    bool hi0, hi1;
+    bool kw0, kw1;  // the lane's x faces (west of c0, west of c1) are inside the mesh
+    double omega;
+    double* rr;  // LDS rings [ring rows][64][2]
+    double* kxr;
+    double* kyr;
+};
+
+template <bool ZERO>
+__device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t, int PHk, int y, FRow& v)
+{
+    const unsigned oy = t[2 * PHk], of = t[2 * PHk + 1];
+    if (!ZERO) v.z = ldp(A.z_in, oy + c.bcp);
+    v.r = ldp(A.r, oy + c.bcp);
+    v.kx0 = ld(A.kx, oy + c.bf0);
+    v.kx1 = ld(A.kx, oy + c.bf1);
+    v.ky = ldp(A.ky, of + c.bcp);
+    // the south face of row y is a mesh face iff 3 <= y <= ny + 1 (rows
+    // outside the mesh mirror rows inside; their faces are never used)
+    v.ky_in = y > HALO && y <= A.ny + 1;
+}
+
+template <int K, bool ZERO, int PH>
+__device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
+{
+    using R = tb::TBRing<K>;
+    constexpr int S0 = PH % 3, S1 = (PH + 2) % 3, S2 = (PH + 1) % 3;  // rounds y, y-1, y-2
+    const int lane = threadIdx.x;
+    FRow& v = st.pf[PH % 2];
+    double2* rr = reinterpret_cast<double2*>(c.rr) + lane;
+    double2* kxr = reinterpret_cast<double2*>(c.kxr) + lane;
+    double2* kyr = reinterpret_cast<double2*>(c.kyr) + lane;
+    const double* kxe = c.kxr + 2 * lane + 2;
+
+    double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE], nr = rr[R::row(PH, 1) * WAVE];
+    double nkxe = kxe[2 * WAVE * R::row(PH, 1)];
+
+    // row y enters: sweep 0 (the start), r and the masked coefficients
+    st.zw[0][S0][0] = ZERO ? 0.0 : pick(v.z, c.hi0);
+    st.zw[0][S0][1] = ZERO ? 0.0 : pick(v.z, c.hi1);
+    rr[R::row(PH, 0) * WAVE] = make_double2(pick(v.r, c.hi0), pick(v.r, c.hi1));
+    kxr[R::row(PH, 0) * WAVE] = make_double2(c.kw0 ? v.kx0 : 0.0, c.kw1 ? v.kx1 : 0.0);
+    double kyn0 = v.ky_in ? pick(v.ky, c.hi0) : 0.0, kyn1 = v.ky_in ? pick(v.ky, c.hi1) : 0.0;
+    kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
+    f_load<ZERO>(A, c, tab, PH, y + 2, v);
+
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+        const int z = y - s;
+        const double2 kx = nkx, ky = nky, r = nr;
+        const double kxe1 = nkxe;
+        if (s < K) {
+            nkx = kxr[R::row(PH, s + 1) * WAVE];
+            nky = kyr[R::row(PH, s + 1) * WAVE];
+            nr = rr[R::row(PH, s + 1) * WAVE];
+            nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
+        }
+        const double c0 = st.zw[s - 1][S1][0], c1 = st.zw[s - 1][S1][1];
+        const double s0 = st.zw[s - 1][S2][0], s1 = st.zw[s - 1][S2][1];
+        const double n0 = st.zw[s - 1][S0][0], n1 = st.zw[s - 1][S0][1];
+        const double w0 = from_west(c1), e1 = from_east(c0);
+        const double e0 = c1, w1 = c0;
+        // the one-sweep kernel's order: east, west, north, south
+        double az0 = c0, az1 = c1, d0 = 1.0, d1 = 1.0;
+        az0 += kx.y * (c0 - e0);
+        d0 += kx.y;
+        az1 += kxe1 * (c1 - e1);
+        d1 += kxe1;
+        az0 += kx.x * (c0 - w0);
+        d0 += kx.x;
+        az1 += kx.y * (c1 - w1);
+        d1 += kx.y;
+        az0 += kyn0 * (c0 - n0);
+        d0 += kyn0;
+        az1 += kyn1 * (c1 - n1);
+        d1 += kyn1;
+        az0 += ky.x * (c0 - s0);
+        d0 += ky.x;
+        az1 += ky.y * (c1 - s1);
+        d1 += ky.y;
+        const double zn0 = c0 + c.omega * (r.x - az0) / d0;
+        const double zn1 = c1 + c.omega * (r.y - az1) / d1;
+        kyn0 = ky.x;
+        kyn1 = ky.y;
+        if (s < K) {
+            st.zw[s][S0][0] = zn0;
+            st.zw[s][S0][1] = zn1;
+        } else {
+            // every lane stores, lanes and rows that own no cell into a pad pair
+            const bool zin = z >= c.Y0 && z <= c.yhi;
+            const unsigned bz = zin ? c.bsp : c.bpad;
+            const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
+            stp(A.z_out, oz + bz, make_double2(zn0, zn1));
+        }
+    }
+}
+
+template <int K, bool ZERO, int PH>
+__device__ __forceinline__ void f_rounds(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
+{
+    if constexpr (PH < tb::TB_UNROLL) {
+        f_round<K, ZERO, PH>(A, c, st, y + PH, tab);
+        __builtin_amdgcn_sched_barrier(0);
+        f_rounds<K, ZERO, PH + 1>(A, c, st, y, tab);
+    }
+}
+
+template <int K, bool ZERO>
+__global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
+{
+    using R = tb::TBRing<K>;
+    __shared__ double2 r_ring[R::N * WAVE];
+    // one more element: the last lane's east face reads past the last row
+    __shared__ double2 kx_ring[R::N * WAVE + 1];
+    __shared__ double2 ky_ring[R::N * WAVE];
+    const int lane = threadIdx.x;
+    const long t = xcd_tile(blockIdx.x, A.nblocks);
+    const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
+    constexpr int KP = tb::tb_margin(K);
+    constexpr int OW = 2 * WAVE - 2 * KP;
+    FCtx c;
+    c.g = g;
+    const int X0 = HALO + tx * OW;
+    c.Y0 = HALO + ty * A.rows;
+    const int k0 = X0 - KP + 2 * lane, k1 = k0 + 1;
+    {
+        const tb::CellPair cp = tb::cell_pair(k0, g.nx);
+        c.bcp = 8u * (unsigned)cp.start;
+        c.hi0 = cp.hi0;
+        c.hi1 = cp.hi1;
+    }
+    c.bf0 = 8u * (unsigned)fold_face(k0, g.nx);
+    c.bf1 = 8u * (unsigned)fold_face(k1, g.nx);
+    // a face is a mesh face iff it lies strictly between two mesh cells
+    c.kw0 = k0 > HALO && k0 <= g.nx + 1;
+    c.kw1 = k1 > HALO && k1 <= g.nx + 1;
+    const bool out0 = k0 >= X0 && k0 < X0 + OW && k0 <= g.nx + 1;
+    {
+        // pad pairs past the halo are never read; a lane whose first cell is
+        // the last of an odd nx also writes the halo column nx + 2, which no
+        // kernel reads
+        const int xe = (int)g.X + ((int)g.X & 1);
+        const int npair = (int)(g.sx - xe) / 2;
+        c.bpad = 8u * (unsigned)(xe + 2 * (lane % npair));
+        c.bsp = out0 ? 8u * (unsigned)k0 : c.bpad;
+    }
+    c.ylo = c.Y0 - K;
+    c.yhi = min(c.Y0 + A.rows - 1, g.ny + 1);
+    c.omega = A.omega;
+    c.rr = reinterpret_cast<double*>(r_ring);
+    c.kxr = reinterpret_cast<double*>(kx_ring);
+    c.kyr = reinterpret_cast<double*>(ky_ring);
+
+    FState<K> st;
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) st.zw[s][0][q] = st.zw[s][1][q] = st.zw[s][2][q] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double2 z = make_double2(0.0, 0.0);
+        st.pf[k].z = st.pf[k].r = st.pf[k].ky = z;
+        st.pf[k].kx0 = st.pf[k].kx1 = 0.0;
+        st.pf[k].ky_in = false;
+    }
+    ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
+    f_load<ZERO>(A, c, tab, 0, c.ylo, st.pf[0]);
+    f_load<ZERO>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
+    __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
+    const int yend = c.yhi + K;
+    for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
+        f_rounds<K, ZERO, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+}
+
+}  // namespace ftb
+
+bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
+{
+    // the kernel addresses a field with 32-bit byte offsets
+    const bool fits = (double)c.g.alloc * sizeof(double) < 4294967296.0;
+    return sweeps == 4 && c.g.nx >= 2 && c.g.ny >= 2 && fits;
+}
+
+// z_out = `sweeps` weighted Jacobi sweeps on A z = r from z_in (zero when
+// z_in is null), in one pass.  z_out != z_in.
+void op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps)
+{
+    if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
+    ftb::FArgs a{};
+    a.z_in = z_in;
+    a.z_out = z_out;
+    a.r = c.f[F_R];
+    a.kx = c.f[F_KX];
+    a.ky = c.f[F_KY];
+    a.rowtab = tb_rowtab(c);
+    a.row0 = TB_ROWTAB0;
+    a.ny = c.g.ny;
+    a.omega = omega;
+    constexpr int K = 4;
+    const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
+    a.tiles_x = (c.g.nx + ow - 1) / ow;
+    a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds, tb::TB_UNROLL);
+    a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
+    if (z_in)
+        hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false>), dim3((unsigned)a.nblocks), dim3(WAVE), 0, c.stream, c.g,
+                           a);
+    else
+        hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true>), dim3((unsigned)a.nblocks), dim3(WAVE), 0, c.stream, c.g,
+                           a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+#else  // 3D keeps the one-sweep kernel (the flexible CG is 2D)
+
+bool fcg_smooth_tb_supported(const Chunk&, int) { return false; }
+void op_fcg_smooth_tb(Chunk&, const double*, double*, double, int)
+{
+    throw Error("temporal-blocked smoothing is implemented for 2D");
+}
+
+#endif
+
+}  // namespace tl

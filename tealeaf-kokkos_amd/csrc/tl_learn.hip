@@ -186,6 +186,35 @@ __global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double
 
 unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, std::min<long>((n + 255) / 256, 16384)); }
 
+}  // namespace
+
+// z = `sweeps` weighted Jacobi sweeps on A z = r from z (from zero with
+// from_zero): one temporal-blocked pass where it applies, else one launch
+// per sweep.  The result is left in F_Z.
+void fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused)
+{
+    if (sweeps < 1) {
+        if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
+        return;
+    }
+    if (allow_fused && fcg_smooth_tb_supported(c, sweeps)) {
+        op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], omega, sweeps);
+        std::swap(c.f[F_Z], c.f[F_SD]);
+        return;
+    }
+    if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
+    const long n = (long)c.g.nx * c.g.ny * c.g.nz;
+    for (int k = 0; k < sweeps; ++k) {
+        hipLaunchKernelGGL(k_fcg_smooth, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_KX], c.f[F_KY],
+                           c.f[F_R], c.f[F_Z], omega, c.f[F_SD]);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw Error(std::string("FCG smoothing: ") + hipGetErrorString(e));
+        std::swap(c.f[F_Z], c.f[F_SD]);
+    }
+}
+
+namespace {
+
 void launched(const char* what)
 {
     const hipError_t e = hipGetLastError();
@@ -423,14 +452,9 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             unet_apply(*L->net, L->pr, pf, pz, 1, c.g.ny, c.g.nx);
             hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s, c.f[F_Z]);
             launched("FCG output");
+            fcg_smooth(c, opt.smooth, opt.omega, false);
         } else {
-            TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
-        }
-        for (int k = 0; k < opt.smooth; ++k) {
-            hipLaunchKernelGGL(k_fcg_smooth, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_KX], c.f[F_KY],
-                               c.f[F_R], c.f[F_Z], opt.omega, c.f[F_SD]);
-            launched("FCG smoothing");
-            std::swap(c.f[F_Z], c.f[F_SD]);
+            fcg_smooth(c, opt.smooth, opt.omega, true);
         }
         double beta = 0.0;
         if (!first) beta = -op_dot(c, c.f[F_Z], c.f[F_W]) / pw_prev;

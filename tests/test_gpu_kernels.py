@@ -426,3 +426,28 @@ def test_stencil_refused_with_neighbour_chunks(hip, oracle):
     with pytest.raises(RuntimeError, match="neighbour chunks"):
         g.cg_calc_w()
     g.close()
+
+
+@pytest.mark.parametrize("shape", [(37, 23), (5, 4), (9, 7), (131, 6), (130, 41), (2, 17), (64, 64)])
+@pytest.mark.parametrize("from_zero", [True, False])
+def test_fcg_smooth_fused(hip, shape, from_zero):
+    """The flexible CG's four weighted Jacobi sweeps as one temporal-blocked
+    pass against one launch per sweep: z equal on every mesh cell (the pass
+    adds + 0 for a boundary face where the one-sweep kernel skips the term,
+    which can only change the sign of a zero)."""
+    import ctypes
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 131 + ny + from_zero)
+    vals = {"r": rng.uniform(-1, 1, size=(1, ny + 4, nx + 4)), "z": rng.uniform(-1, 1, size=(1, ny + 4, nx + 4)),
+            "kx": rng.uniform(0.1, 40.0, size=(1, ny + 4, nx + 4)), "ky": rng.uniform(0.1, 40.0, size=(1, ny + 4, nx + 4))}
+    out = []
+    for fused in (0, 1):
+        g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+        for k, v in vals.items():
+            g.set(k, v)
+        rc = g.L.tl_fcg_smooth(g.c, ctypes.c_int(4), ctypes.c_double(0.8), ctypes.c_int(int(from_zero)),
+                               ctypes.c_int(fused))
+        assert rc == 0, g.L.tl_last_error()
+        out.append(interior(g, g.get("z")))
+        g.close()
+    np.testing.assert_array_equal(out[0], out[1])
