@@ -249,6 +249,12 @@ void op_jacobi_tb_undo(Chunk& c);
 // Flexible CG (tl_learn.hip): a . b over the interior; p = z + beta p.
 double op_dot(Chunk& c, const double* a, const double* b);
 void op_fcg_update_p(Chunk& c, double beta);
+// One flexible-CG step after the preconditioner, its scalars on the device:
+// beta = -(z.w)/pw_prev (not on the first), p = z + beta p, w = A p, alpha =
+// (p.r)/(p.w), u += alpha p, r -= alpha w; result(S_RRN, 0) is ||r||^2 and
+// result(S_RRN, 1) is 1 when alpha could not be formed (p.w <= 0 or a
+// non-finite p.r).
+void op_fcg_iteration_dev(Chunk& c, bool first);
 
 // Observers of the driver's FP64 PPCG solve (no counterpart in the
 // reference): after every outer iteration (u, r current) and at the end of

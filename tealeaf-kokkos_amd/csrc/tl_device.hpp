@@ -132,7 +132,7 @@ struct Reduce {
 struct DevScalars {
     double rro, pw, alpha, rrn, beta, error;
     double rro_prev;  // rro before the last rrn epilogue (a replaced residual recomputes beta from it)
-    double pad;
+    double bad;       // flexible CG: 1 when the last alpha had p.w <= 0 or a non-finite p.r
 };
 
 template <int NV>

@@ -420,15 +420,18 @@ struct PWStrip {
     long nwaves;
 };
 
-template <bool U>
+// With PR (the flexible CG), p' . r is summed as well (a second partial),
+// r read beside the other inputs.
+template <bool U, bool PR = false>
 __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k, const double* __restrict__ p,
                                                        const double* __restrict__ q,
                                                        const DevScalars* __restrict__ ds, double* __restrict__ p_out,
-                                                       double* __restrict__ w, double* __restrict__ u, Reduce red)
+                                                       double* __restrict__ w, double* __restrict__ u, Reduce red,
+                                                       const double* __restrict__ rin = nullptr)
 {
     const int lane = threadIdx.x % WAVE;
     const long wv = (long)blockIdx.x * WAVES_PER_BLOCK + threadIdx.x / WAVE;
-    double acc = 0.0;
+    double acc = 0.0, acc_pr = 0.0;
     if (wv < t.nwaves) {
         const long tw = xcd_tile(wv, t.nwaves);
         const int tx = (int)(tw % t.tiles_x), ty = (int)(tw / t.tiles_x);
@@ -468,6 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
         double c0v = t2.x, c1v = t2.y;
         double2 pold_c = raw;  // old p of rows j and j + 1, u of row j
         double2 uc = U ? ld2(u, gidx(g, k0, j0, 0)) : make_double2(0.0, 0.0);
+        double2 rc = PR ? ld2(rin, gidx(g, k0, j0, 0)) : make_double2(0.0, 0.0);
         double ec = edge ? pnew(ce, j0) : 0.0;
         if (j0 == HALO) {  // bottom face: the reflective mirror
             s0 = c0v;
@@ -502,6 +506,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             const double kxq0 = kxq.x, kxq1 = kxq.y;
             const double kxql = lane == WAVE - 1 ? k.kx[gidx(g, min(c1 + 1, nx + 2), j1k, 0)] : 0.0;
             const double2 un = U ? ld2(u, gidx(g, k0, j1k, 0)) : make_double2(0.0, 0.0);
+            const double2 rn = PR ? ld2(rin, gidx(g, k0, j1k, 0)) : make_double2(0.0, 0.0);
 
             double nn0 = n0, nn1 = n1;
             if (j == ny + 1) {  // top face
@@ -530,15 +535,21 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
                 if (U) *reinterpret_cast<double2*>(u + i0) = uo;
                 acc += m0 * c0v;
                 acc += m1 * c1v;
+                if (PR) {
+                    acc_pr += c0v * rc.x;
+                    acc_pr += c1v * rc.y;
+                }
             } else if (v0) {
                 p_out[i0] = c0v;
                 w[i0] = m0;
                 if (U) u[i0] = uo.x;
                 acc += m0 * c0v;
+                if (PR) acc_pr += c0v * rc.x;
             }
             pold_c = pold_n;
             pold_n = pp;
             uc = un;
+            rc = rn;
             s0 = c0v;
             s1 = c1v;
             c0v = n0;
@@ -557,8 +568,13 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
             kyn1 = kyq1;
         }
     }
-    double v[1] = {acc};
-    block_partial<1>(v, red);
+    if (PR) {
+        double v[2] = {acc, acc_pr};
+        block_partial<2>(v, red);
+    } else {
+        double v[1] = {acc};
+        block_partial<1>(v, red);
+    }
 }
 #endif
 
@@ -618,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, cons
 // Second phase of every reduction: one workgroup adds the block partials in
 // block order and applies the scalar recurrence of tea_solve.f90 that
 // consumes the total, so the next kernel can read it from `ds`.
-enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE };
+enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE, EP_FCG_BETA, EP_FCG_PW, EP_FCG_ALPHA, EP_FCG_PWPR, EP_FCG_RRN };
 
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
@@ -664,6 +680,24 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
         out[1] = beta;
         break;
     }
+    case EP_FCG_BETA:  // flexible CG: beta = -(z . w_prev) / (p . w)_prev
+        ds->beta = -v[0] / ds->pw;
+        break;
+    case EP_FCG_PW:  // flexible CG: p . w
+        ds->pw = v[0];
+        break;
+    case EP_FCG_ALPHA:  // flexible CG: alpha = (p . r) / (p . w); a pw <= 0 or non-finite p.r stops the solve
+        ds->alpha = v[0] / ds->pw;
+        ds->bad = (ds->pw > 0.0 && isfinite(v[0])) ? 0.0 : 1.0;
+        break;
+    case EP_FCG_PWPR:  // flexible CG: p . w and p . r in one reduction -> alpha
+        ds->pw = v[0];
+        ds->alpha = v[NV - 1] / v[0];
+        ds->bad = (v[0] > 0.0 && isfinite(v[NV - 1])) ? 0.0 : 1.0;
+        break;
+    case EP_FCG_RRN:  // flexible CG: rrn, and the stop flag of this iteration's alpha
+        out[1] = ds->bad;
+        break;
     case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
         const double beta = v[0] / ds->rro_prev;
         ds->rrn = v[0];
@@ -881,11 +915,13 @@ __global__ __launch_bounds__(BLOCK) void k_dot(Geom g, const double* __restrict_
     block_partial<1>(v, red);
 }
 
-// p = z + beta p over the interior (flexible CG direction)
-__global__ __launch_bounds__(BLOCK) void k_fcg_p(Geom g, double beta, const double* __restrict__ z,
-                                                 double* __restrict__ p)
+// p = z + beta p over the interior (flexible CG direction); beta from the
+// device scalars when ds is given (0 on the first iteration)
+__global__ __launch_bounds__(BLOCK) void k_fcg_p(Geom g, double beta_v, const DevScalars* __restrict__ ds,
+                                                 int first, const double* __restrict__ z, double* __restrict__ p)
 {
     Strip s = make_strip(g);
+    const double beta = ds ? (first ? 0.0 : ds->beta) : beta_v;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) p[s.i] = z[s.i] + beta * p[s.i];
 }
 
@@ -1761,7 +1797,46 @@ double op_dot(Chunk& c, const double* a, const double* b)
     return c.result(S_NORM);
 }
 
-void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, c.f[F_Z], c.f[F_P]); }
+void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, nullptr, 0, c.f[F_Z], c.f[F_P]); }
+
+// The flexible CG's scalars on the device: one host read per iteration.
+void op_fcg_iteration_dev(Chunk& c, bool first)
+{
+    if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
+        STRIP_LAUNCH(k_dot, c, c.f[F_Z], c.f[F_W], c.reducer());
+        finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_BETA);
+    }
+#if TL_DIM == 2
+    if (!first) {
+        // p = z + beta p, w = A p, p.w and p.r in one streaming sweep
+        PWStrip t;
+        t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
+        t.rows = 64;
+        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
+        if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
+        t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
+        const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        TIMED(c, K_CG_WP, {
+            hipLaunchKernelGGL((k_cg_pw_strip<false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
+                               kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(), c.f[F_R]);
+            check_launch();
+        });
+        std::swap(c.f[F_P], c.p_alt);
+        finish<2>(c, nb, S_PW, EP_FCG_PWPR);
+    } else
+#endif
+    {
+        STRIP_LAUNCH(k_fcg_p, c, 0.0, c.ds, first ? 1 : 0, c.f[F_Z], c.f[F_P]);
+        TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
+                                      c.reducer()));
+        finish<1>(c, c.g.nblocks, S_PW, EP_FCG_PW);
+        STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer());
+        finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_ALPHA);
+    }
+    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R],
+                                   c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    finish<1>(c, c.g.nblocks, S_RRN, EP_FCG_RRN);
+}
 
 double op_calc_2norm(Chunk& c, bool norm_r)
 {

@@ -438,7 +438,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         error = rrn;
         return true;
     }
-    double pw_prev = 0.0, best = rrn;
+    double best = rrn;
     int since_best = 0;
     const int window = 200;
     bool first = true;
@@ -456,15 +456,11 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         } else {
             fcg_smooth(c, opt.smooth, opt.omega, true);
         }
-        double beta = 0.0;
-        if (!first) beta = -op_dot(c, c.f[F_Z], c.f[F_W]) / pw_prev;
-        op_fcg_update_p(c, beta);
+        // beta, p, w, alpha, u, r with the scalars on the device; one read
+        op_fcg_iteration_dev(c, first);
         first = false;
-        const double pw = op_cg_calc_w(c);
-        const double pr = op_dot(c, c.f[F_P], c.f[F_R]);
-        if (!(pw > 0.0) || !std::isfinite(pr)) return false;
-        rrn = op_cg_calc_ur(c, pr / pw);
-        pw_prev = pw;
+        rrn = c.result(S_RRN, 0);
+        if (c.result(S_RRN, 1) != 0.0) return false;
         iters = it;
         if (!std::isfinite(rrn)) return false;
         if (rrn < eps) {
