@@ -254,7 +254,9 @@ void op_fcg_update_p(Chunk& c, double beta);
 // (p.r)/(p.w), u += alpha p, r -= alpha w; result(S_RRN, 0) is ||r||^2 and
 // result(S_RRN, 1) is 1 when alpha could not be formed (p.w <= 0 or a
 // non-finite p.r).
-void op_fcg_iteration_dev(Chunk& c, bool first);
+// beta_blocks > 0: z . w is already in the reduction partials (from the
+// preconditioner pass), with that many blocks.
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks = 0);
 
 // Observers of the driver's FP64 PPCG solve (no counterpart in the
 // reference): after every outer iteration (u, r current) and at the end of
@@ -283,9 +285,12 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
                        double& error);
 // The flexible CG's preconditioner sweeps (weighted Jacobi on A z = r, into
 // F_Z): one temporal-blocked pass where supported and allowed.
-void fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused = true);
+// With dot_w the pass also leaves the first phase of z . w in the reduction
+// partials and returns their count (else 0).
+long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused = true, bool dot_w = false);
 bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);
-void op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps);
+long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps,
+                      const double* w = nullptr);
 
 // The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps
 // timesteps when that is positive; returns the JSON report.

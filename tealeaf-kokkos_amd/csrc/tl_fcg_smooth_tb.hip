@@ -38,6 +38,8 @@ using tb::stp;
 struct FArgs {
     const double* z_in;  // start (unread when the pass starts from zero)
     double* z_out;
+    const double* w;     // with DOTW: the previous direction's A p, for z . w
+    double* partials;
     const double* r;
     const double* kx;
     const double* ky;
@@ -52,6 +54,7 @@ struct FArgs {
 
 struct FRow {
     double2 z, r, ky;
+    double2 wo;  // w of the output row two rounds on (DOTW)
     double kx0, kx1;
     bool ky_in;  // the row's south face is inside the mesh
 };
@@ -60,6 +63,7 @@ template <int K>
 struct FState {
     double zw[K][3][2];  // z after sweep s, slot (round) % 3, column
     FRow pf[2];
+    double acc;          // z . w over the wave's own cells (DOTW)
 };
 
 struct FCtx {
@@ -68,17 +72,21 @@ struct FCtx {
     unsigned bcp, bf0, bf1, bsp, bpad;
     bool hi0, hi1;
     bool kw0, kw1;  // the lane's x faces (west of c0, west of c1) are inside the mesh
+    bool out0, out1;
     double omega;
     double* rr;  // LDS rings [ring rows][64][2]
     double* kxr;
     double* kyr;
 };
 
-template <bool ZERO>
+template <int K, bool ZERO, bool DOTW>
 __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t, int PHk, int y, FRow& v)
 {
     const unsigned oy = t[2 * PHk], of = t[2 * PHk + 1];
     if (!ZERO) v.z = ldp(A.z_in, oy + c.bcp);
+    // the row that leaves the pass when row y enters it (slack rows hold
+    // those past the mesh); lanes without output read a pad pair
+    if (DOTW) v.wo = ldp(A.w, 8u * (unsigned)(c.g.base + (long)(y - K) * c.g.sx) + c.bsp);
     v.r = ldp(A.r, oy + c.bcp);
     v.kx0 = ld(A.kx, oy + c.bf0);
     v.kx1 = ld(A.kx, oy + c.bf1);
@@ -88,7 +96,7 @@ __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t,
     v.ky_in = y > HALO && y <= A.ny + 1;
 }
 
-template <int K, bool ZERO, int PH>
+template <int K, bool ZERO, bool DOTW, int PH>
 __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     using R = tb::TBRing<K>;
@@ -110,7 +118,8 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     kxr[R::row(PH, 0) * WAVE] = make_double2(c.kw0 ? v.kx0 : 0.0, c.kw1 ? v.kx1 : 0.0);
     double kyn0 = v.ky_in ? pick(v.ky, c.hi0) : 0.0, kyn1 = v.ky_in ? pick(v.ky, c.hi1) : 0.0;
     kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
-    f_load<ZERO>(A, c, tab, PH, y + 2, v);
+    const double2 wo = v.wo;  // w of this round's output row
+    f_load<K, ZERO, DOTW>(A, c, tab, PH, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
@@ -159,21 +168,25 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
             const unsigned bz = zin ? c.bsp : c.bpad;
             const unsigned oz = 8u * (unsigned)(c.g.base + (long)z * c.g.sx);
             stp(A.z_out, oz + bz, make_double2(zn0, zn1));
+            if (DOTW) {
+                st.acc += zin && c.out0 ? zn0 * wo.x : 0.0;
+                st.acc += zin && c.out1 ? zn1 * wo.y : 0.0;
+            }
         }
     }
 }
 
-template <int K, bool ZERO, int PH>
+template <int K, bool ZERO, bool DOTW, int PH>
 __device__ __forceinline__ void f_rounds(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     if constexpr (PH < tb::TB_UNROLL) {
-        f_round<K, ZERO, PH>(A, c, st, y + PH, tab);
+        f_round<K, ZERO, DOTW, PH>(A, c, st, y + PH, tab);
         __builtin_amdgcn_sched_barrier(0);
-        f_rounds<K, ZERO, PH + 1>(A, c, st, y, tab);
+        f_rounds<K, ZERO, DOTW, PH + 1>(A, c, st, y, tab);
     }
 }
 
-template <int K, bool ZERO>
+template <int K, bool ZERO, bool DOTW>
 __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
 {
     using R = tb::TBRing<K>;
@@ -203,6 +216,8 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     c.kw0 = k0 > HALO && k0 <= g.nx + 1;
     c.kw1 = k1 > HALO && k1 <= g.nx + 1;
     const bool out0 = k0 >= X0 && k0 < X0 + OW && k0 <= g.nx + 1;
+    c.out0 = out0;
+    c.out1 = k1 >= X0 && k1 < X0 + OW && k1 <= g.nx + 1;
     {
         // pad pairs past the halo are never read; a lane whose first cell is
         // the last of an odd nx also writes the halo column nx + 2, which no
@@ -227,17 +242,22 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const double2 z = make_double2(0.0, 0.0);
-        st.pf[k].z = st.pf[k].r = st.pf[k].ky = z;
+        st.pf[k].z = st.pf[k].r = st.pf[k].ky = st.pf[k].wo = z;
         st.pf[k].kx0 = st.pf[k].kx1 = 0.0;
         st.pf[k].ky_in = false;
     }
+    st.acc = 0.0;
     ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
-    f_load<ZERO>(A, c, tab, 0, c.ylo, st.pf[0]);
-    f_load<ZERO>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
+    f_load<K, ZERO, DOTW>(A, c, tab, 0, c.ylo, st.pf[0]);
+    f_load<K, ZERO, DOTW>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
     __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
     const int yend = c.yhi + K;
     for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
-        f_rounds<K, ZERO, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+        f_rounds<K, ZERO, DOTW, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+    if (DOTW) {
+        double v[1] = {st.acc};
+        block_partial<1, 1>(v, Reduce{A.partials});
+    }
 }
 
 }  // namespace ftb
@@ -250,13 +270,17 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 }
 
 // z_out = `sweeps` weighted Jacobi sweeps on A z = r from z_in (zero when
-// z_in is null), in one pass.  z_out != z_in.
-void op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps)
+// z_in is null), in one pass.  z_out != z_in.  With w, the pass also leaves
+// the first phase of z_out . w in the reduction partials (returns their
+// count, else 0).
+long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps, const double* w)
 {
     if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
     ftb::FArgs a{};
     a.z_in = z_in;
     a.z_out = z_out;
+    a.w = w;
+    a.partials = c.partials;
     a.r = c.f[F_R];
     a.kx = c.f[F_KX];
     a.ky = c.f[F_KY];
@@ -269,20 +293,21 @@ void op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
     a.tiles_x = (c.g.nx + ow - 1) / ow;
     a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds, tb::TB_UNROLL);
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
-    if (z_in)
-        hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false>), dim3((unsigned)a.nblocks), dim3(WAVE), 0, c.stream, c.g,
-                           a);
-    else
-        hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true>), dim3((unsigned)a.nblocks), dim3(WAVE), 0, c.stream, c.g,
-                           a);
+    if (w && a.nblocks > c.partials_cap) throw Error("temporal-blocked smoothing: more waves than reduction partials");
+    const dim3 grid((unsigned)a.nblocks), block(WAVE);
+    if (z_in && w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true>), grid, block, 0, c.stream, c.g, a);
+    else if (z_in) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false>), grid, block, 0, c.stream, c.g, a);
+    else if (w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true>), grid, block, 0, c.stream, c.g, a);
+    else hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false>), grid, block, 0, c.stream, c.g, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
+    return w ? a.nblocks : 0;
 }
 
 #else  // 3D keeps the one-sweep kernel (the flexible CG is 2D)
 
 bool fcg_smooth_tb_supported(const Chunk&, int) { return false; }
-void op_fcg_smooth_tb(Chunk&, const double*, double*, double, int)
+long op_fcg_smooth_tb(Chunk&, const double*, double*, double, int, const double*)
 {
     throw Error("temporal-blocked smoothing is implemented for 2D");
 }

@@ -1800,9 +1800,11 @@ double op_dot(Chunk& c, const double* a, const double* b)
 void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, nullptr, 0, c.f[F_Z], c.f[F_P]); }
 
 // The flexible CG's scalars on the device: one host read per iteration.
-void op_fcg_iteration_dev(Chunk& c, bool first)
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks)
 {
-    if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
+    if (!first && beta_blocks > 0) {
+        finish<1>(c, beta_blocks, S_ERR, EP_FCG_BETA);
+    } else if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
         STRIP_LAUNCH(k_dot, c, c.f[F_Z], c.f[F_W], c.reducer());
         finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_BETA);
     }

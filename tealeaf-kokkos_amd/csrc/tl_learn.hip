@@ -191,16 +191,17 @@ unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, std::min<long>(
 // z = `sweeps` weighted Jacobi sweeps on A z = r from z (from zero with
 // from_zero): one temporal-blocked pass where it applies, else one launch
 // per sweep.  The result is left in F_Z.
-void fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused)
+long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused, bool dot_w)
 {
     if (sweeps < 1) {
         if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
-        return;
+        return 0;
     }
     if (allow_fused && fcg_smooth_tb_supported(c, sweeps)) {
-        op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], omega, sweeps);
+        const long nb = op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], omega, sweeps,
+                                         dot_w ? c.f[F_W] : nullptr);
         std::swap(c.f[F_Z], c.f[F_SD]);
-        return;
+        return nb;
     }
     if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
     const long n = (long)c.g.nx * c.g.ny * c.g.nz;
@@ -211,6 +212,7 @@ void fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_f
         if (e != hipSuccess) throw Error(std::string("FCG smoothing: ") + hipGetErrorString(e));
         std::swap(c.f[F_Z], c.f[F_SD]);
     }
+    return 0;
 }
 
 namespace {
@@ -445,6 +447,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     for (int it = 1; it <= max_iters; ++it) {
         // z = s N(r / s), then weighted Jacobi sweeps on A z = r from it
         const double s = std::sqrt(rrn / cells);
+        long beta_blocks = 0;  // z . w left by the preconditioner pass
         if (opt.network && (it - 1) % std::max(1, opt.net_every) == 0) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
@@ -452,12 +455,12 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             unet_apply(*L->net, L->pr, pf, pz, 1, c.g.ny, c.g.nx);
             hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s, c.f[F_Z]);
             launched("FCG output");
-            fcg_smooth(c, opt.smooth, opt.omega, false);
+            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, false, true, !first);
         } else {
-            fcg_smooth(c, opt.smooth, opt.omega, true);
+            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
         }
         // beta, p, w, alpha, u, r with the scalars on the device; one read
-        op_fcg_iteration_dev(c, first);
+        op_fcg_iteration_dev(c, first, beta_blocks);
         first = false;
         rrn = c.result(S_RRN, 0);
         if (c.result(S_RRN, 1) != 0.0) return false;
