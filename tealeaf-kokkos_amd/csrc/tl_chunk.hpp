@@ -255,8 +255,10 @@ void op_fcg_update_p(Chunk& c, double beta);
 // result(S_RRN, 1) is 1 when alpha could not be formed (p.w <= 0 or a
 // non-finite p.r).
 // beta_blocks > 0: z . w is already in the reduction partials (from the
-// preconditioner pass), with that many blocks.
-void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks = 0);
+// preconditioner pass), with that many blocks.  In 2D the step leaves
+// u += alpha p pending (u_pending) for the next step's strip; apply it with
+// op_cg_calc_u_dev before u is read.
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending);
 
 // Observers of the driver's FP64 PPCG solve (no counterpart in the
 // reference): after every outer iteration (u, r current) and at the end of

@@ -1800,7 +1800,7 @@ double op_dot(Chunk& c, const double* a, const double* b)
 void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, nullptr, 0, c.f[F_Z], c.f[F_P]); }
 
 // The flexible CG's scalars on the device: one host read per iteration.
-void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks)
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending)
 {
     if (!first && beta_blocks > 0) {
         finish<1>(c, beta_blocks, S_ERR, EP_FCG_BETA);
@@ -1818,16 +1818,29 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks)
         if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
         t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        // with u_pending the previous iteration's u += alpha p rides along
+        // (alpha and the old p are still in place)
         TIMED(c, K_CG_WP, {
-            hipLaunchKernelGGL((k_cg_pw_strip<false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
-                               kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(), c.f[F_R]);
+            if (u_pending)
+                hipLaunchKernelGGL((k_cg_pw_strip<true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
+                                   kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer(),
+                                   c.f[F_R]);
+            else
+                hipLaunchKernelGGL((k_cg_pw_strip<false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g,
+                                   t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(),
+                                   c.f[F_R]);
             check_launch();
         });
+        u_pending = false;
         std::swap(c.f[F_P], c.p_alt);
         finish<2>(c, nb, S_PW, EP_FCG_PWPR);
     } else
 #endif
     {
+        if (u_pending) {
+            op_cg_calc_u_dev(c);
+            u_pending = false;
+        }
         STRIP_LAUNCH(k_fcg_p, c, 0.0, c.ds, first ? 1 : 0, c.f[F_Z], c.f[F_P]);
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
                                       c.reducer()));
@@ -1835,8 +1848,15 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks)
         STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer());
         finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_ALPHA);
     }
-    TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R],
-                                   c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    if (TL_DIM == 2) {
+        // r only: u += alpha p is left to the next strip (or op_cg_calc_u_dev)
+        TIMED(c, K_CG_R, STRIP_LAUNCH(k_cg_calc_ur<false>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U],
+                                      c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+        u_pending = true;
+    } else {
+        TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U],
+                                       c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+    }
     finish<1>(c, c.g.nblocks, S_RRN, EP_FCG_RRN);
 }
 

@@ -441,6 +441,13 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         return true;
     }
     double best = rrn;
+    bool u_pending = false;  // u += alpha p of the last step not yet applied
+    // (a solve that returns false is re-run from its start by the caller,
+    // so only the accepting path needs u current)
+    auto flush_u = [&] {
+        if (u_pending) op_cg_calc_u_dev(c);
+        u_pending = false;
+    };
     int since_best = 0;
     const int window = 200;
     bool first = true;
@@ -460,14 +467,15 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
         }
         // beta, p, w, alpha, u, r with the scalars on the device; one read
-        op_fcg_iteration_dev(c, first, beta_blocks);
+        op_fcg_iteration_dev(c, first, beta_blocks, u_pending);
         first = false;
         rrn = c.result(S_RRN, 0);
         if (c.result(S_RRN, 1) != 0.0) return false;
         iters = it;
         if (!std::isfinite(rrn)) return false;
         if (rrn < eps) {
-            // accept on the true residual only
+            // accept on the true residual only (u brought up to date first)
+            flush_u();
             op_calc_residual(c);
             rrn = op_calc_2norm(c, true);
             if (rrn < eps) {
