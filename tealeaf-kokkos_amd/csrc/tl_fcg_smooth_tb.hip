@@ -1,7 +1,7 @@
 // The flexible CG's preconditioner sweeps as one temporal-blocked pass (2D).
 //
 // The preconditioner applies K weighted Jacobi sweeps on A z = r,
-// z_(s) = z_(s-1) + omega (r - A z_(s-1)) / D, D the diagonal of A (1 plus
+// z_(s) = z_(s-1) + (r - A z_(s-1)) (omega / D), D the diagonal of A (1 plus
 // the coefficients of the cell's interior faces; a face on the mesh boundary
 // carries no flux), starting from z = 0 or from the network's output.  Sweep
 // s needs sweep s - 1 on the neighbouring rows only, so the wavefront of the
@@ -64,6 +64,7 @@ struct FState {
     double zw[K][3][2];  // z after sweep s, slot (round) % 3, column
     FRow pf[2];
     double acc;          // z . w over the wave's own cells (DOTW)
+    double2 wd1;         // omega / D of level 1's row, formed this round
 };
 
 struct FCtx {
@@ -77,6 +78,7 @@ struct FCtx {
     double* rr;  // LDS rings [ring rows][64][2]
     double* kxr;
     double* kyr;
+    double* wdr;  // omega / D of the row's cells
 };
 
 template <int K, bool ZERO, bool DOTW>
@@ -106,10 +108,12 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     double2* rr = reinterpret_cast<double2*>(c.rr) + lane;
     double2* kxr = reinterpret_cast<double2*>(c.kxr) + lane;
     double2* kyr = reinterpret_cast<double2*>(c.kyr) + lane;
+    double2* wdr = reinterpret_cast<double2*>(c.wdr) + lane;
     const double* kxe = c.kxr + 2 * lane + 2;
 
     double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE], nr = rr[R::row(PH, 1) * WAVE];
     double nkxe = kxe[2 * WAVE * R::row(PH, 1)];
+    double2 nwd = wdr[R::row(PH, 2) * WAVE];  // level 2's row (level 1's is formed below)
 
     // row y enters: sweep 0 (the start), r and the masked coefficients
     st.zw[0][S0][0] = ZERO ? 0.0 : pick(v.z, c.hi0);
@@ -118,6 +122,23 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     kxr[R::row(PH, 0) * WAVE] = make_double2(c.kw0 ? v.kx0 : 0.0, c.kw1 ? v.kx1 : 0.0);
     double kyn0 = v.ky_in ? pick(v.ky, c.hi0) : 0.0, kyn1 = v.ky_in ? pick(v.ky, c.hi1) : 0.0;
     kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
+    {
+        // omega / D of row y - 1, whose north faces just arrived with row y:
+        // D = 1 + east + west + north + south in that order (0 for a face on
+        // the mesh boundary), once per cell instead of once per sweep
+        double d0 = 1.0, d1 = 1.0;
+        d0 += nkx.y;
+        d1 += nkxe;
+        d0 += nkx.x;
+        d1 += nkx.y;
+        d0 += kyn0;
+        d1 += kyn1;
+        d0 += nky.x;
+        d1 += nky.y;
+        const double2 wd = make_double2(c.omega / d0, c.omega / d1);
+        wdr[R::row(PH, 1) * WAVE] = wd;
+        st.wd1 = wd;
+    }
     const double2 wo = v.wo;  // w of this round's output row
     f_load<K, ZERO, DOTW>(A, c, tab, PH, y + 2, v);
 
@@ -126,11 +147,13 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         const int z = y - s;
         const double2 kx = nkx, ky = nky, r = nr;
         const double kxe1 = nkxe;
+        const double2 wd = s == 1 ? st.wd1 : nwd;
         if (s < K) {
             nkx = kxr[R::row(PH, s + 1) * WAVE];
             nky = kyr[R::row(PH, s + 1) * WAVE];
             nr = rr[R::row(PH, s + 1) * WAVE];
             nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
+            if (s >= 2) nwd = wdr[R::row(PH, s + 1) * WAVE];
         }
         const double c0 = st.zw[s - 1][S1][0], c1 = st.zw[s - 1][S1][1];
         const double s0 = st.zw[s - 1][S2][0], s1 = st.zw[s - 1][S2][1];
@@ -138,25 +161,17 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         const double w0 = from_west(c1), e1 = from_east(c0);
         const double e0 = c1, w1 = c0;
         // the one-sweep kernel's order: east, west, north, south
-        double az0 = c0, az1 = c1, d0 = 1.0, d1 = 1.0;
+        double az0 = c0, az1 = c1;
         az0 += kx.y * (c0 - e0);
-        d0 += kx.y;
         az1 += kxe1 * (c1 - e1);
-        d1 += kxe1;
         az0 += kx.x * (c0 - w0);
-        d0 += kx.x;
         az1 += kx.y * (c1 - w1);
-        d1 += kx.y;
         az0 += kyn0 * (c0 - n0);
-        d0 += kyn0;
         az1 += kyn1 * (c1 - n1);
-        d1 += kyn1;
         az0 += ky.x * (c0 - s0);
-        d0 += ky.x;
         az1 += ky.y * (c1 - s1);
-        d1 += ky.y;
-        const double zn0 = c0 + c.omega * (r.x - az0) / d0;
-        const double zn1 = c1 + c.omega * (r.y - az1) / d1;
+        const double zn0 = c0 + (r.x - az0) * wd.x;
+        const double zn1 = c1 + (r.y - az1) * wd.y;
         kyn0 = ky.x;
         kyn1 = ky.y;
         if (s < K) {
@@ -194,6 +209,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     // one more element: the last lane's east face reads past the last row
     __shared__ double2 kx_ring[R::N * WAVE + 1];
     __shared__ double2 ky_ring[R::N * WAVE];
+    __shared__ double2 wd_ring[R::N * WAVE];
     const int lane = threadIdx.x;
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
@@ -233,6 +249,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     c.rr = reinterpret_cast<double*>(r_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
+    c.wdr = reinterpret_cast<double*>(wd_ring);
 
     FState<K> st;
 #pragma unroll
@@ -247,6 +264,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
         st.pf[k].ky_in = false;
     }
     st.acc = 0.0;
+    st.wd1 = make_double2(0.0, 0.0);
     ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
     f_load<K, ZERO, DOTW>(A, c, tab, 0, c.ylo, st.pf[0]);
     f_load<K, ZERO, DOTW>(A, c, tab, 1, c.ylo + 1, st.pf[1]);

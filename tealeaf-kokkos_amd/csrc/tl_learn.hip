@@ -163,7 +163,7 @@ __global__ void k_plane_out(Geom g, const float* __restrict__ pz, double s, doub
     }
 }
 
-// One weighted Jacobi sweep on A z = r: z_out = z + omega (r - A z) / D,
+// One weighted Jacobi sweep on A z = r: z_out = z + (r - A z) (omega / D),
 // D the diagonal of A (1 + the coefficients of the cell's interior faces;
 // a face on the mesh boundary carries no flux).  Interior cells.
 __global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double* __restrict__ ky,
@@ -180,7 +180,7 @@ __global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double
         if (x > HALO) { az += kx[c] * (zc - z[c - 1]); d += kx[c]; }
         if (y < g.ny + 1) { az += ky[c + g.sx] * (zc - z[c + g.sx]); d += ky[c + g.sx]; }
         if (y > HALO) { az += ky[c] * (zc - z[c - g.sx]); d += ky[c]; }
-        zo[c] = zc + omega * (r[c] - az) / d;
+        zo[c] = zc + (r[c] - az) * (omega / d);
     }
 }
 
