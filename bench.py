@@ -361,7 +361,8 @@ def main() -> int:
     ap.add_argument("--net-every", type=int, default=1, help="c5: apply the network every m-th FCG iteration")
     ap.add_argument("--crop", type=int, default=64, help="c5: training pair size (cells per side)")
     ap.add_argument("--n", type=int, default=None, help="cells per side (default 4096, 256 for c3, 8192 for c4)")
-    ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi"])
+    ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi", "fcg"],
+                    help="fcg: flexible CG with Jacobi-sweep preconditioning on the PPCG deck")
     ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision PPCG solve")
     ap.add_argument("--no-fcg", action="store_true", help="skip the flexible CG (Jacobi sweeps) solve")
@@ -413,17 +414,25 @@ def main() -> int:
     max_iters = 10000 if args.solver != "jacobi" else 20000
     extra = ("tl_check_result",)
     dim = 3 if args.workload == "c3" else 2
+    # fcg: the flexible CG with Jacobi-sweep preconditioning (the learned
+    # form with its network off) on the PPCG deck
+    solver = "ppcg" if args.solver == "fcg" else args.solver
     if args.workload == "c2":
-        deck = decks.two_rect_2d(n, args.solver, steps=total, presteps=30, inner=10, max_iters=max_iters,
+        deck = decks.two_rect_2d(n, solver, steps=total, presteps=30, inner=10, max_iters=max_iters,
                                  extra=extra)
         what = f"2D {n}x{n} {args.solver.upper()} two-rectangle deck"
     elif args.workload == "c3":
-        deck = decks.cube_3d(n, args.solver, steps=total, presteps=30, inner=10, max_iters=max_iters, extra=extra)
+        deck = decks.cube_3d(n, solver, steps=total, presteps=30, inner=10, max_iters=max_iters, extra=extra)
         what = f"3D {n}^3 {args.solver.upper()} hot-box/cuboid deck"
     else:
-        deck = decks.random_rects_2d(n, args.seed, steps=total, solver=args.solver, presteps=30, inner=10,
+        deck = decks.random_rects_2d(n, args.seed, steps=total, solver=solver, presteps=30, inner=10,
                                      max_iters=max_iters, extra=extra)
         what = f"2D {n}x{n} {args.solver.upper()} 32 random rectangles (splitmix64 seed {args.seed})"
+    if args.solver == "fcg":
+        if dim != 2:
+            print("bench: the flexible CG is 2D", file=sys.stderr)
+            return 2
+        deck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network")
 
     print(f"bench: {what}, {args.warmup} + {args.steps} timesteps", file=sys.stderr, flush=True)
     # every kernel timed over the first timestep: the kernel table and the

@@ -313,10 +313,12 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
     if (w && a.nblocks > c.partials_cap) throw Error("temporal-blocked smoothing: more waves than reduction partials");
     const dim3 grid((unsigned)a.nblocks), block(WAVE);
-    if (z_in && w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true>), grid, block, 0, c.stream, c.g, a);
-    else if (z_in) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false>), grid, block, 0, c.stream, c.g, a);
-    else if (w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true>), grid, block, 0, c.stream, c.g, a);
-    else hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false>), grid, block, 0, c.stream, c.g, a);
+    TL_TIMED(c, K_FCG_SMOOTH, {
+        if (z_in && w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true>), grid, block, 0, c.stream, c.g, a);
+        else if (z_in) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false>), grid, block, 0, c.stream, c.g, a);
+        else if (w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true>), grid, block, 0, c.stream, c.g, a);
+        else hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false>), grid, block, 0, c.stream, c.g, a);
+    });
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
     return w ? a.nblocks : 0;

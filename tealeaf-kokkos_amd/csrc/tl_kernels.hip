@@ -1010,7 +1010,8 @@ const char* kernel_name(int k)
                                          "cheby_iterate", "ppcg_init_sd", "ppcg_inner", "ppcg_outer",
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
                                          "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
-                                         "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u"};
+                                         "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u", "fcg_smooth",
+                                         "fcg_p_w"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -1038,6 +1039,8 @@ double kernel_bytes_per_cell(int k)
     case K_CG_WPU: return 48.0 + kb;          // u, p, r, coefficients -> u', p', w
     case K_CG_R: return 24.0;                 // r, w -> r
     case K_CG_U: return 24.0;                 // u, p -> u
+    case K_FCG_SMOOTH: return 24.0 + kb;      // r, w, coefficients -> z (4 sweeps, z.w)
+    case K_FCG_STRIP: return 56.0 + kb;       // u, p, z, r, coefficients -> u', p', w (p.w, p.r)
     default: return 0.0;
     }
 }
@@ -1820,7 +1823,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         // with u_pending the previous iteration's u += alpha p rides along
         // (alpha and the old p are still in place)
-        TIMED(c, K_CG_WP, {
+        TIMED(c, K_FCG_STRIP, {
             if (u_pending)
                 hipLaunchKernelGGL((k_cg_pw_strip<true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
                                    kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer(),
