@@ -52,7 +52,7 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
     K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_JACOBI_TB,
-    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_COUNT
+    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_FCG_SMOOTH_R, K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or
@@ -257,8 +257,12 @@ void op_fcg_update_p(Chunk& c, double beta);
 // beta_blocks > 0: z . w is already in the reduction partials (from the
 // preconditioner pass), with that many blocks.  In 2D the step leaves
 // u += alpha p pending (u_pending) for the next step's strip; apply it with
-// op_cg_calc_u_dev before u is read.
-void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending);
+// op_cg_calc_u_dev before u is read.  With beta_rrn the pass also applied
+// the previous iteration's deferred r update and left its ||r||^2 (read from
+// S_ERR); with defer_r this iteration's r update is deferred the same way and
+// S_RRN gets nothing.
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending, bool beta_rrn = false,
+                          bool defer_r = false);
 
 // Observers of the driver's FP64 PPCG solve (no counterpart in the
 // reference): after every outer iteration (u, r current) and at the end of
@@ -292,7 +296,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
 long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused = true, bool dot_w = false);
 bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);
 long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps,
-                      const double* w = nullptr);
+                      const double* w = nullptr, double* r_out = nullptr);
 
 // The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps
 // timesteps when that is positive; returns the JSON report.

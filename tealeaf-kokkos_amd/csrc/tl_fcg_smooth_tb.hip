@@ -40,6 +40,8 @@ struct FArgs {
     double* z_out;
     const double* w;     // with DOTW: the previous direction's A p, for z . w
     double* partials;
+    double* r_out;       // with RUPD: r - alpha w (!= r), the previous step's residual update
+    const DevScalars* ds;
     const double* r;
     const double* kx;
     const double* ky;
@@ -55,6 +57,7 @@ struct FArgs {
 struct FRow {
     double2 z, r, ky;
     double2 wo;  // w of the output row two rounds on (DOTW)
+    double2 wi;  // w of the row itself (RUPD)
     double kx0, kx1;
     bool ky_in;  // the row's south face is inside the mesh
 };
@@ -64,6 +67,7 @@ struct FState {
     double zw[K][3][2];  // z after sweep s, slot (round) % 3, column
     FRow pf[2];
     double acc;          // z . w over the wave's own cells (DOTW)
+    double acc_rr;       // ||r - alpha w||^2 over the wave's own cells (RUPD)
     double2 wd1;         // omega / D of level 1's row, formed this round
 };
 
@@ -74,18 +78,19 @@ struct FCtx {
     bool hi0, hi1;
     bool kw0, kw1;  // the lane's x faces (west of c0, west of c1) are inside the mesh
     bool out0, out1;
-    double omega;
+    double omega, alpha;
     double* rr;  // LDS rings [ring rows][64][2]
     double* kxr;
     double* kyr;
     double* wdr;  // omega / D of the row's cells
 };
 
-template <int K, bool ZERO, bool DOTW>
+template <int K, bool ZERO, bool DOTW, bool RUPD>
 __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t, int PHk, int y, FRow& v)
 {
     const unsigned oy = t[2 * PHk], of = t[2 * PHk + 1];
     if (!ZERO) v.z = ldp(A.z_in, oy + c.bcp);
+    if (RUPD) v.wi = ldp(A.w, oy + c.bcp);
     // the row that leaves the pass when row y enters it (slack rows hold
     // those past the mesh); lanes without output read a pad pair
     if (DOTW) v.wo = ldp(A.w, 8u * (unsigned)(c.g.base + (long)(y - K) * c.g.sx) + c.bsp);
@@ -98,7 +103,7 @@ __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t,
     v.ky_in = y > HALO && y <= A.ny + 1;
 }
 
-template <int K, bool ZERO, bool DOTW, int PH>
+template <int K, bool ZERO, bool DOTW, bool RUPD, int PH>
 __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     using R = tb::TBRing<K>;
@@ -118,7 +123,20 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     // row y enters: sweep 0 (the start), r and the masked coefficients
     st.zw[0][S0][0] = ZERO ? 0.0 : pick(v.z, c.hi0);
     st.zw[0][S0][1] = ZERO ? 0.0 : pick(v.z, c.hi1);
-    rr[R::row(PH, 0) * WAVE] = make_double2(pick(v.r, c.hi0), pick(v.r, c.hi1));
+    double2 rin = v.r;
+    if (RUPD) {
+        // CGCalcUr's r -= alpha w of the previous step, on the pair as loaded;
+        // the wave's own cells store it and sum its square
+        rin.x = v.r.x - c.alpha * v.wi.x;
+        rin.y = v.r.y - c.alpha * v.wi.y;
+        const double r0 = pick(rin, c.hi0), r1 = pick(rin, c.hi1);
+        const bool yin = y >= c.Y0 && y <= c.yhi;
+        const unsigned oy = 8u * (unsigned)(c.g.base + (long)y * c.g.sx);
+        stp(A.r_out, oy + (yin ? c.bsp : c.bpad), make_double2(r0, r1));
+        st.acc_rr += yin && c.out0 ? r0 * r0 : 0.0;
+        st.acc_rr += yin && c.out1 ? r1 * r1 : 0.0;
+    }
+    rr[R::row(PH, 0) * WAVE] = make_double2(pick(rin, c.hi0), pick(rin, c.hi1));
     kxr[R::row(PH, 0) * WAVE] = make_double2(c.kw0 ? v.kx0 : 0.0, c.kw1 ? v.kx1 : 0.0);
     double kyn0 = v.ky_in ? pick(v.ky, c.hi0) : 0.0, kyn1 = v.ky_in ? pick(v.ky, c.hi1) : 0.0;
     kyr[R::row(PH, 0) * WAVE] = make_double2(kyn0, kyn1);
@@ -140,7 +158,7 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         st.wd1 = wd;
     }
     const double2 wo = v.wo;  // w of this round's output row
-    f_load<K, ZERO, DOTW>(A, c, tab, PH, y + 2, v);
+    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, PH, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
@@ -191,17 +209,18 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     }
 }
 
-template <int K, bool ZERO, bool DOTW, int PH>
+template <int K, bool ZERO, bool DOTW, bool RUPD, int PH>
 __device__ __forceinline__ void f_rounds(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     if constexpr (PH < tb::TB_UNROLL) {
-        f_round<K, ZERO, DOTW, PH>(A, c, st, y + PH, tab);
+        f_round<K, ZERO, DOTW, RUPD, PH>(A, c, st, y + PH, tab);
         __builtin_amdgcn_sched_barrier(0);
-        f_rounds<K, ZERO, DOTW, PH + 1>(A, c, st, y, tab);
+        f_rounds<K, ZERO, DOTW, RUPD, PH + 1>(A, c, st, y, tab);
     }
 }
 
-template <int K, bool ZERO, bool DOTW>
+// RUPD (the previous step's r update folded in) comes with DOTW.
+template <int K, bool ZERO, bool DOTW, bool RUPD>
 __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
 {
     using R = tb::TBRing<K>;
@@ -246,6 +265,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     c.ylo = c.Y0 - K;
     c.yhi = min(c.Y0 + A.rows - 1, g.ny + 1);
     c.omega = A.omega;
+    c.alpha = RUPD ? A.ds->alpha : 0.0;
     c.rr = reinterpret_cast<double*>(r_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
@@ -259,20 +279,24 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const double2 z = make_double2(0.0, 0.0);
-        st.pf[k].z = st.pf[k].r = st.pf[k].ky = st.pf[k].wo = z;
+        st.pf[k].z = st.pf[k].r = st.pf[k].ky = st.pf[k].wo = st.pf[k].wi = z;
         st.pf[k].kx0 = st.pf[k].kx1 = 0.0;
         st.pf[k].ky_in = false;
     }
     st.acc = 0.0;
+    st.acc_rr = 0.0;
     st.wd1 = make_double2(0.0, 0.0);
     ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
-    f_load<K, ZERO, DOTW>(A, c, tab, 0, c.ylo, st.pf[0]);
-    f_load<K, ZERO, DOTW>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
+    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, 0, c.ylo, st.pf[0]);
+    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
     __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
     const int yend = c.yhi + K;
     for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
-        f_rounds<K, ZERO, DOTW, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
-    if (DOTW) {
+        f_rounds<K, ZERO, DOTW, RUPD, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+    if (RUPD) {
+        double v[2] = {st.acc, st.acc_rr};
+        block_partial<2, 1>(v, Reduce{A.partials});
+    } else if (DOTW) {
         double v[1] = {st.acc};
         block_partial<1, 1>(v, Reduce{A.partials});
     }
@@ -290,15 +314,22 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 // z_out = `sweeps` weighted Jacobi sweeps on A z = r from z_in (zero when
 // z_in is null), in one pass.  z_out != z_in.  With w, the pass also leaves
 // the first phase of z_out . w in the reduction partials (returns their
-// count, else 0).
-long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps, const double* w)
+// count, else 0).  With r_out (and w, from zero) it first completes the
+// previous step's r -= alpha w (alpha from the device scalars) into r_out,
+// sweeps on that r, and leaves z_out . w and ||r_out||^2 as two partials per
+// block.
+long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps, const double* w,
+                      double* r_out)
 {
     if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
+    if (r_out && (!w || z_in)) throw Error("temporal-blocked smoothing: the r update needs w and a zero start");
     ftb::FArgs a{};
     a.z_in = z_in;
     a.z_out = z_out;
     a.w = w;
     a.partials = c.partials;
+    a.r_out = r_out;
+    a.ds = c.ds;
     a.r = c.f[F_R];
     a.kx = c.f[F_KX];
     a.ky = c.f[F_KY];
@@ -313,11 +344,17 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
     if (w && a.nblocks > c.partials_cap) throw Error("temporal-blocked smoothing: more waves than reduction partials");
     const dim3 grid((unsigned)a.nblocks), block(WAVE);
-    TL_TIMED(c, K_FCG_SMOOTH, {
-        if (z_in && w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true>), grid, block, 0, c.stream, c.g, a);
-        else if (z_in) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false>), grid, block, 0, c.stream, c.g, a);
-        else if (w) hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true>), grid, block, 0, c.stream, c.g, a);
-        else hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false>), grid, block, 0, c.stream, c.g, a);
+    TL_TIMED(c, r_out ? K_FCG_SMOOTH_R : K_FCG_SMOOTH, {
+        if (r_out)
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, true>), grid, block, 0, c.stream, c.g, a);
+        else if (z_in && w)
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true, false>), grid, block, 0, c.stream, c.g, a);
+        else if (z_in)
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false, false>), grid, block, 0, c.stream, c.g, a);
+        else if (w)
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, false>), grid, block, 0, c.stream, c.g, a);
+        else
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false, false>), grid, block, 0, c.stream, c.g, a);
     });
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
@@ -327,7 +364,7 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
 #else  // 3D keeps the one-sweep kernel (the flexible CG is 2D)
 
 bool fcg_smooth_tb_supported(const Chunk&, int) { return false; }
-long op_fcg_smooth_tb(Chunk&, const double*, double*, double, int, const double*)
+long op_fcg_smooth_tb(Chunk&, const double*, double*, double, int, const double*, double*)
 {
     throw Error("temporal-blocked smoothing is implemented for 2D");
 }

@@ -634,7 +634,10 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, cons
 // Second phase of every reduction: one workgroup adds the block partials in
 // block order and applies the scalar recurrence of tea_solve.f90 that
 // consumes the total, so the next kernel can read it from `ds`.
-enum Epilogue { EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE, EP_FCG_BETA, EP_FCG_PW, EP_FCG_ALPHA, EP_FCG_PWPR, EP_FCG_RRN };
+enum Epilogue {
+    EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE, EP_FCG_BETA, EP_FCG_PW, EP_FCG_ALPHA, EP_FCG_PWPR,
+    EP_FCG_RRN, EP_FCG_BETA_RRN
+};
 
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
@@ -696,6 +699,11 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
         ds->bad = (v[0] > 0.0 && isfinite(v[NV - 1])) ? 0.0 : 1.0;
         break;
     case EP_FCG_RRN:  // flexible CG: rrn, and the stop flag of this iteration's alpha
+        out[1] = ds->bad;
+        break;
+    case EP_FCG_BETA_RRN:  // flexible CG: beta, with the previous iteration's rrn and stop flag
+        ds->beta = -v[0] / ds->pw;
+        out[0] = v[NV - 1];
         out[1] = ds->bad;
         break;
     case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
@@ -1011,7 +1019,7 @@ const char* kernel_name(int k)
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
                                          "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
                                          "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u", "fcg_smooth",
-                                         "fcg_p_w"};
+                                         "fcg_p_w", "fcg_smooth_r"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -1041,6 +1049,7 @@ double kernel_bytes_per_cell(int k)
     case K_CG_U: return 24.0;                 // u, p -> u
     case K_FCG_SMOOTH: return 24.0 + kb;      // r, w, coefficients -> z (4 sweeps, z.w)
     case K_FCG_STRIP: return 56.0 + kb;       // u, p, z, r, coefficients -> u', p', w (p.w, p.r)
+    case K_FCG_SMOOTH_R: return 32.0 + kb;    // r, w, coefficients -> r', z (4 sweeps, z.w, r'.r')
     default: return 0.0;
     }
 }
@@ -1803,9 +1812,16 @@ double op_dot(Chunk& c, const double* a, const double* b)
 void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, nullptr, 0, c.f[F_Z], c.f[F_P]); }
 
 // The flexible CG's scalars on the device: one host read per iteration.
-void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending)
+// beta_rrn: the preconditioner pass left z . w and the previous iteration's
+// ||r||^2 (it applied that iteration's r update), read from S_ERR.  defer_r:
+// leave this iteration's r -= alpha w to the next preconditioner pass (2D);
+// no rrn reaches S_RRN then.
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending, bool beta_rrn, bool defer_r)
 {
-    if (!first && beta_blocks > 0) {
+    if (beta_rrn) {
+        if (first || beta_blocks <= 0) throw Error("flexible CG: no preconditioner partials for beta and rrn");
+        finish<2>(c, beta_blocks, S_ERR, EP_FCG_BETA_RRN);
+    } else if (!first && beta_blocks > 0) {
         finish<1>(c, beta_blocks, S_ERR, EP_FCG_BETA);
     } else if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
         STRIP_LAUNCH(k_dot, c, c.f[F_Z], c.f[F_W], c.reducer());
@@ -1850,6 +1866,10 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         finish<1>(c, c.g.nblocks, S_PW, EP_FCG_PW);
         STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer());
         finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_ALPHA);
+    }
+    if (TL_DIM == 2 && defer_r) {
+        u_pending = true;
+        return;
     }
     if (TL_DIM == 2) {
         // r only: u += alpha p is left to the next strip (or op_cg_calc_u_dev)

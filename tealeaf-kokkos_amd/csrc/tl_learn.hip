@@ -450,12 +450,46 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     };
     int since_best = 0;
     const int window = 200;
+    // returns 1 to accept (u current, true residual below eps), -1 stalled
+    auto check = [&](double r2, int it) -> int {
+        iters = it;
+        if (!std::isfinite(r2)) return -1;
+        if (r2 < eps) {
+            // accept on the true residual only (u brought up to date first)
+            flush_u();
+            op_calc_residual(c);
+            r2 = op_calc_2norm(c, true);
+            if (r2 < eps) {
+                error = r2;
+                return 1;
+            }
+        }
+        rrn = r2;
+        if (r2 < 0.5 * best) {
+            best = r2;
+            since_best = 0;
+        } else if (++since_best >= window) {
+            return -1;
+        }
+        return 0;
+    };
+    // Sweeps only (2D): iteration k's r -= alpha w and its ||r||^2 ride in
+    // iteration k + 1's preconditioner pass, and the host reads them while
+    // that iteration's strip runs (which also brings u up to iteration k).
+    const bool defer = !opt.network && TL_DIM == 2 && fcg_smooth_tb_supported(c, opt.smooth);
+    bool r_pending = false;  // r -= alpha w of the last step not yet applied
     bool first = true;
     for (int it = 1; it <= max_iters; ++it) {
         // z = s N(r / s), then weighted Jacobi sweeps on A z = r from it
         const double s = std::sqrt(rrn / cells);
         long beta_blocks = 0;  // z . w left by the preconditioner pass
-        if (opt.network && (it - 1) % std::max(1, opt.net_every) == 0) {
+        const bool beta_rrn = r_pending;
+        if (r_pending) {
+            beta_blocks = op_fcg_smooth_tb(c, nullptr, c.f[F_SD], opt.omega, opt.smooth, c.f[F_W], c.r_alt);
+            std::swap(c.f[F_Z], c.f[F_SD]);
+            std::swap(c.f[F_R], c.r_alt);
+            r_pending = false;
+        } else if (opt.network && (it - 1) % std::max(1, opt.net_every) == 0) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
             launched("FCG input");
@@ -466,29 +500,29 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         } else {
             beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
         }
-        // beta, p, w, alpha, u, r with the scalars on the device; one read
-        op_fcg_iteration_dev(c, first, beta_blocks, u_pending);
+        // beta, p, w, alpha, u[, r] with the scalars on the device; one read
+        const bool defer_r = defer && it < max_iters;
+        op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r);
         first = false;
-        rrn = c.result(S_RRN, 0);
+        if (beta_rrn) {
+            // iteration it - 1 is complete: r updated by the pass, u by the
+            // strip; what is pending now belongs to iteration it
+            const double r2 = c.result(S_ERR, 0);
+            if (c.result(S_ERR, 1) != 0.0) return false;
+            const bool pend = u_pending;
+            u_pending = false;
+            const int v = check(r2, it - 1);
+            if (v != 0) return v > 0;
+            u_pending = pend;
+        }
+        if (defer_r) {
+            r_pending = true;
+            continue;
+        }
+        const double r2 = c.result(S_RRN, 0);
         if (c.result(S_RRN, 1) != 0.0) return false;
-        iters = it;
-        if (!std::isfinite(rrn)) return false;
-        if (rrn < eps) {
-            // accept on the true residual only (u brought up to date first)
-            flush_u();
-            op_calc_residual(c);
-            rrn = op_calc_2norm(c, true);
-            if (rrn < eps) {
-                error = rrn;
-                return true;
-            }
-        }
-        if (rrn < 0.5 * best) {
-            best = rrn;
-            since_best = 0;
-        } else if (++since_best >= window) {
-            return false;  // stalled
-        }
+        const int v = check(r2, it);
+        if (v != 0) return v > 0;
     }
     return false;
 }

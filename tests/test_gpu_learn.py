@@ -153,3 +153,33 @@ def test_train_fcg_and_ablation(hip, oracle):
         trained.use(False)
         trained.close()
         untrained.close()
+
+
+@pytest.mark.parametrize("kind,n,kw", [("two_rect", 256, {}), ("shapes", 192, {}), ("two_rect", 136, {"ny": 72}),
+                                       ("two_rect", 128, {"max_iters": 7})])
+def test_fcg_sweeps_only(hip, oracle, kind, n, kw):
+    """Flexible CG with the four Jacobi sweeps only (no network): each
+    iteration's r update and ||r||^2 ride in the next iteration's
+    preconditioner pass.  It must meet eps on the true residual and agree
+    with the oracle's FP64 PPCG; with an iteration cap it cannot meet, it
+    falls back to FP64 PPCG, which then matches the oracle's under the same
+    cap."""
+    import decks
+    cap = kw.get("max_iters")
+    if kind == "shapes":
+        deck = decks.random_shapes_2d(n, 7, steps=2, extra=("tl_check_result",))
+    else:
+        deck = decks.two_rect_2d(n, "ppcg", steps=2, extra=("tl_check_result",), **kw)
+    ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
+    fdeck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network")
+    print(f"sweeps-only FCG {kind} {n} {kw} cap={cap}", flush=True)
+    got = hip.run_deck(fdeck, dim=2)
+    for sg, so in zip(got["steps"], ref["steps"]):
+        print(f"  step {sg['step']}: {sg['fcg_iters']} it (fallback {sg['fcg_fallback']}), exact "
+              f"{sg['exact_error']:.3e}; PPCG {so['iters']} it", flush=True)
+        assert sg["fcg"] == 1
+        assert sg["fcg_fallback"] == (1 if cap is not None else 0)
+        if cap is None:
+            assert sg["exact_error"] < 1e-10
+        for key in ("vol", "mass", "ie", "temp"):
+            assert abs(sg[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sg[key], so[key])
