@@ -115,6 +115,18 @@ void timer_c_(double* elapsed);
  * -1 on error (tl_driver_error()), -2 when cap is too small. */
 int tl_run_deck(const char* deck, char* json, int cap, int max_steps);
 const char* tl_driver_error(void);
+/* The last tl_run_deck's tea.out lines that the reference writes with fixed
+ * formats: the field summaries (field_summary.f90:44-45,81, from start.f90:154
+ * and diffuse.f90:60,92), the switch to Chebyshev/PPCG and est itc
+ * (tea_solve.f90:216-218,744-745), Conduction error / EXACT error / Iteration
+ * count (tea_solve.f90:535-543) and the QA verdict (field_summary.f90:148-155).
+ * The list-directed lines (times, wall clocks, banners) are not written. */
+const char* tl_tea_out(void);
+/* One value through the Fortran edit descriptor desc ("aW", "iW", "fW.D",
+ * "eW.D"), as those lines use them; v_int, v_real or text per the letter.
+ * Returns the length, -1 for an unhandled descriptor, -2 if cap is too small.
+ * No counterpart in the reference (its Fortran runtime formats). */
+int tl_fortran_edit(const char* desc, long v_int, double v_real, const char* text, char* out, int cap);
 /* Per-launch kernel timing with HIP events on the chunk's stream for the
  * timesteps from `from_step` on (0 = off); tl_run_deck's report then has a
  * "kernels" table: launches, summed milliseconds, algorithmic bytes/cell. */

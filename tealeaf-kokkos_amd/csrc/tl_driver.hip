@@ -12,10 +12,13 @@
 #include "tl_chunk.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -355,6 +358,99 @@ void calc_ch_coefs(std::vector<double>& ca, std::vector<double>& cb, double emin
     }
 }
 
+// ---- tea.out lines ----------------------------------------------------------
+//
+// The reference writes its per-step diagnostics and field summaries to
+// tea.out with fixed Fortran edit descriptors (field_summary.f90:44,81,148;
+// tea_solve.f90:216-218,535-543,744-745).  These produce the same text as a
+// Fortran runtime for Aw, Iw, Fw.d and Ew.d: right-justified in w columns,
+// w asterisks when the value does not fit, Ew.d as 0.d1..dd E+ee with the
+// E dropped for a three-digit exponent, NaN and Inf spelled as they are.
+
+namespace fortran {
+
+std::string edit_A(const std::string& t, int w)
+{
+    if ((int)t.size() >= w) return t.substr(0, (size_t)w);
+    return std::string((size_t)w - t.size(), ' ') + t;
+}
+
+std::string stars(int w) { return std::string((size_t)std::max(w, 0), '*'); }
+
+std::string edit_I(long v, int w)
+{
+    const std::string t = std::to_string(v);
+    return (int)t.size() > w ? stars(w) : edit_A(t, w);
+}
+
+std::string non_finite(double v, int w)
+{
+    const std::string t = std::isnan(v) ? "NaN" : (v < 0 ? "-Inf" : "Inf");
+    return (int)t.size() > w ? stars(w) : edit_A(t, w);
+}
+
+std::string edit_F(double v, int w, int d)
+{
+    if (!std::isfinite(v)) return non_finite(v, w);
+    char buf[400];
+    std::snprintf(buf, sizeof buf, "%.*f", d, v);
+    const std::string t = buf;
+    return (int)t.size() > w ? stars(w) : edit_A(t, w);
+}
+
+std::string edit_E(double v, int w, int d)
+{
+    if (!std::isfinite(v)) return non_finite(v, w);
+    // d significant digits, correctly rounded, as d1.d2..dd e+XX
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*e", d - 1, std::fabs(v));
+    std::string digits;
+    const char* q = buf;
+    for (; *q && *q != 'e'; ++q)
+        if (*q != '.') digits += *q;
+    int ex = v == 0.0 ? 0 : std::atoi(q + 1) + 1;  // 0.d1d2.. x 10^ex
+    char eb[16];
+    if (std::abs(ex) <= 99)
+        std::snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', std::abs(ex));
+    else
+        std::snprintf(eb, sizeof eb, "%c%03d", ex < 0 ? '-' : '+', std::abs(ex));
+    const std::string sign = std::signbit(v) ? "-" : "";
+    std::string t = sign + "0." + digits + eb;
+    if ((int)t.size() > w) t = sign + "." + digits + eb;  // the leading zero is optional
+    return (int)t.size() > w ? stars(w) : edit_A(t, w);
+}
+
+}  // namespace fortran
+
+// One summary in tea.out (field_summary.f90:44-45,81): the column heads and
+// the ' step:' line; the list-directed 'Time' line is left out.
+void tea_out_summary(std::string& out, int step, double vol, double mass, double ie, double temp)
+{
+    using namespace fortran;
+    out += edit_A("           ", 13) + edit_A("Volume", 16) + edit_A("Mass", 16) + edit_A("Density", 16) +
+           edit_A("Energy", 16) + edit_A("U", 16) + "\n";
+    out += edit_A(" step:", 6) + edit_I(step, 7) + edit_E(vol, 16, 7) + edit_E(mass, 16, 7) +
+           edit_E(mass / vol, 16, 7) + edit_E(ie, 16, 7) + edit_E(temp, 16, 7) + "\n";
+}
+
+// The reference's QA line for its test problems 6-23 (field_summary.f90:86-155).
+void tea_out_qa(std::string& out, int test_problem, double temp)
+{
+    // expected final temperatures of the shipped test problems
+    static const double expected[24] = {0, 0, 0, 0, 0, 0,
+                                        103.88639125996923, 177.00175203427580, 139.97102840522848,
+                                        118.33195538063963, 108.12687625327908, 103.46970921076068,
+                                        101.21009327612924, 100.16633923123922, 99.695059768963887,
+                                        99.756584330021866, 106.22536751207716, 103.75236286020100,
+                                        102.95322957172701, 102.61257379535277, 101.99792927438858,
+                                        101.93825682423312, 101.54636465546844, 101.51302189154072};
+    if (test_problem < 6 || test_problem > 23) return;
+    const double qa_diff = std::fabs((100.0 * (temp / expected[test_problem])) - 100.0);
+    out += "Test problem" + fortran::edit_I(test_problem, 4) + " is within" + fortran::edit_E(qa_diff, 16, 7) +
+           "% of the expected solution\n";
+    out += qa_diff < 0.001 ? " This test is considered PASSED\n" : " This test is considered NOT PASSED\n";
+}
+
 // ---- solve ----------------------------------------------------------------
 
 struct StepInfo {
@@ -367,6 +463,7 @@ struct StepInfo {
     // learned flexible CG: its iterations, and whether FP64 PPCG took over
     int fcg_iters = 0;
     bool fcg = false, fcg_fallback = false;
+    std::string tea_out;  // the step's tea.out lines
 };
 
 const int kAllExternal[6] = {EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE, EXTERNAL_FACE};
@@ -465,6 +562,12 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 calc_ch_coefs(ch_alphas, ch_betas, eigmin, eigmax, theta,
                               d.use_cheby ? max_cheby_iters : d.ppcg_inner_steps);
                 cn = eigmax / eigmin;
+                // tea_solve.f90:216-218 (rro = error at the switch)
+                using namespace fortran;
+                si.tea_out += "Switching after " + edit_I(n, 3) + " steps, error " + edit_E(error, 15, 7) + "\n";
+                si.tea_out += edit_A("eigmin", 11) + edit_A("eigmax", 11) + edit_A("cn", 11) + edit_A("error", 11) + "\n";
+                si.tea_out += edit_F(eigmin, 11, 5) + edit_F(eigmax, 11, 5) + edit_E(cn, 11, 4) + edit_E(error, 11, 4) +
+                              "\n";
             }
             if (d.use_cheby) {
                 if (cheby_calc_steps == 0) {
@@ -478,6 +581,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     const double it_alpha = DBL_EPSILON * bb / (4.0 * error);
                     const double gamm = (std::sqrt(cn) - 1.0) / (std::sqrt(cn) + 1.0);
                     est_itc = (int)std::lround(std::log(it_alpha) / (2.0 * std::log(gamm)));
+                    si.tea_out += fortran::edit_A("est itc", 11) + "\n" + fortran::edit_I(est_itc, 11) + "\n";
                     cheby_calc_steps = 1;
                     switch_step = n;
                 } else if (cheby_tb_supported(c)) {
@@ -611,6 +715,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
         op_calc_residual(c);
         exact_error = op_calc_2norm(c, true);
     }
+    // tea_solve.f90:535-543
+    si.tea_out += "Conduction error " + fortran::edit_E(error, 14, 7) + "\n";
+    if (d.check_result) si.tea_out += "EXACT error calculated as" + fortran::edit_E(exact_error, 14, 7) + "\n";
+    si.tea_out += "Iteration count " + fortran::edit_I(n - 1, 8) + "\n";
     si.iters = n - 1;
     si.error = error;
     si.exact_error = exact_error;
@@ -640,16 +748,21 @@ struct Json {
     }
 };
 
-void summary(Json& o, Chunk& c, int step, double time)
+// field_summary.f90: the totals into the report and their tea.out lines;
+// returns temp
+double summary(Json& o, std::string& tea_out, Chunk& c, int step, double time)
 {
     double vol, mass, ie, temp;
     op_field_summary(c, &vol, &mass, &ie, &temp);
     o.f("\"step\": %d, \"time\": %.17g, \"vol\": %.17g, \"mass\": %.17g, \"ie\": %.17g, \"temp\": %.17g", step, time,
         vol, mass, ie, temp);
+    tea_out_summary(tea_out, step, vol, mass, ie, temp);
+    return temp;
 }
 
 // initialise -> start -> diffuse (start.f90:24-162, diffuse.f90:23-160).
 bool g_progress = false;  // a line on stderr after every timestep
+std::string g_tea_out;    // the last run's tea.out lines
 
 std::string run_deck(const std::string& text, int max_steps)
 {
@@ -679,8 +792,10 @@ std::string run_deck(const std::string& text, int max_steps)
     o.f("\"solver\": \"%s\", \"eps\": %.17g, \"dtinit\": %.17g, \"max_iters\": %d, ",
         d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? (d.use_fcg ? "fcg_learned" : d.ppcg_mixed ? "ppcg_mixed" : "ppcg") : "jacobi",
         d.eps, d.dtinit, d.max_iters);
+    std::string& tea_out = g_tea_out;
+    tea_out.clear();
     o.f("\"initial\": {");
-    summary(o, *c, 0, 0.0);
+    summary(o, tea_out, *c, 0, 0.0);
     o.f("}, \"steps\": [");
 
     op_set_field(*c);
@@ -699,6 +814,7 @@ std::string run_deck(const std::string& text, int max_steps)
             std::fprintf(stderr, "timestep %d: %d iterations, %.3f s\n", step, si.iters, si.solve_s);
             std::fflush(stderr);
         }
+        tea_out += si.tea_out;
         o.f("%s{", step > 1 ? ", " : "");
         o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
             "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
@@ -708,14 +824,14 @@ std::string run_deck(const std::string& text, int max_steps)
             si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0,
             si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0);
         if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
-            summary(o, *c, step, time);
+            summary(o, tea_out, *c, step, time);
         else
             o.f("\"step\": %d, \"time\": %.17g", step, time);
         o.f("}");
         if (time + g_small > d.end_time || step >= d.end_step || (max_steps > 0 && step >= max_steps)) break;
     }
     o.f("], \"final\": {");
-    summary(o, *c, step, time);
+    tea_out_qa(tea_out, d.test_problem, summary(o, tea_out, *c, step, time));
     o.f("}, \"test_problem\": %d, \"total_solve_s\": %.9g, \"cells\": %ld", d.test_problem, total,
         (long)d.x_cells * d.y_cells * nz);
     if (kernel_timing_from() > 0) {
@@ -766,6 +882,31 @@ int tl_run_deck(const char* deck, char* json, int cap, int max_steps)
 }
 
 const char* tl_driver_error(void) { return tl::g_driver_error.c_str(); }
+
+// The last tl_run_deck's tea.out lines with fixed formats (see the tea.out
+// section above), NUL-terminated; empty before the first run.
+const char* tl_tea_out(void) { return tl::g_tea_out.c_str(); }
+
+// One value through a Fortran edit descriptor "aW", "iW", "fW.D" or "eW.D"
+// (the first of v_int, v_real or text that the letter reads) into out[0..cap).
+// Returns the length, -1 for a descriptor it does not handle, -2 if cap is
+// too small.
+int tl_fortran_edit(const char* desc, long v_int, double v_real, const char* text, char* out, int cap)
+{
+    if (!desc || !out) return -1;
+    const char k = (char)std::tolower((unsigned char)desc[0]);
+    int w = 0, dd = -1;
+    if (std::sscanf(desc + 1, "%d.%d", &w, &dd) < 1 || w <= 0 || w > 200) return -1;
+    std::string r;
+    if (k == 'a') r = tl::fortran::edit_A(text ? text : "", w);
+    else if (k == 'i') r = tl::fortran::edit_I(v_int, w);
+    else if (k == 'f' && dd >= 0) r = tl::fortran::edit_F(v_real, w, dd);
+    else if (k == 'e' && dd >= 1) r = tl::fortran::edit_E(v_real, w, dd);
+    else return -1;
+    if ((int)r.size() + 1 > cap) return -2;
+    std::memcpy(out, r.c_str(), r.size() + 1);
+    return (int)r.size();
+}
 
 // Per-launch kernel timing with HIP events on the chunk's stream, for the
 // timesteps from `from_step` on (0 turns it off); tl_run_deck then adds a

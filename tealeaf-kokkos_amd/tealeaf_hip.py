@@ -68,6 +68,10 @@ def lib(dim: int) -> ctypes.CDLL:
         L.tl_run_deck.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         L.tl_run_deck.restype = ctypes.c_int
         L.tl_driver_error.restype = ctypes.c_char_p
+        L.tl_tea_out.restype = ctypes.c_char_p
+        L.tl_fortran_edit.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_double, ctypes.c_char_p,
+                                      ctypes.c_char_p, ctypes.c_int]
+        L.tl_fortran_edit.restype = ctypes.c_int
         L.tl_last_error.restype = ctypes.c_char_p
         L.tl_copy_probe.argtypes = [ctypes.c_long, ctypes.c_int]
         L.tl_copy_probe.restype = ctypes.c_double
@@ -106,7 +110,22 @@ def run_deck(deck: str, dim: int = 2, max_steps: int = 0, json_cap: int = 1 << 2
     rc = L.tl_run_deck(deck.encode(), buf, json_cap, int(max_steps))
     if rc != 0:
         raise RuntimeError(f"tl_run_deck failed ({rc}): {L.tl_driver_error().decode()}")
-    return parse_report(buf.value.decode())
+    rep = parse_report(buf.value.decode())
+    rep["tea_out"] = L.tl_tea_out().decode()
+    return rep
+
+
+def fortran_edit(desc: str, value, dim: int = 2) -> str:
+    """``value`` written through the Fortran edit descriptor ``desc``
+    (``aW``, ``iW``, ``fW.D``, ``eW.D``) as the driver writes tea.out."""
+    L = lib(dim)
+    buf = ctypes.create_string_buffer(256)
+    k = desc[0].lower()
+    n = L.tl_fortran_edit(desc.encode(), int(value) if k == "i" else 0,
+                          float(value) if k in "ef" else 0.0, str(value).encode() if k == "a" else None, buf, 256)
+    if n < 0:
+        raise ValueError(f"tl_fortran_edit({desc!r}) failed ({n})")
+    return buf.value.decode()
 
 
 def _i(v: int):

@@ -52,3 +52,26 @@ def test_missing_library_fails_loudly(hip, monkeypatch):
     monkeypatch.setattr(hip, "_LIBS", {})
     with pytest.raises(FileNotFoundError):
         hip.lib(2)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fortran_edit_golden(hip, dim):
+    """The driver's tea.out formatter against a Fortran runtime: every value
+    of tests/golden/fortran_format.json (written by flang from exact bit
+    patterns through the descriptors tea.out uses) comes out the same.  Host
+    code only: no GPU call."""
+    import json
+    import os
+    import struct
+    golden = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fortran_format.json")))
+    bad = []
+    for c in golden["cases"]:
+        if c["kind"] == "real":
+            v = struct.unpack("<d", struct.pack("<Q", int(c["bits"], 16)))[0]
+        else:
+            v = c["value"]
+        got = hip.fortran_edit(c["desc"], v, dim=dim)
+        if got != c["out"]:
+            bad.append((c["desc"], c["value"], got, c["out"]))
+    assert not bad, bad[:10]
+    assert len(golden["cases"]) > 250

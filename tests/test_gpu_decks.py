@@ -119,3 +119,112 @@ def test_qa_tp11_3d(hip, oracle):
     compare(gpu, ref)
     t = gpu["final"]["temp"]
     assert abs(100.0 * (t / 58.442896702081427) - 100.0) < 0.001
+
+
+# ---- tea.out lines -----------------------------------------------------------
+
+def _num(text: str) -> float:
+    """A Fortran-written number; Ew.d drops the E of a three-digit exponent."""
+    t = text.strip()
+    if "E" not in t and ("+" in t[1:] or "-" in t[1:]):
+        k = max(t.rfind("+"), t.rfind("-"))
+        t = t[:k] + "E" + t[k:]
+    return float(t)
+
+
+def _expected_tea_out(hip, ref: dict, check_result: bool, cheby: bool, dim: int):
+    """The tea.out lines for the oracle's report, as fixed-width pieces:
+    text, or (descriptor, value, kind) with kind "num" (within the printed
+    precision), "iters" (within +-1) or "any" (not in the oracle's report)."""
+    lines = []
+
+    def summary(s):
+        lines.append([hip.fortran_edit("a13", "           ", dim)] +
+                     [hip.fortran_edit("a16", h, dim) for h in ("Volume", "Mass", "Density", "Energy", "U")])
+        lines.append([" step:", ("i7", s["step"], "exact")] +
+                     [("e16.7", v, "num") for v in (s["vol"], s["mass"], s["mass"] / s["vol"], s["ie"], s["temp"])])
+
+    summary(ref["initial"])
+    for s in ref["steps"]:
+        if s["switch_step"] > 0:
+            lines.append(["Switching after ", ("i3", s["switch_step"], "iters"), " steps, error ",
+                          ("e15.7", 0.0, "any")])
+            lines.append([hip.fortran_edit("a11", h, dim) for h in ("eigmin", "eigmax", "cn", "error")])
+            lines.append([("f11.5", s["eigmin"], "f5"), ("f11.5", s["eigmax"], "f5"),
+                          ("e11.4", s["eigmax"] / s["eigmin"], "num4"), ("e11.4", 0.0, "any")])
+            if cheby:
+                lines.append([hip.fortran_edit("a11", "est itc", dim)])
+                lines.append([("i11", s["est_itc"], "iters")])
+        lines.append(["Conduction error ", ("e14.7", s["error"], "any")])
+        if check_result:
+            lines.append(["EXACT error calculated as", ("e14.7", s["exact_error"], "any")])
+        lines.append(["Iteration count ", ("i8", s["iters"], "iters")])
+        if "vol" in s:
+            summary(s)
+    summary(ref["final"])
+    return lines
+
+
+def _check_line(hip, got: str, pieces, dim: int):
+    pos = 0
+    for p in pieces:
+        if isinstance(p, str):
+            assert got[pos:pos + len(p)] == p, (got, p)
+            pos += len(p)
+            continue
+        desc, value, kind = p
+        w = int(desc[1:].split(".")[0])
+        field = got[pos:pos + w]
+        pos += w
+        assert len(field) == w and field == field.rjust(w) and "*" not in field, (got, desc)
+        if kind == "exact":
+            assert field == hip.fortran_edit(desc, value, dim), (got, desc, value)
+        elif kind == "iters":
+            assert abs(int(field) - value) <= 1, (got, value)
+        elif kind == "any":
+            _num(field)
+        else:
+            tol = {"num": 1.1e-6, "num4": 1.1e-3}.get(kind)  # one unit of the last printed digit
+            if tol is None:  # f11.5: five decimals
+                assert abs(float(field) - value) <= 1.1e-5 + 1e-9 * abs(value), (got, value)
+            else:
+                assert abs(_num(field) - value) <= tol * abs(value) + 1e-300, (got, value)
+    assert pos == len(got), (got, pos)
+
+
+TEA_OUT_CASES = [("chebyshev", 2, 128), ("ppcg", 2, 128), ("cg", 2, 48), ("chebyshev", 3, 24)]
+
+
+@pytest.mark.parametrize("solver,dim,n", TEA_OUT_CASES)
+def test_tea_out_lines(hip, oracle, solver, dim, n):
+    """The driver's tea.out lines have the reference's fixed layout and the
+    oracle's values: field summaries within the 7 printed digits, switch
+    step, eigenvalue line and est itc, conduction error, iteration counts
+    within +-1."""
+    import decks
+    if dim == 2:
+        deck = decks.two_rect_2d(n, solver, steps=3, extra=("tl_check_result",))
+    else:
+        deck = decks.cube_3d(n, solver, steps=2, max_iters=3000, extra=("tl_check_result",))
+    print(f"tea.out {solver} dim={dim} n={n}", flush=True)
+    gpu = hip.run_deck(deck, dim=dim)
+    ref = oracle.run_deck(deck, dim=dim, reduce_parts=8, threads=8)
+    got = gpu["tea_out"].splitlines()
+    exp = _expected_tea_out(hip, ref, True, solver == "chebyshev", dim)
+    assert len(got) == len(exp), (len(got), len(exp), got[:12])
+    for g, e in zip(got, exp):
+        _check_line(hip, g, e, dim)
+    if dim == 2 and solver in ("chebyshev", "ppcg"):  # these meshes switch after 30 CG steps
+        assert any(line.startswith("Switching after") for line in got)
+
+
+def test_tea_out_qa_verdict(hip):
+    """The reference's own QA deck (test problem 11, 512^2): tea.out ends
+    with its verdict line."""
+    rep = hip.run_deck(read_deck("tp11_2d.in"), dim=2)
+    lines = rep["tea_out"].splitlines()
+    assert lines[-2].startswith("Test problem  11 is within")
+    assert lines[-2].endswith("% of the expected solution")
+    qa = _num(lines[-2][len("Test problem  11 is within"):len("Test problem  11 is within") + 16])
+    assert qa < 0.001
+    assert lines[-1] == " This test is considered PASSED"
