@@ -122,7 +122,8 @@ const char* tl_driver_error(void);
  * The list-directed lines (times, wall clocks, banners) are not written. */
 const char* tl_tea_out(void);
 /* One value through the Fortran edit descriptor desc ("aW", "iW", "fW.D",
- * "eW.D"), as those lines use them; v_int, v_real or text per the letter.
+ * "eW.D"; W 1..200, D <= 40), as those lines use them; v_int, v_real or
+ * text per the letter.
  * Returns the length, -1 for an unhandled descriptor, -2 if cap is too small.
  * No counterpart in the reference (its Fortran runtime formats). */
 int tl_fortran_edit(const char* desc, long v_int, double v_real, const char* text, char* out, int cap);

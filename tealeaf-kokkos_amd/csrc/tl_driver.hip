@@ -889,14 +889,14 @@ const char* tl_tea_out(void) { return tl::g_tea_out.c_str(); }
 
 // One value through a Fortran edit descriptor "aW", "iW", "fW.D" or "eW.D"
 // (the first of v_int, v_real or text that the letter reads) into out[0..cap).
-// Returns the length, -1 for a descriptor it does not handle, -2 if cap is
-// too small.
+// W is 1..200 and D at most 40.  Returns the length, -1 for a descriptor it
+// does not handle, -2 if cap is too small.
 int tl_fortran_edit(const char* desc, long v_int, double v_real, const char* text, char* out, int cap)
 {
-    if (!desc || !out) return -1;
+    if (!desc || !out || !desc[0]) return -1;
     const char k = (char)std::tolower((unsigned char)desc[0]);
     int w = 0, dd = -1;
-    if (std::sscanf(desc + 1, "%d.%d", &w, &dd) < 1 || w <= 0 || w > 200) return -1;
+    if (std::sscanf(desc + 1, "%d.%d", &w, &dd) < 1 || w <= 0 || w > 200 || dd > 40) return -1;
     std::string r;
     if (k == 'a') r = tl::fortran::edit_A(text ? text : "", w);
     else if (k == 'i') r = tl::fortran::edit_I(v_int, w);

@@ -75,3 +75,14 @@ def test_fortran_edit_golden(hip, dim):
             bad.append((c["desc"], c["value"], got, c["out"]))
     assert not bad, bad[:10]
     assert len(golden["cases"]) > 250
+
+
+def test_fortran_edit_rejects_bad_descriptors(hip):
+    """Unhandled or out-of-range descriptors are refused, not read past."""
+    import ctypes
+    L = hip.lib(2)
+    buf = ctypes.create_string_buffer(64)
+    for desc in (b"", b"x5", b"e16", b"e0.7", b"f300.2", b"e16.41", b"q3"):
+        assert L.tl_fortran_edit(desc, 0, 1.0, None, buf, 64) == -1, desc
+    assert L.tl_fortran_edit(b"e16.7", 0, 1.0, None, buf, 8) == -2
+    assert L.tl_fortran_edit(b"i4", 42, 0.0, None, buf, 64) == 4 and buf.value == b"  42"
