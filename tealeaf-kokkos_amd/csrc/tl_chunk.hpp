@@ -52,7 +52,8 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
     K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_JACOBI_TB,
-    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_FCG_SMOOTH_R, K_COUNT
+    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_FCG_SMOOTH_R, K_FCG_STRIP_ZR,
+    K_COUNT
 };
 const char* kernel_name(int k);
 // Algorithmic HBM bytes per interior cell of one launch (each array read or

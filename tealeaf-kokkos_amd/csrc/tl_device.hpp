@@ -133,6 +133,7 @@ struct DevScalars {
     double rro, pw, alpha, rrn, beta, error;
     double rro_prev;  // rro before the last rrn epilogue (a replaced residual recomputes beta from it)
     double bad;       // flexible CG: 1 when the last alpha had p.w <= 0 or a non-finite p.r
+    double zr;        // flexible CG, sweeps only: z . r of the iteration, alpha's numerator
 };
 
 template <int NV>

@@ -68,6 +68,7 @@ struct FState {
     FRow pf[2];
     double acc;          // z . w over the wave's own cells (DOTW)
     double acc_rr;       // ||r - alpha w||^2 over the wave's own cells (RUPD)
+    double acc_zr;       // z . (r - alpha w) over the wave's own cells (RUPD)
     double2 wd1;         // omega / D of level 1's row, formed this round
 };
 
@@ -205,6 +206,10 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
                 st.acc += zin && c.out0 ? zn0 * wo.x : 0.0;
                 st.acc += zin && c.out1 ? zn1 * wo.y : 0.0;
             }
+            if (RUPD) {  // r of the output row, as updated on entry
+                st.acc_zr += zin && c.out0 ? zn0 * r.x : 0.0;
+                st.acc_zr += zin && c.out1 ? zn1 * r.y : 0.0;
+            }
         }
     }
 }
@@ -285,6 +290,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     }
     st.acc = 0.0;
     st.acc_rr = 0.0;
+    st.acc_zr = 0.0;
     st.wd1 = make_double2(0.0, 0.0);
     ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
     f_load<K, ZERO, DOTW, RUPD>(A, c, tab, 0, c.ylo, st.pf[0]);
@@ -294,8 +300,8 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
         f_rounds<K, ZERO, DOTW, RUPD, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
     if (RUPD) {
-        double v[2] = {st.acc, st.acc_rr};
-        block_partial<2, 1>(v, Reduce{A.partials});
+        double v[3] = {st.acc, st.acc_rr, st.acc_zr};
+        block_partial<3, 1>(v, Reduce{A.partials});
     } else if (DOTW) {
         double v[1] = {st.acc};
         block_partial<1, 1>(v, Reduce{A.partials});
@@ -316,8 +322,8 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 // the first phase of z_out . w in the reduction partials (returns their
 // count, else 0).  With r_out (and w, from zero) it first completes the
 // previous step's r -= alpha w (alpha from the device scalars) into r_out,
-// sweeps on that r, and leaves z_out . w and ||r_out||^2 as two partials per
-// block.
+// sweeps on that r, and leaves z_out . w, ||r_out||^2 and z_out . r_out as
+// three partials per block.
 long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps, const double* w,
                       double* r_out)
 {

@@ -636,7 +636,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, cons
 // consumes the total, so the next kernel can read it from `ds`.
 enum Epilogue {
     EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE, EP_FCG_BETA, EP_FCG_PW, EP_FCG_ALPHA, EP_FCG_PWPR,
-    EP_FCG_RRN, EP_FCG_BETA_RRN
+    EP_FCG_RRN, EP_FCG_BETA_RRN, EP_FCG_PW_ZR
 };
 
 template <int NV>
@@ -701,10 +701,16 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
     case EP_FCG_RRN:  // flexible CG: rrn, and the stop flag of this iteration's alpha
         out[1] = ds->bad;
         break;
-    case EP_FCG_BETA_RRN:  // flexible CG: beta, with the previous iteration's rrn and stop flag
+    case EP_FCG_BETA_RRN:  // flexible CG: beta and z . r; the previous iteration's rrn and stop flag
         ds->beta = -v[0] / ds->pw;
-        out[0] = v[NV - 1];
+        ds->zr = v[NV > 2 ? 2 : 0];
+        out[0] = v[NV > 1 ? 1 : 0];
         out[1] = ds->bad;
+        break;
+    case EP_FCG_PW_ZR:  // flexible CG, sweeps only: alpha = (z . r) / (p . w)
+        ds->pw = v[0];
+        ds->alpha = ds->zr / v[0];
+        ds->bad = (v[0] > 0.0 && isfinite(ds->zr)) ? 0.0 : 1.0;
         break;
     case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
         const double beta = v[0] / ds->rro_prev;
@@ -1019,7 +1025,7 @@ const char* kernel_name(int k)
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
                                          "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
                                          "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u", "fcg_smooth",
-                                         "fcg_p_w", "fcg_smooth_r"};
+                                         "fcg_p_w", "fcg_smooth_r", "fcg_p_w_zr"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -1049,7 +1055,8 @@ double kernel_bytes_per_cell(int k)
     case K_CG_U: return 24.0;                 // u, p -> u
     case K_FCG_SMOOTH: return 24.0 + kb;      // r, w, coefficients -> z (4 sweeps, z.w)
     case K_FCG_STRIP: return 56.0 + kb;       // u, p, z, r, coefficients -> u', p', w (p.w, p.r)
-    case K_FCG_SMOOTH_R: return 32.0 + kb;    // r, w, coefficients -> r', z (4 sweeps, z.w, r'.r')
+    case K_FCG_SMOOTH_R: return 32.0 + kb;    // r, w, coefficients -> r', z (4 sweeps, z.w, r'.r', z.r')
+    case K_FCG_STRIP_ZR: return 48.0 + kb;    // u, p, z, coefficients -> u', p', w (p.w)
     default: return 0.0;
     }
 }
@@ -1820,7 +1827,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
 {
     if (beta_rrn) {
         if (first || beta_blocks <= 0) throw Error("flexible CG: no preconditioner partials for beta and rrn");
-        finish<2>(c, beta_blocks, S_ERR, EP_FCG_BETA_RRN);
+        finish<3>(c, beta_blocks, S_ERR, EP_FCG_BETA_RRN);
     } else if (!first && beta_blocks > 0) {
         finish<1>(c, beta_blocks, S_ERR, EP_FCG_BETA);
     } else if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
@@ -1838,7 +1845,25 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         // with u_pending the previous iteration's u += alpha p rides along
-        // (alpha and the old p are still in place)
+        // (alpha and the old p are still in place).  With beta_rrn the
+        // preconditioner pass left z . r, alpha's numerator for the fixed
+        // (symmetric) sweep preconditioner, and the strip reads no r.
+        if (beta_rrn) {
+            TIMED(c, K_FCG_STRIP_ZR, {
+                if (u_pending)
+                    hipLaunchKernelGGL((k_cg_pw_strip<true, false>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream,
+                                       c.g, t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U],
+                                       c.reducer());
+                else
+                    hipLaunchKernelGGL((k_cg_pw_strip<false, false>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream,
+                                       c.g, t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr,
+                                       c.reducer());
+                check_launch();
+            });
+            u_pending = false;
+            std::swap(c.f[F_P], c.p_alt);
+            finish<1>(c, nb, S_PW, EP_FCG_PW_ZR);
+        } else {
         TIMED(c, K_FCG_STRIP, {
             if (u_pending)
                 hipLaunchKernelGGL((k_cg_pw_strip<true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
@@ -1853,6 +1878,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         u_pending = false;
         std::swap(c.f[F_P], c.p_alt);
         finish<2>(c, nb, S_PW, EP_FCG_PWPR);
+        }
     } else
 #endif
     {
