@@ -167,6 +167,12 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
  * (fused = 1; 2D, 4 sweeps) or one launch per sweep (fused = 0).  Returns 0,
  * or -1 with tl_last_error() set. */
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
+/* The sweeps-only flexible CG's preconditioner pass with the previous
+ * step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four
+ * weighted Jacobi sweeps on that r from zero, in one pass; sums[0..3) =
+ * z.w (w as given), ||r||^2, z.r.  No counterpart in the reference (a test
+ * hook of the flexible CG form).  Returns 0, or -1 (tl_last_error()). */
+int tl_fcg_smooth_update(const int* chunk, double omega, double alpha, double* sums);
 
 /* Mixed-precision form of tl_ppcg_outer_fused (no counterpart in the
  * reference): the nsteps inner steps in FP32 on the FP32 rounding of the

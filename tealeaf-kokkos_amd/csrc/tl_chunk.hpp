@@ -239,6 +239,8 @@ const uint2* tb_rowtab(Chunk& c);
 bool cheby_tb_supported(const Chunk& c);
 long op_cheby_tb(Chunk& c, int step1, int count, bool norm);
 double op_finish_norm(Chunk& c, long nblocks);
+// The same for three sums per block (out[0..3)).
+void op_finish_sums3(Chunk& c, long nblocks, double* out);
 // Temporal-blocked Jacobi (2D): one pass of K (1, 2, 5 or 10) iterations,
 // K = jacobi_tb_pass(iterations left); returns each iteration's error sum;
 // op_jacobi_tb_undo returns u to the pass's input.

@@ -180,6 +180,27 @@ int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int
     return rc;
 }
 
+// The sweeps-only flexible CG's preconditioner pass with the previous
+// step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four
+// sweeps on that r from zero; sums[0..3) = z.w (w as given), ||r||^2, z.r.
+int tl_fcg_smooth_update(const int* chunk, double omega, double alpha, double* sums)
+{
+    int rc = -1;
+    guarded([&] {
+        Chunk* c = stencil_chunk(chunk);
+        if (!fcg_smooth_tb_supported(*c, 4)) throw Error("no temporal-blocked pass for this chunk");
+        DevScalars s{};
+        s.alpha = alpha;
+        TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+        const long nb = op_fcg_smooth_tb(*c, nullptr, c->f[F_SD], omega, 4, c->f[F_W], c->r_alt);
+        std::swap(c->f[F_Z], c->f[F_SD]);
+        std::swap(c->f[F_R], c->r_alt);
+        op_finish_sums3(*c, nb, sums);
+        rc = 0;
+    });
+    return rc;
+}
+
 // The mixed-precision form of the same outer iteration: FP32 inner steps,
 // u and r updated in FP64 (2D only; nsteps with a compiled kernel).
 int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,

@@ -1632,6 +1632,12 @@ double op_finish_norm(Chunk& c, long nblocks)
     return c.result(S_NORM);
 }
 
+void op_finish_sums3(Chunk& c, long nblocks, double* out)
+{
+    finish<3>(c, nblocks, S_NORM, EP_STORE);
+    for (int k = 0; k < 3; ++k) out[k] = c.result(S_NORM, k);
+}
+
 void op_2norm_replace_dev(Chunk& c)
 {
     TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));

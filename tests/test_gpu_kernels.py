@@ -451,3 +451,44 @@ def test_fcg_smooth_fused(hip, shape, from_zero):
         out.append(interior(g, g.get("z")))
         g.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("shape", [(37, 23), (5, 4), (131, 6), (130, 41), (2, 17), (64, 64)])
+def test_fcg_smooth_update(hip, shape):
+    """The preconditioner pass with the previous step's r update folded in:
+    r - alpha w bitwise on every mesh cell, z equal to one launch per sweep
+    on that r (the fused pass adds + 0 where the one-sweep kernel skips a
+    boundary term), and z.w, ||r||^2, z.r against numpy over the mesh."""
+    import ctypes
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 977 + ny)
+    full = (1, ny + 4, nx + 4)
+    r, w = rng.uniform(-1, 1, size=full), rng.uniform(-1, 1, size=full)
+    kx, ky = rng.uniform(0.1, 40.0, size=full), rng.uniform(0.1, 40.0, size=full)
+    alpha, omega = 0.37, 0.8
+    print(f"fcg smooth update {shape}", flush=True)
+    g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+    for k, v in (("r", r), ("w", w), ("kx", kx), ("ky", ky)):
+        g.set(k, v)
+    sums = (ctypes.c_double * 3)()
+    rc = g.L.tl_fcg_smooth_update(g.c, ctypes.c_double(omega), ctypes.c_double(alpha), sums)
+    assert rc == 0, g.L.tl_last_error()
+    r_got, z_got = interior(g, g.get("r")), interior(g, g.get("z"))
+    g.close()
+    r_new = r - alpha * w
+    np.testing.assert_array_equal(r_got, interior(g, r_new))
+
+    g = hip.ExtChunk(hip.lib(2), 2, nx, ny)
+    for k, v in (("r", r_new), ("kx", kx), ("ky", ky)):
+        g.set(k, v)
+    rc = g.L.tl_fcg_smooth(g.c, ctypes.c_int(4), ctypes.c_double(omega), ctypes.c_int(1), ctypes.c_int(0))
+    assert rc == 0, g.L.tl_last_error()
+    z_ref = interior(g, g.get("z"))
+    g.close()
+    np.testing.assert_array_equal(z_got, z_ref)
+
+    wi, ri = interior(g, w), interior(g, r_new)
+    want = [float(np.sum(z_ref * wi)), float(np.sum(ri * ri)), float(np.sum(z_ref * ri))]
+    for k in range(3):
+        assert abs(sums[k] - want[k]) <= 1e-12 * max(1.0, float(np.sum(np.abs(z_ref * wi))), float(np.sum(ri * ri))), \
+            (k, sums[k], want[k])
