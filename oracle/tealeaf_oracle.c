@@ -112,6 +112,11 @@ static void set_error(const char *fmt, ...)
 
 void tl_oracle_set_reduce_parts(int parts) { g_parts = parts < 1 ? 1 : parts; }
 void tl_oracle_set_threads(int threads) { g_threads = threads; if (threads > 0) omp_set_num_threads(threads); }
+/* keep != 0: tl_run_deck leaves its chunk allocated after the run, so the
+ * tests can read the final fields through tl_get_field; the next
+ * tl_run_deck or ext_finalise_ releases it. */
+static int g_keep_chunk = 0;
+void tl_oracle_keep_chunk(int keep) { g_keep_chunk = keep; }
 int tl_oracle_get_reduce_parts(void) { return g_parts; }
 const char *tl_last_error(void) { return g_err ? g_errbuf : ""; }
 void tl_clear_error(void) { g_err = 0; g_errbuf[0] = 0; }
@@ -1722,6 +1727,6 @@ int tl_run_deck(const char *deck_text, char *json, int cap, int max_steps)
         if (o.buf && o.len >= o.cap) { set_error("JSON buffer too small"); rc = -2; }
     }
     free(d.states);
-    ext_finalise_();
+    if (!g_keep_chunk) ext_finalise_();
     return rc;
 }

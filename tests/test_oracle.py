@@ -35,6 +35,22 @@ def test_qa_tp11_2d(oracle):
     assert r["initial"]["vol"] == pytest.approx(100.0, rel=1e-13)
 
 
+@pytest.mark.parametrize("n,qa", [(32, 177.00175203427580), (64, 139.97102840522848), (128, 118.33195538063963)])
+def test_qa_2d_converged(oracle, n, qa):
+    """The 2D QA temperatures of test problems 7-9 (the shipped TP11 deck's
+    layout at 32^2, 64^2, 128^2; tealeaf_2d/field_summary.f90:94-101) are
+    met to 1e-13 once the solve is converged past the deck's tl_eps: with
+    tl_eps=1e-15 every mesh sits ~1e-9 away (the QA runs used a stricter
+    convergence than the decks state, README.md:189), with 1e-25 the gap is
+    rounding.  This pins the 2D operator, CG and the deck reader far below
+    the reference's 0.001 % bar."""
+    deck = read_deck("tp11_2d.in").replace("x_cells=512", f"x_cells={n}").replace("y_cells=512", f"y_cells={n}")
+    r = oracle.run_deck(deck.replace("tl_eps=1.0e-15", "tl_eps=1.0e-25"), dim=2, reduce_parts=1, threads=8)
+    assert abs(r["final"]["temp"] / qa - 1.0) < 1e-13
+    loose = oracle.run_deck(deck, dim=2, reduce_parts=1, threads=8)
+    assert 1e-10 < abs(loose["final"]["temp"] / qa - 1.0) < 1e-8
+
+
 def test_qa_tp11_3d(oracle):
     r = oracle.run_deck(read_deck("tp11_3d.in"), dim=3, reduce_parts=8, threads=8)
     t = r["final"]["temp"]
