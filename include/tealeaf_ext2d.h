@@ -187,6 +187,19 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
  * calibrates HBM byte counters.  Negative on error. */
 double tl_copy_probe(long n, int reps);
 
+/* No counterpart in the reference: ext_cg_calc_w_ for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  p, kx, ky, w: caller-owned device arrays
+ * in the reference's flat layout ((nx + 4) x (ny + 4), x fastest, halo 2; p's
+ * halo holding the reflective copies update_halo writes); w is written on the
+ * interior; *pw (one device double) receives sum w.p.  Launches on `stream`
+ * (a hipStream_t, NULL for the default stream) without synchronising.  Work
+ * memory is a static 512 KiB device array of one partial per mesh row:
+ * meshes of more than 65536 rows are refused; calls that overlap on one GPU
+ * race on it.  0, or -1 with tl_last_error() set. */
+int tl_cg_calc_w_noalloc(int nx, int ny, const double* p, const double* kx, const double* ky, double* w,
+                         double* pw, void* stream);
+
 /* ---- Learned preconditioner (no counterpart in the reference) ------------
  * A 4-level BF16 U-Net (3x3 convs, 16/32/64/128 channels, FP32 accumulate)
  * trained on the GPU on (residual, PPCG correction) pairs, applied by the

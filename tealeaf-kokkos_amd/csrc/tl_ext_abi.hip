@@ -390,6 +390,32 @@ double tl_copy_probe(long n, int reps)
     return gbs;
 }
 
+// ext_cg_calc_w_ on caller-owned device arrays with no device allocation
+// (the work partials live in a static device array); 0, or -1 on error.
+#if TL_DIM == 3
+int tl_cg_calc_w_noalloc(int nx, int ny, int nz, const double* p, const double* kx, const double* ky,
+                         const double* kz, double* w, double* pw, void* stream)
+{
+    int rc = -1;
+    guarded([&] {
+        cg_calc_w_noalloc(nx, ny, nz, p, kx, ky, kz, w, pw, static_cast<hipStream_t>(stream));
+        rc = 0;
+    });
+    return rc;
+}
+#else
+int tl_cg_calc_w_noalloc(int nx, int ny, const double* p, const double* kx, const double* ky, double* w,
+                         double* pw, void* stream)
+{
+    int rc = -1;
+    guarded([&] {
+        cg_calc_w_noalloc(nx, ny, 1, p, kx, ky, nullptr, w, pw, static_cast<hipStream_t>(stream));
+        rc = 0;
+    });
+    return rc;
+}
+#endif
+
 // ---- ext_chunk.cpp:18-33 --------------------------------------------------
 
 #if TL_DIM == 3
