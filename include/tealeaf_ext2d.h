@@ -215,6 +215,21 @@ void tl_learn_destroy(void* learner);
 long tl_learn_add_deck(void* learner, const char* deck, int snapshots, int crops, int crop,
                        unsigned long long seed);
 long tl_learn_pairs(void* learner);
+/* on != 0 (the default): each pair's input residual is A_c e, the crop's
+ * operator (no flux across the crop edge) applied to its correction e, so the
+ * target is exactly A_c^-1 of the input; 0: the solve's residual r_k over the
+ * crop.  Applies to pairs added afterwards. */
+int tl_learn_consistent_pairs(void* learner, int on);
+/* Solve phases later pairs are sampled from: 1 the CG presteps of the FP64
+ * PPCG solve, 2 its outer iterations, 3 both (the default); 0, or -1. */
+int tl_learn_sample_phases(void* learner, int mask);
+/* The A-norm training loss measured after `sweeps` weighted Jacobi sweeps of
+ * weight omega, as the flexible CG applies them after the network:
+ * per sample sum(Md.A Md) / max(sum(Mt.A Mt), batch mean), averaged,
+ * M = (I - omega D^-1 A)^sweeps (default 4 and
+ * 0.8, the flexible CG's defaults; 0 sweeps: the plain A-norm).  sweeps in
+ * [0, 16], omega in (0, 1]; 0, or -1 on error. */
+int tl_learn_loss_smoothing(void* learner, int sweeps, double omega);
 /* Adam on random batches of the pairs; losses[steps] receives each step's loss. */
 int tl_learn_train(void* learner, int steps, int batch, double lr, double* losses);
 /* The network tl_run_deck's flexible CG uses (NULL: none). */

@@ -217,6 +217,12 @@ void tl_learn_destroy(void* learner);
 long tl_learn_add_deck(void* learner, const char* deck, int snapshots, int crops, int crop,
                        unsigned long long seed);
 long tl_learn_pairs(void* learner);
+/* See tealeaf_ext2d.h (the learned preconditioner is 2D only). */
+int tl_learn_consistent_pairs(void* learner, int on);
+/* See tealeaf_ext2d.h (the learned preconditioner is 2D only). */
+int tl_learn_sample_phases(void* learner, int mask);
+/* See tealeaf_ext2d.h (the learned preconditioner is 2D only). */
+int tl_learn_loss_smoothing(void* learner, int sweeps, double omega);
 /* Adam on random batches of the pairs; losses[steps] receives each step's loss. */
 int tl_learn_train(void* learner, int steps, int batch, double lr, double* losses);
 /* The network tl_run_deck's flexible CG uses (NULL: none). */

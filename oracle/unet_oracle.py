@@ -160,10 +160,40 @@ def a_apply(v, kx, ky):
     return out
 
 
-def loss_grad(params, r, f, target, bf: bool = True, kx=None, ky=None):
+def a_diag(kx, ky):
+    """Diagonal of a_apply's operator: 1 + the coefficients of the faces
+    inside the sample."""
+    d = np.ones(kx.shape)
+    d[:, :, :-1] += kx[:, :, 1:]
+    d[:, :, 1:] += kx[:, :, 1:]
+    d[:, :-1, :] += ky[:, 1:, :]
+    d[:, 1:, :] += ky[:, 1:, :]
+    return d
+
+
+def smooth(v, kx, ky, sweeps, omega):
+    """M v, M = (I - omega D^-1 A)^sweeps: the error propagation of
+    `sweeps` weighted Jacobi sweeps on A z = r."""
+    D = a_diag(kx, ky)
+    for _ in range(sweeps):
+        v = v - omega * a_apply(v, kx, ky) / D
+    return v
+
+
+def smooth_t(g, kx, ky, sweeps, omega):
+    """M^T g = (I - omega A D^-1)^sweeps g."""
+    D = a_diag(kx, ky)
+    for _ in range(sweeps):
+        g = g - omega * a_apply(g / D, kx, ky)
+    return g
+
+
+def loss_grad(params, r, f, target, bf: bool = True, kx=None, ky=None, sweeps: int = 0, omega: float = 0.8):
     """(loss, flat parameter gradients) of out[..., 0] against target: the
     mean squared error, or with kx, ky the A-norm (d.A d)/(t.A t) of
-    d = out - target averaged over the samples."""
+    d = out - target over the batch; with sweeps > 0 the A-norm after
+    `sweeps` weighted Jacobi sweeps, (Md.A Md)/max(Mt.A Mt, batch mean) per
+    sample averaged over the samples (M their error propagation)."""
     q = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     a, wq = forward(params, r, f, bf)
     pixels = target.size
@@ -173,10 +203,22 @@ def loss_grad(params, r, f, target, bf: bool = True, kx=None, ky=None):
         loss = float(np.sum(d * d) / pixels)
         g_out[..., 0] = q(2.0 * d / pixels)
     else:
-        ad = a_apply(d, kx, ky)
-        den = float(np.sum(target * a_apply(target, kx, ky)))
-        loss = float(np.sum(d * ad) / den)
-        g_out[..., 0] = q(2.0 * ad / den)
+        if sweeps == 0:
+            ad = a_apply(d, kx, ky)
+            den = float(np.sum(target * a_apply(target, kx, ky)))
+            loss = float(np.sum(d * ad) / den)
+            g_out[..., 0] = q(2.0 * ad / den)
+        else:
+            # per sample sum(Md.A Md) / max(sum(Mt.A Mt), its batch mean),
+            # averaged over the samples
+            n = d.shape[0]
+            md = smooth(d, kx, ky, sweeps, omega)
+            mt = smooth(target, kx, ky, sweeps, omega)
+            ad = a_apply(md, kx, ky)
+            den = np.sum(mt * a_apply(mt, kx, ky), axis=(1, 2))
+            den = np.maximum(den, den.mean())
+            loss = float(np.mean(np.sum(md * ad, axis=(1, 2)) / den))
+            g_out[..., 0] = q(smooth_t(2.0 * ad / (n * den[:, None, None]), kx, ky, sweeps, omega))
     grads = [None] * len(SHAPES)
 
     def back(i, x, y, g, need_dx=True):

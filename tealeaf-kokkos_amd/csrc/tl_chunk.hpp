@@ -276,6 +276,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
 // the solve before the energy is finalised.  Used to record training pairs.
 struct SolveHooks {
     std::function<void(Chunk&, int)> on_outer;
+    std::function<void(Chunk&, int)> on_cg;  // after CG iteration n (u may lag r by the pending alpha p)
     std::function<void(Chunk&)> on_end;
 };
 SolveHooks& solve_hooks();
@@ -320,6 +321,8 @@ void learn_destroy(Learner* L);
 long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop,
                     unsigned long long seed);
 long learn_pairs(const Learner& L);
+void learn_set_consistent(Learner& L, bool on);
+void learn_set_phases(Learner& L, int mask);
 void learn_train(Learner& L, int steps, int batch, double lr, double* losses);
 void learn_set_active(Learner* L);
 struct UNet;

@@ -672,6 +672,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             }
             p_pending = true;
             error = op_error(c);
+            if (solve_hooks().on_cg) solve_hooks().on_cg(c, n);
         } else if (d.use_jacobi && jacobi_tb_supported(c)) {
             // iterations n .. n + K - 1 in one temporal-blocked pass, ending
             // at the next multiple of 50 (the residual check) at the latest;

@@ -246,6 +246,43 @@ long tl_learn_add_deck(void* h, const char* deck, int snapshots, int crops, int 
 
 long tl_learn_pairs(void* h) { return h ? learn_pairs(*static_cast<Learner*>(h)) : -1; }
 
+// Solve phases later pairs are sampled from: 1 the CG presteps, 2 the PPCG
+// outer iterations, 3 both (the default).  0, or -1 on error.
+int tl_learn_sample_phases(void* h, int mask)
+{
+    int rc = -1;
+    guarded([&] {
+        if (!h) throw Error("tl_learn_sample_phases: null learner");
+        learn_set_phases(*static_cast<Learner*>(h), mask);
+        rc = 0;
+    });
+    return rc;
+}
+
+// The A-norm training loss after `sweeps` weighted Jacobi sweeps of weight
+// omega (the flexible CG's smoothing after the network); 0 sweeps: the plain
+// A-norm.  0, or -1 on error.
+int tl_learn_loss_smoothing(void* h, int sweeps, double omega)
+{
+    int rc = -1;
+    guarded([&] {
+        if (!h) throw Error("tl_learn_loss_smoothing: null learner");
+        unet_set_loss_smoothing(learn_net(*static_cast<Learner*>(h)), sweeps, (float)omega);
+        rc = 0;
+    });
+    return rc;
+}
+
+// on != 0 (the default): a pair's input residual is A_c e, the crop
+// operator applied to the crop's correction; 0: the solve's own residual
+// r_k over the crop.  Applies to pairs added afterwards.
+int tl_learn_consistent_pairs(void* h, int on)
+{
+    if (!h) return -1;
+    learn_set_consistent(*static_cast<Learner*>(h), on != 0);
+    return 0;
+}
+
 int tl_learn_train(void* h, int steps, int batch, double lr, double* losses)
 {
     int rc = -1;

@@ -14,7 +14,7 @@
 // factor s, so the map stays linear in r as A^-1 is.  Training minimises the
 // A-norm of the error over each batch, sum (z - e).A(z - e) / sum e.A e.
 //
-// Flexible CG (Notay's FCG with one stored direction): z = s N(r / s),
+// Flexible CG (Notay's FCG with one stored direction): z = s G N(r / s),
 // p = z - (z.w_prev / p_prev.w_prev) p_prev, w = A p, alpha = p.r / p.w,
 // u += alpha p, r -= alpha w, all vectors and dot products in FP64.  It is
 // accepted only on the true residual ||u0 - A u||^2 < eps; when the
@@ -36,6 +36,8 @@ struct Learner {
     uint64_t rng = 0;
     int crop = 0;
     long pairs = 0;
+    bool consistent = true;                             // input residual A_c e of the crop (else r_k)
+    int phases = 3;                                     // snapshots from: 1 CG presteps, 2 PPCG outer iterations
     float *dr = nullptr, *df = nullptr, *de = nullptr;  // [pairs][crop][crop]
     float *dkx = nullptr, *dky = nullptr;               // the pair's face coefficients
     long dr_cap = 0, df_cap = 0, de_cap = 0, dkx_cap = 0, dky_cap = 0;
@@ -100,6 +102,33 @@ __global__ void k_extract(Geom g, const double* __restrict__ uk, const double* _
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, WAVE);
     if (threadIdx.x % WAVE == 0) atomicAdd(ss + q, s);
+}
+
+// The residual a crop's error leaves under the crop's own operator:
+// r = A_c e, A_c = I + sum over the faces inside the crop of k (e_i - e_j)
+// (no flux across the crop's edge; kx the west and ky the south face of each
+// cell), FP64 arithmetic on the stored FP32 values.  The pair (A_c e, e) is
+// then consistent: the target is exactly A_c^-1 of the input, a function of
+// what the network sees, whereas the solve's own r_k inside a crop also
+// carries the part of e driven by residual outside it.
+__global__ void k_crop_residual(int crop, const float* __restrict__ e, const float* __restrict__ kx,
+                                const float* __restrict__ ky, float* __restrict__ r)
+{
+    const int q = blockIdx.x;
+    const long base = (long)q * crop * crop;
+    const float* eq = e + base;
+    const float* kxq = kx + base;
+    const float* kyq = ky + base;
+    for (int i = threadIdx.x; i < crop * crop; i += blockDim.x) {
+        const int x = i % crop, y = i / crop;
+        const double c = eq[i];
+        double v = c;
+        if (x + 1 < crop) v += (double)kxq[i + 1] * (c - eq[i + 1]);
+        if (x > 0) v += (double)kxq[i] * (c - eq[i - 1]);
+        if (y + 1 < crop) v += (double)kyq[i + crop] * (c - eq[i + crop]);
+        if (y > 0) v += (double)kyq[i] * (c - eq[i - crop]);
+        r[base + i] = (float)v;
+    }
 }
 
 // r /= s, e /= s G over every crop of one snapshot (s the snapshot's rms residual)
@@ -235,6 +264,12 @@ Learner* active_learner() { return g_active; }
 void learn_set_active(Learner* L) { g_active = L; }
 UNet& learn_net(Learner& L) { return *L.net; }
 long learn_pairs(const Learner& L) { return L.pairs; }
+void learn_set_consistent(Learner& L, bool on) { L.consistent = on; }
+void learn_set_phases(Learner& L, int mask)
+{
+    if (mask < 1 || mask > 3) throw Error("snapshot phases: 1 (CG presteps), 2 (PPCG outer) or 3 (both)");
+    L.phases = mask;
+}
 
 Learner* learn_create(unsigned long long seed)
 {
@@ -278,8 +313,14 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         (void)hipFree(kx);
         (void)hipFree(ky);
     };
-    g_hooks.on_outer = [&](Chunk& c, int k) {
-        if (TL_DIM != 2 || (int)snaps.size() >= snapshots || k != kSchedule[snaps.size()]) return;
+    // `snapshots` iterates of each phase, at the schedule's iterations: the
+    // CG presteps (early, sharp residuals, like the flexible CG's own) and
+    // the PPCG outer iterations (late, smooth errors)
+    int taken[2] = {0, 0};
+    auto snap = [&](Chunk& c, int k, int phase) {
+        if (TL_DIM != 2 || !(L.phases & (1 << phase)) || taken[phase] >= snapshots || k != kSchedule[taken[phase]])
+            return;
+        ++taken[phase];
         const size_t bytes = sizeof(double) * (size_t)c.g.alloc;
         Snap s;
         TL_HIP(hipMalloc(&s.u, bytes));
@@ -288,6 +329,8 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         TL_HIP(hipMemcpyAsync(s.r, c.f[F_R], bytes, hipMemcpyDeviceToDevice, c.stream));
         snaps.push_back(s);
     };
+    g_hooks.on_cg = [&](Chunk& c, int n) { snap(c, n, 0); };
+    g_hooks.on_outer = [&](Chunk& c, int k) { snap(c, k, 1); };
     g_hooks.on_end = [&](Chunk& c) {
         if (seen) return;  // the first timestep only
         seen = true;
@@ -339,6 +382,11 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         hipLaunchKernelGGL(k_extract, dim3(crops), dim3(256), 0, nullptr, geom, snaps[k].u, ustar, snaps[k].r, dens, kx,
                            ky, dxy, crop, L.dr + off, L.df + off, L.de + off, L.dkx + off, L.dky + off, L.scratch);
         launched("training pair extraction");
+        if (L.consistent) {
+            hipLaunchKernelGGL(k_crop_residual, dim3(crops), dim3(256), 0, nullptr, crop, L.de + off, L.dkx + off,
+                               L.dky + off, L.dr + off);
+            launched("training pair residual");
+        }
         // the snapshot's residual rms over the whole mesh scales r and e of
         // all its crops, as the flexible CG scales the network's input
         double* fs = L.scratch + crops;
@@ -494,7 +542,10 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
             launched("FCG input");
             unet_apply(*L->net, L->pr, pf, pz, 1, c.g.ny, c.g.nx);
-            hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s, c.f[F_Z]);
+            // the network predicts e / (s G): undo both scales
+            const double g_scale = L->escale > 0.0 ? L->escale : 1.0;
+            hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s * g_scale,
+                               c.f[F_Z]);
             launched("FCG output");
             beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, false, true, !first);
         } else {

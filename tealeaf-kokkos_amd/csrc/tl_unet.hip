@@ -419,6 +419,127 @@ __global__ __launch_bounds__(256) void k_anorm_grad(const float* kx, const float
     if (q == 0 && threadIdx.x == 0) *loss = sums[0] * inv;
 }
 
+// Diagonal of a sample's operator: 1 + the coefficients of the cell's faces
+// inside the sample.
+__device__ inline float a_diag(const float* kx, const float* ky, int x, int y, int h, int w)
+{
+    const int i = y * w + x;
+    float d = 1.0f;
+    if (x + 1 < w) d += kx[i + 1];
+    if (x > 0) d += kx[i];
+    if (y + 1 < h) d += ky[i + w];
+    if (y > 0) d += ky[i];
+    return d;
+}
+
+// A-norm loss after the preconditioner's smoothing: the flexible CG applies
+// m weighted Jacobi sweeps (weight omega) on A z = r starting from the
+// network's output, which multiply the output's error d = out - t by
+// M = (I - omega D^-1 A)^m.  Per sample, sum(Md.A Md) / den with
+// den = max(sum(Mt.A Mt), its batch mean); the loss is the mean over the n
+// samples (at most 1 for a zero output: the sweeps alone), so a sample of
+// large error does not drown the others and one the sweeps already remove
+// does not divide by ~0; dout = 2 M^T A M d / (n den),
+// M^T = (I - omega A D^-1)^m.  buf holds four planes of the
+// batch: d and its ping-pong partner, t and its partner.  First pass, one
+// workgroup per sample: M d (left in plane m % 2) and the sample's two sums
+// (sums[2q], sums[2q + 1]).
+__global__ __launch_bounds__(256) void k_anorm_smooth_sums(const u16* out, int cout, const float* target,
+                                                           const float* kx, const float* ky, float* buf, long plane,
+                                                           int h, int w, int m, float omega, float* sums)
+{
+    __shared__ float red[2][256 / WAVE];
+    const int q = blockIdx.x;
+    const long px = (long)h * w, base = (long)q * px;
+    const float* kxq = kx + base;
+    const float* kyq = ky + base;
+    float* d[2] = {buf + base, buf + plane + base};
+    float* t[2] = {buf + 2 * plane + base, buf + 3 * plane + base};
+    for (long i = threadIdx.x; i < px; i += blockDim.x) {
+        d[0][i] = bf2f(out[(base + i) * cout]) - target[base + i];
+        t[0][i] = target[base + i];
+    }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        const int a = j & 1, b = a ^ 1;
+        for (long i = threadIdx.x; i < px; i += blockDim.x) {
+            const int x = (int)(i % w), y = (int)(i / w);
+            const float s = omega / a_diag(kxq, kyq, x, y, h, w);
+            d[b][i] = d[a][i] - s * a_apply(d[a], kxq, kyq, x, y, h, w);
+            t[b][i] = t[a][i] - s * a_apply(t[a], kxq, kyq, x, y, h, w);
+        }
+        __syncthreads();
+    }
+    const float* dm = d[m & 1];
+    const float* tm = t[m & 1];
+    float num = 0.0f, den = 0.0f;
+    for (long i = threadIdx.x; i < px; i += blockDim.x) {
+        const int x = (int)(i % w), y = (int)(i / w);
+        num += dm[i] * a_apply(dm, kxq, kyq, x, y, h, w);
+        den += tm[i] * a_apply(tm, kxq, kyq, x, y, h, w);
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        num += __shfl_xor(num, off, WAVE);
+        den += __shfl_xor(den, off, WAVE);
+    }
+    if (threadIdx.x % WAVE == 0) {
+        red[0][threadIdx.x / WAVE] = num;
+        red[1][threadIdx.x / WAVE] = den;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        num = den = 0.0f;
+        for (int k = 0; k < (int)blockDim.x / WAVE; ++k) {
+            num += red[0][k];
+            den += red[1][k];
+        }
+        sums[2 * q] = num;
+        sums[2 * q + 1] = den;
+    }
+}
+
+// Second pass: g = 2 A (M d) / (n sum(Mt.A Mt)) per sample, then M^T g by m steps
+// g <- g - omega A (D^-1 g) (planes 2 and 3 hold g and D^-1 g), into dout.
+__global__ __launch_bounds__(256) void k_anorm_smooth_grad(const float* kx, const float* ky, float* buf, long plane,
+                                                           u16* dout, int cout, int h, int w, int m, float omega,
+                                                           const float* sums, float* loss)
+{
+    const int q = blockIdx.x;
+    const long px = (long)h * w, base = (long)q * px;
+    const float* kxq = kx + base;
+    const float* kyq = ky + base;
+    const float* dm = buf + (m & 1) * plane + base;
+    float* g = buf + 2 * plane + base;
+    float* v = buf + 3 * plane + base;
+    // each sample's denominator floored at the batch mean, so a sample whose
+    // target the sweeps already remove does not divide by ~0
+    float mean = 0.0f;
+    for (int k = 0; k < (int)gridDim.x; ++k) mean += sums[2 * k + 1];
+    mean /= (float)gridDim.x;
+    const float den = fmaxf(sums[2 * q + 1], mean);
+    const float inv = den > 0.0f ? 1.0f / (den * (float)gridDim.x) : 0.0f;
+    for (long i = threadIdx.x; i < px; i += blockDim.x) {
+        const int x = (int)(i % w), y = (int)(i / w);
+        g[i] = 2.0f * a_apply(dm, kxq, kyq, x, y, h, w) * inv;
+    }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        for (long i = threadIdx.x; i < px; i += blockDim.x) {
+            const int x = (int)(i % w), y = (int)(i / w);
+            v[i] = g[i] / a_diag(kxq, kyq, x, y, h, w);
+        }
+        __syncthreads();
+        for (long i = threadIdx.x; i < px; i += blockDim.x) {
+            const int x = (int)(i % w), y = (int)(i / w);
+            g[i] -= omega * a_apply(v, kxq, kyq, x, y, h, w);
+        }
+        __syncthreads();
+    }
+    for (long i = threadIdx.x; i < px; i += blockDim.x)
+        for (int c = 0; c < cout; ++c) dout[(base + i) * cout + c] = c == 0 ? f2bf(g[i]) : (u16)0;
+    if (threadIdx.x == 0) atomicAdd(loss, sums[2 * q] * inv);
+}
+
 __global__ void k_adam(float* w, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
                        float bc1, float bc2)
 {
@@ -494,9 +615,12 @@ struct UNet {
     UBuf act[NSLOT], grad[NSLOT];
     float* target = nullptr;
     long target_cap = 0;
-    float* dbuf = nullptr;  // A-norm loss: out - target
+    float* dbuf = nullptr;  // A-norm loss: out - target (and the smoothed loss's work planes)
     long dbuf_cap = 0;
-    float* sums = nullptr;  // A-norm loss: sum(d.A d), sum(t.A t)
+    int smooth = 4;         // A-norm loss measured after this many weighted Jacobi sweeps
+    float omega = 0.8f;     // their weight
+    float* sums = nullptr;  // A-norm loss: sum(d.A d), sum(t.A t) (per sample for the smoothed loss)
+    long sums_cap = 2;
     float* loss = nullptr;
     float* stage_r = nullptr;  // input staging (FP32 planes)
     float* stage_f = nullptr;
@@ -801,16 +925,28 @@ float UNet::loss_and_backward(int n, int h, int w, const float* kx, const float*
     }
     TL_HIP(hipMemsetAsync(loss, 0, sizeof(float), stream));
     if (kx && ky) {
-        if (p0 > dbuf_cap) {
+        if (4 * p0 > dbuf_cap) {
             (void)hipFree(dbuf);
-            dbuf = dalloc<float>(p0);
-            dbuf_cap = p0;
+            dbuf = dalloc<float>(4 * p0);
+            dbuf_cap = 4 * p0;
         }
         TL_HIP(hipMemsetAsync(sums, 0, sizeof(float) * 2, stream));
-        hipLaunchKernelGGL(k_anorm_sums, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky, dbuf, h, w,
-                           sums);
-        hipLaunchKernelGGL(k_anorm_grad, dim3(n), dim3(256), 0, stream, kx, ky, dbuf, grad[OUT].d, 16, h, w, sums,
-                           loss);
+        if (smooth > 0) {
+            if (2L * n > sums_cap) {
+                (void)hipFree(sums);
+                sums = dalloc<float>(2L * n);
+                sums_cap = 2L * n;
+            }
+            hipLaunchKernelGGL(k_anorm_smooth_sums, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky,
+                               dbuf, p0, h, w, smooth, omega, sums);
+            hipLaunchKernelGGL(k_anorm_smooth_grad, dim3(n), dim3(256), 0, stream, kx, ky, dbuf, p0, grad[OUT].d, 16,
+                               h, w, smooth, omega, sums, loss);
+        } else {
+            hipLaunchKernelGGL(k_anorm_sums, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky, dbuf, h,
+                               w, sums);
+            hipLaunchKernelGGL(k_anorm_grad, dim3(n), dim3(256), 0, stream, kx, ky, dbuf, grad[OUT].d, 16, h, w, sums,
+                               loss);
+        }
     } else {
         hipLaunchKernelGGL(k_mse, dim3(grid_for(p0)), dim3(256), 0, stream, act[OUT].d, 16, target, grad[OUT].d, p0,
                            loss);
@@ -905,6 +1041,14 @@ float unet_loss_grad(UNet& u, const float* r, const float* f, const float* targe
 }
 
 void unet_adam(UNet& u, float lr) { u.adam(lr); }
+
+void unet_set_loss_smoothing(UNet& u, int sweeps, float omega)
+{
+    if (sweeps < 0 || sweeps > 16) throw Error("loss smoothing: sweeps must be in [0, 16]");
+    if (!(omega > 0.0f && omega <= 1.0f)) throw Error("loss smoothing: omega must be in (0, 1]");
+    u.smooth = sweeps;
+    u.omega = omega;
+}
 
 double unet_forward_macs(int n, int h, int w)
 {

@@ -18,11 +18,17 @@ void unet_apply(UNet& u, const float* r, const float* f, float* z, int n, int h,
 // parameter; returns the loss.  With kx, ky (per-sample face coefficients,
 // kx the west face and ky the south face of each pixel) the loss is the
 // A-norm sum(d.A d) / sum(t.A t) of d = out - target over the batch, A the
-// 5-point operator with no flux across the sample's edge; without them the
-// mean squared error.
+// 5-point operator with no flux across the sample's edge (after smoothing:
+// see unet_set_loss_smoothing); without them the mean squared error.
 float unet_loss_grad(UNet& u, const float* r, const float* f, const float* target, const float* kx, const float* ky,
                      int n, int h, int w);
 void unet_adam(UNet& u, float lr);
+// The A-norm loss measures the error after `sweeps` weighted Jacobi sweeps
+// (weight omega) of the output, as the flexible CG applies them after the
+// network: per sample sum(Md.A Md) / max(sum(Mt.A Mt), batch mean), averaged,
+// M = (I - omega D^-1 A)^sweeps.
+// sweeps = 0 gives the plain A-norm loss.  Default 4 sweeps, omega 0.8.
+void unet_set_loss_smoothing(UNet& u, int sweeps, float omega);
 void unet_set_stream(UNet& u, hipStream_t s);  // the stream all kernels of u run on
 hipStream_t unet_stream(UNet& u);
 long unet_param_count(const UNet& u);

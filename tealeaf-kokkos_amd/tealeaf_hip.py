@@ -82,6 +82,9 @@ def lib(dim: int) -> ctypes.CDLL:
             "tl_learn_add_deck": (ctypes.c_long, [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_ulonglong]),
             "tl_learn_pairs": (ctypes.c_long, [vp]),
+            "tl_learn_consistent_pairs": (ctypes.c_int, [vp, ctypes.c_int]),
+            "tl_learn_loss_smoothing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
+            "tl_learn_sample_phases": (ctypes.c_int, [vp, ctypes.c_int]),
             "tl_learn_train": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]),
             "tl_learn_use": (ctypes.c_int, [vp]),
             "tl_learn_param_count": (ctypes.c_long, [vp]),
@@ -326,6 +329,19 @@ class Learner:
 
     def add_deck(self, deck: str, snapshots: int = 12, crops: int = 64, crop: int = 64, seed: int = 0) -> int:
         return self._check(self.L.tl_learn_add_deck(self.h, deck.encode(), snapshots, crops, crop, seed))
+
+    def consistent_pairs(self, on: bool = True) -> None:
+        """Input residual of later pairs: A_c e of the crop (default) or the
+        solve's r_k."""
+        self._check(self.L.tl_learn_consistent_pairs(self.h, 1 if on else 0))
+
+    def sample_phases(self, cg: bool = True, outer: bool = True) -> None:
+        """Solve phases later pairs are sampled from."""
+        self._check(self.L.tl_learn_sample_phases(self.h, (1 if cg else 0) | (2 if outer else 0)))
+
+    def loss_smoothing(self, sweeps: int = 4, omega: float = 0.8) -> None:
+        """Measure the A-norm loss after `sweeps` weighted Jacobi sweeps."""
+        self._check(self.L.tl_learn_loss_smoothing(self.h, sweeps, omega))
 
     @property
     def pairs(self) -> int:

@@ -82,16 +82,28 @@ def test_gradients_match_oracle(learner, seed):
     print(f"  worst layer relative L2 {worst:.3e}", flush=True)
 
 
-def test_anorm_gradients_match_oracle(learner):
-    """The training loss: A-norm of the error with per-pixel face coefficients."""
-    print("unet A-norm gradients vs oracle", flush=True)
+@pytest.mark.parametrize("sweeps", [0, 4])
+def test_anorm_gradients_match_oracle(learner, sweeps):
+    """The training loss: A-norm of the error with per-pixel face
+    coefficients, plain (0 sweeps) and measured after the flexible CG's 4
+    weighted Jacobi sweeps (the default)."""
+    print(f"unet A-norm gradients vs oracle, {sweeps} sweeps", flush=True)
     params = learner.params()
     r, f, t = _inputs(4)
     rng = np.random.default_rng(40)
     kx, ky = rng.uniform(0.05, 20.0, size=(2,) + r.shape).astype(np.float32)
-    loss_g, grad_g = learner.loss_grad(r, f, t, kx, ky)
-    loss_o, grad_o = U.loss_grad(params, r, f, t, bf=True, kx=kx.astype(np.float64), ky=ky.astype(np.float64))
+    learner.loss_smoothing(sweeps, 0.8)
+    try:
+        loss_g, grad_g = learner.loss_grad(r, f, t, kx, ky)
+    finally:
+        learner.loss_smoothing(4, 0.8)
+    loss_o, grad_o = U.loss_grad(params, r, f, t, bf=True, kx=kx.astype(np.float64), ky=ky.astype(np.float64),
+                                 sweeps=sweeps, omega=0.8)
     assert abs(loss_g - loss_o) <= 1e-2 * loss_o
+    # After the sweeps the deep layers' gradients are small differences: BF16
+    # rounding alone moves them by up to 11% here (the oracle with and without
+    # BF16 rounding), so a flipped rounding weighs more; bar 10%, cosine 0.99.
+    rtol, cos_min = (GRAD_RTOL, GRAD_COS) if sweeps == 0 else (1e-1, 0.99)
     worst = 0.0
     for i, ((gw, gb), (ow, ob)) in enumerate(zip(_layers(grad_g), _layers(grad_o))):
         a = np.concatenate([gw, gb]).astype(np.float64)
@@ -100,8 +112,9 @@ def test_anorm_gradients_match_oracle(learner):
         if nb == 0.0:
             continue
         rel = np.linalg.norm(a - b) / nb
+        cos = float(a @ b / (np.linalg.norm(a) * nb))
         worst = max(worst, rel)
-        assert rel <= GRAD_RTOL and float(a @ b / (np.linalg.norm(a) * nb)) >= GRAD_COS, (i, rel)
+        assert rel <= rtol and cos >= cos_min, (i, rel, cos)
     print(f"  worst layer relative L2 {worst:.3e}", flush=True)
 
 

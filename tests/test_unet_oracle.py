@@ -72,8 +72,29 @@ def test_pool_and_upsample_adjoints():
     assert np.isclose(np.sum(U.up(g) * h), np.sum(g * U.up_bwd(h)))
 
 
-@pytest.mark.parametrize("seed,anorm", [(1, False), (2, False), (3, True)])
-def test_backward_matches_finite_differences(seed, anorm):
+def test_smoothing_operator_and_its_transpose():
+    """smooth() is the error propagation of weighted Jacobi sweeps on the
+    assembled operator (M = (I - omega D^-1 A)^m with D = diag(A)) and
+    smooth_t() is its transpose: <M v, g> = <v, M^T g>."""
+    rng = np.random.default_rng(8)
+    kx, ky = rng.uniform(0.1, 4.0, size=(2, 1, 5, 6))
+    n = 30
+    A = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros((1, 5, 6))
+        e.flat[i] = 1.0
+        A[:, i] = U.a_apply(e, kx, ky).ravel()
+    D = np.diag(np.diag(A))
+    M = np.linalg.matrix_power(np.eye(n) - 0.7 * np.linalg.solve(D, A), 3)
+    v = rng.normal(size=(1, 5, 6))
+    g = rng.normal(size=(1, 5, 6))
+    assert np.allclose(U.smooth(v, kx, ky, 3, 0.7).ravel(), M @ v.ravel(), rtol=1e-12, atol=1e-12)
+    assert np.allclose(U.smooth_t(g, kx, ky, 3, 0.7).ravel(), M.T @ g.ravel(), rtol=1e-12, atol=1e-12)
+    assert np.allclose(U.a_diag(kx, ky).ravel(), np.diag(A))
+
+
+@pytest.mark.parametrize("seed,anorm,sweeps", [(1, False, 0), (2, False, 0), (3, True, 0), (4, True, 3)])
+def test_backward_matches_finite_differences(seed, anorm, sweeps):
     rng = np.random.default_rng(seed)
     params = U.init_params(seed).astype(np.float64)
     # positive biases keep the deep 1x1 levels of an 8x8 input alive
@@ -82,13 +103,14 @@ def test_backward_matches_finite_differences(seed, anorm):
         k += cout * 9 * cin
         params[k:k + cout] = 0.05
         k += cout
-    r = rng.normal(size=(1, 8, 8))
-    f = rng.uniform(-1, 1, size=(1, 8, 8))
-    t = rng.normal(size=(1, 8, 8))
+    ns = 2 if sweeps else 1  # two samples pin the per-sample normalisation
+    r = rng.normal(size=(ns, 8, 8))
+    f = rng.uniform(-1, 1, size=(ns, 8, 8))
+    t = rng.normal(size=(ns, 8, 8)) * np.arange(1, ns + 1)[:, None, None]
     kx = ky = None
     if anorm:
-        kx, ky = rng.uniform(0.1, 5.0, size=(2, 1, 8, 8))
-    loss, grad = U.loss_grad(params, r, f, t, bf=False, kx=kx, ky=ky)
+        kx, ky = rng.uniform(0.1, 5.0, size=(2, ns, 8, 8))
+    loss, grad = U.loss_grad(params, r, f, t, bf=False, kx=kx, ky=ky, sweeps=sweeps)
     # parameters spread over all layers, away from unused input channels
     picks, k = [], 0
     for i, (cin, cout, _) in enumerate(U.SHAPES):
@@ -104,8 +126,8 @@ def test_backward_matches_finite_differences(seed, anorm):
         p1, p2 = params.copy(), params.copy()
         p1[j] += h
         p2[j] -= h
-        return (U.loss_grad(p1, r, f, t, bf=False, kx=kx, ky=ky)[0]
-                - U.loss_grad(p2, r, f, t, bf=False, kx=kx, ky=ky)[0]) / (2 * h)
+        return (U.loss_grad(p1, r, f, t, bf=False, kx=kx, ky=ky, sweeps=sweeps)[0]
+                - U.loss_grad(p2, r, f, t, bf=False, kx=kx, ky=ky, sweeps=sweeps)[0]) / (2 * h)
 
     for j in picks:
         a, b = fd(j, 1e-6), fd(j, 2e-7)
