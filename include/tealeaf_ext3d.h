@@ -162,11 +162,9 @@ int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn);
  * with tl_last_error() set. */
 int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
 
-/* The flexible CG's preconditioner (no counterpart in the reference): z =
- * `sweeps` weighted Jacobi sweeps z += omega (r - A z) / D on the chunk's z
- * and r fields (from z = 0 with from_zero), as one temporal-blocked pass
- * (fused = 1; 2D, 4 sweeps) or one launch per sweep (fused = 0).  Returns 0,
- * or -1 with tl_last_error() set. */
+/* The flexible CG's preconditioner sweeps (no counterpart in the reference)
+ * are 2D only (see tealeaf_ext2d.h): the 3D library exports the symbol, and a
+ * call returns -1 with tl_last_error() set. */
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
 /* The sweeps-only flexible CG's preconditioner pass with the previous
  * step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four

@@ -269,7 +269,7 @@ void op_fcg_update_p(Chunk& c, double beta);
 // S_ERR); with defer_r this iteration's r update is deferred the same way and
 // S_RRN gets nothing.
 void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending, bool beta_rrn = false,
-                          bool defer_r = false);
+                          bool defer_r = false, bool alpha_zr = true);
 
 // Observers of the driver's FP64 PPCG solve (no counterpart in the
 // reference): after every outer iteration (u, r current) and at the end of

@@ -215,7 +215,11 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = d.use_fcg = false; d.use_ppcg = d.ppcg_mixed = true; }
             else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
             else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
-            else if (k == "tl_fcg_omega") d.fcg.omega = R();
+            else if (k == "tl_fcg_omega") {
+                d.fcg.omega = R();
+                // beyond 1 the sweeps stop being an SPD preconditioner (and can diverge)
+                if (!(d.fcg.omega > 0.0 && d.fcg.omega <= 1.0)) throw Error("tl_fcg_omega must lie in (0, 1]");
+            }
             else if (k == "tl_fcg_no_network") d.fcg.network = false;
             else if (k == "tl_fcg_net_every") d.fcg.net_every = I();
             else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }

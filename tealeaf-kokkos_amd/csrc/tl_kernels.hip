@@ -708,9 +708,11 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ par
         out[1] = ds->bad;
         break;
     case EP_FCG_PW_ZR:  // flexible CG, sweeps only: alpha = (z . r) / (p . w)
+        // z . r > 0 for the SPD sweep preconditioner and r != 0: anything
+        // else is a breakdown and stops the solve
         ds->pw = v[0];
         ds->alpha = ds->zr / v[0];
-        ds->bad = (v[0] > 0.0 && isfinite(ds->zr)) ? 0.0 : 1.0;
+        ds->bad = (v[0] > 0.0 && ds->zr > 0.0 && isfinite(ds->zr)) ? 0.0 : 1.0;
         break;
     case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
         const double beta = v[0] / ds->rro_prev;
@@ -1829,7 +1831,8 @@ void op_fcg_update_p(Chunk& c, double beta) { STRIP_LAUNCH(k_fcg_p, c, beta, nul
 // ||r||^2 (it applied that iteration's r update), read from S_ERR.  defer_r:
 // leave this iteration's r -= alpha w to the next preconditioner pass (2D);
 // no rrn reaches S_RRN then.
-void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending, bool beta_rrn, bool defer_r)
+void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pending, bool beta_rrn, bool defer_r,
+                          bool alpha_zr)
 {
     if (beta_rrn) {
         if (first || beta_blocks <= 0) throw Error("flexible CG: no preconditioner partials for beta and rrn");
@@ -1853,8 +1856,10 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         // with u_pending the previous iteration's u += alpha p rides along
         // (alpha and the old p are still in place).  With beta_rrn the
         // preconditioner pass left z . r, alpha's numerator for the fixed
-        // (symmetric) sweep preconditioner, and the strip reads no r.
-        if (beta_rrn) {
+        // (symmetric) sweep preconditioner, and the strip reads no r (unless
+        // alpha_zr is off: after a residual replacement p_prev . r = 0, on
+        // which z . r = p . r rests, no longer holds).
+        if (beta_rrn && alpha_zr) {
             TIMED(c, K_FCG_STRIP_ZR, {
                 if (u_pending)
                     hipLaunchKernelGGL((k_cg_pw_strip<true, false>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream,

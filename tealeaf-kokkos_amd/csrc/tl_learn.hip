@@ -222,6 +222,8 @@ unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, std::min<long>(
 // per sweep.  The result is left in F_Z.
 long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused, bool dot_w)
 {
+    // the sweep kernels are the 5-point operator on one plane
+    if (TL_DIM != 2) throw Error("the flexible CG's preconditioner sweeps are 2D only");
     if (sweeps < 1) {
         if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
         return 0;
@@ -498,6 +500,8 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     };
     int since_best = 0;
     const int window = 200;
+    bool replaced = false;  // check() put the true residual in r
+    bool pr_next = false;   // so the next strip takes alpha from p . r
     // returns 1 to accept (u current, true residual below eps), -1 stalled
     auto check = [&](double r2, int it) -> int {
         iters = it;
@@ -511,6 +515,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
                 error = r2;
                 return 1;
             }
+            replaced = true;
         }
         rrn = r2;
         if (r2 < 0.5 * best) {
@@ -553,7 +558,8 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         }
         // beta, p, w, alpha, u[, r] with the scalars on the device; one read
         const bool defer_r = defer && it < max_iters;
-        op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r);
+        op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r, !pr_next);
+        pr_next = false;
         first = false;
         if (beta_rrn) {
             // iteration it - 1 is complete: r updated by the pass, u by the
@@ -565,6 +571,8 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             const int v = check(r2, it - 1);
             if (v != 0) return v > 0;
             u_pending = pend;
+            pr_next = replaced;
+            replaced = false;
         }
         if (defer_r) {
             r_pending = true;

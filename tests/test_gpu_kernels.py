@@ -428,6 +428,29 @@ def test_stencil_refused_with_neighbour_chunks(hip, oracle):
     g.close()
 
 
+def jacobi_sweeps_numpy(r, z, kx, ky, omega, sweeps):
+    """Weighted Jacobi sweeps z <- z + omega (r - A z) / D on the mesh cells,
+    A = I + sum over interior faces of k (e_i - e_j)(e_i - e_j)^T (a face on
+    the mesh boundary carries no flux) and D = diag(A); arrays in the
+    reference's flat layout, (1, ny + 4, nx + 4), kx the west and ky the south
+    face of each cell.  Written from the operator, independently of the
+    kernels."""
+    R = r[0, 2:-2, 2:-2]
+    Z = z[0, 2:-2, 2:-2].copy()
+    kw, ke = kx[0, 2:-2, 2:-2].copy(), kx[0, 2:-2, 3:-1].copy()
+    ks, kn = ky[0, 2:-2, 2:-2].copy(), ky[0, 3:-1, 2:-2].copy()
+    kw[:, 0] = 0.0
+    ke[:, -1] = 0.0
+    ks[0, :] = 0.0
+    kn[-1, :] = 0.0
+    D = 1.0 + kw + ke + ks + kn
+    for _ in range(sweeps):
+        P = np.pad(Z, 1, mode="edge")
+        AZ = Z + kw * (Z - P[1:-1, :-2]) + ke * (Z - P[1:-1, 2:]) + ks * (Z - P[:-2, 1:-1]) + kn * (Z - P[2:, 1:-1])
+        Z = Z + omega * (R - AZ) / D
+    return Z
+
+
 @pytest.mark.parametrize("shape", [(37, 23), (5, 4), (9, 7), (131, 6), (130, 41), (2, 17), (64, 64)])
 @pytest.mark.parametrize("from_zero", [True, False])
 def test_fcg_smooth_fused(hip, shape, from_zero):
@@ -451,6 +474,10 @@ def test_fcg_smooth_fused(hip, shape, from_zero):
         out.append(interior(g, g.get("z")))
         g.close()
     np.testing.assert_array_equal(out[0], out[1])
+    z0 = np.zeros_like(vals["z"]) if from_zero else vals["z"]
+    want = jacobi_sweeps_numpy(vals["r"], z0, vals["kx"], vals["ky"], 0.8, 4)
+    got = out[1].reshape(want.shape)
+    assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
 
 
 @pytest.mark.parametrize("shape", [(37, 23), (5, 4), (131, 6), (130, 41), (2, 17), (64, 64)])
@@ -486,9 +513,22 @@ def test_fcg_smooth_update(hip, shape):
     z_ref = interior(g, g.get("z"))
     g.close()
     np.testing.assert_array_equal(z_got, z_ref)
+    want = jacobi_sweeps_numpy(r_new, np.zeros_like(r_new), kx, ky, omega, 4)
+    assert np.max(np.abs(z_got.reshape(want.shape) - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
 
     wi, ri = interior(g, w), interior(g, r_new)
     want = [float(np.sum(z_ref * wi)), float(np.sum(ri * ri)), float(np.sum(z_ref * ri))]
     for k in range(3):
         assert abs(sums[k] - want[k]) <= 1e-12 * max(1.0, float(np.sum(np.abs(z_ref * wi))), float(np.sum(ri * ri))), \
             (k, sums[k], want[k])
+
+
+def test_fcg_smooth_refused_in_3d(hip):
+    """The flexible CG's sweeps are 2D only: the 3D library refuses them."""
+    import ctypes
+    g = hip.ExtChunk(hip.lib(3), 3, 6, 5, 4)
+    rc = g.L.tl_fcg_smooth(g.c, ctypes.c_int(4), ctypes.c_double(0.8), ctypes.c_int(1), ctypes.c_int(0))
+    assert rc == -1
+    assert b"2D only" in g.L.tl_last_error()
+    g.L.tl_clear_error()
+    g.close()
