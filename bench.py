@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark: TeaLeaf's linear solve on one MI355X.
 
-Workload c2 (default, the 2D 4096^2 configuration): the two-rectangle layout
-on a 10x10 domain, FP64 PPCG (30 CG presteps for the eigenvalue estimate,
-10 inner Chebyshev steps), eps 1e-10, dt 0.004.  A step is one timestep,
-i.e. one ``tea_leaf()`` solve from its first halo update to the finalised
-energy (the reference's "Tea Solve" time per step).  --warmup W timesteps
-run untimed first, then --steps K are timed; the deck runs W + K timesteps
-and the driver times each solve with a device synchronisation at its end.
-c3 is the 3D 256^3 cube, c4 the 2D 8192^2 layout of 32 seeded random
-rectangles; --solver picks CG, Chebyshev or Jacobi instead of PPCG.
+Default workload c4 (the 2D 8192^2 configuration the target is stated on):
+32 seeded random rectangles (splitmix64 seed 0, density log-uniform in
+[0.1, 100], energy uniform in [1e-4, 25]) on a 10x10 domain, eps 1e-10,
+dt 0.004.  A step is one timestep, i.e. one ``tea_leaf()`` solve from its
+first halo update to the finalised energy (the reference's "Tea Solve" time
+per step, tealeaf_2d/diffuse.f90:149).  --warmup W timesteps run untimed
+first, then --steps K are timed; the driver times each solve with a device
+synchronisation at its end.
 
-Printed: one JSON line with the mean seconds per timestep of the FP64 solve
-over the K timed steps (`value`), the roofline of its dominant kernel (its
-algorithmic bytes per launch over its mean launch time, from HIP events on
-the chunk's stream during the timed steps), under `forms` the
-mixed-precision PPCG solve of the same deck (time, speedup, per-step error
-against the FP64 solve, true residual), and a CPU baseline from the oracle
-(see cpu_baseline()).
+`value` is the FP64 PPCG solve (30 CG presteps, 10 inner Chebyshev steps:
+the reference's solver, rebuilt) in seconds per timestep.  Under `forms`
+the same deck and timesteps are solved by the forms the target compares
+against it: mixed-precision PPCG (FP32 inner steps), learned flexible CG
+(BF16 U-Net + 4 weighted Jacobi sweeps, trained first), the same flexible
+CG without the network (the ablation), and FP64 PPCG accepted on the true
+residual ||u0 - A u||^2 (the equal-accuracy baseline: the reference's
+PPCG stops on its recurrence residual).  `north_star` sums up the speedups
+against the 1.5x target.  The line also carries the roofline of the FP64
+solve's dominant kernel (algorithmic bytes per launch over its mean
+HIP-event launch time) and a CPU baseline from the oracle (see
+cpu_baseline()).  c2 is the 4096^2 two-rectangle deck, c3 the 3D 256^3
+cube, c5 the 4096^2 learned-FCG configuration; --solver picks CG,
+Chebyshev or Jacobi instead of PPCG for c2/c3.
 """
 from __future__ import annotations
 
@@ -39,27 +45,58 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def cpu_baseline(n: int, threads: int, steps: list[dict], budget_s: float = 20.0) -> dict:
+def cpu_info() -> dict:
+    """The host CPU as the baseline uses it: model name, physical cores in
+    this process's CPU set, and the OpenMP thread cap of the environment."""
+    model = ""
+    cores = set()
+    allowed = os.sched_getaffinity(0)
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = phys = core = None
+            for ln in f.read().splitlines() + [""]:
+                if ln.startswith("processor"):
+                    cpu = int(ln.split(":")[1])
+                elif ln.startswith("model name") and not model:
+                    model = ln.split(":", 1)[1].strip()
+                elif ln.startswith("physical id"):
+                    phys = ln.split(":")[1].strip()
+                elif ln.startswith("core id"):
+                    core = ln.split(":")[1].strip()
+                elif not ln.strip() and cpu is not None:
+                    if cpu in allowed:
+                        cores.add((phys, core))
+                    cpu = phys = core = None
+    except OSError:
+        pass
+    physical = len(cores) or len(allowed)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"model": model, "physical_cores": physical, "logical_cpus": len(allowed),
+            "threads": min(physical, cap) if cap > 0 else physical}
+
+
+def cpu_baseline(deck: str, n: int, threads: int, steps: list[dict], budget_s: float = 20.0) -> dict:
     """The oracle (C/OpenMP restatement of the reference) timed on a bounded
-    sample of the same workload: on the 4096^2 chunk in its first-timestep
-    state it runs k CG iterations and m PPCG outer iterations (each with the
-    halo updates the reference's driver issues), sized to about `budget_s`
+    sample of the same workload: the deck's n^2 chunk in its first-timestep
+    state (layout rebuilt through ext_generate_chunk_ from the deck's states)
+    runs k CG iterations and m PPCG outer iterations (each with the halo
+    updates the reference's driver issues), sized to about `budget_s`
     seconds.  Seconds per timestep = init + cg_steps * t_cg + outer * t_outer
     with the iteration counts of the timed GPU steps (the per-step counts the
     parity tests show the oracle reproduces)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
+    info = cpu_info()
+    threads = threads or info["threads"]
     L = oracle.lib(2)
     L.tl_oracle_set_reduce_parts(threads)
     L.tl_oracle_set_threads(threads)
     ch = tealeaf_hip.ExtChunk(L, 2, n, n)
-    dx = 10.0 / n
-    s = dx / 100.0
-    ch.initialise_chunk(0.0, 0.0, dx, dx)
-    ch.generate_chunk(density=[100.0, 0.1, 0.1], energy=[1e-4, 25.0, 0.1],
-                      xmin=[0, 0 + s, 1 + s], xmax=[0, 1 - s, 6 - s], ymin=[0, 0 + s, 1 + s],
-                      ymax=[0, 2 - s, 6 - s], radius=[0, 0, 0], geometry=[0, 1, 1])
+    st = decks.states(deck)
+    ch.initialise_chunk(st["grid_xmin"], st["grid_ymin"], st["dx"], st["dy"])
+    ch.generate_chunk(density=st["density"], energy=st["energy"], xmin=st["xmin"], xmax=st["xmax"],
+                      ymin=st["ymin"], ymax=st["ymax"], radius=st["radius"], geometry=st["geometry"])
     ch.update_halo([1, 2, 3], 2)
     ch.set_field_kernel()
     t0 = time.perf_counter()
@@ -110,16 +147,20 @@ def cpu_baseline(n: int, threads: int, steps: list[dict], budget_s: float = 20.0
     ch.close()
 
     per_step = []
-    for st in steps:
-        n_outer = max(st["cheby_steps"] - 1, 0)
-        per_step.append(t_init + st["cg_steps"] * t_cg + n_outer * t_outer)
+    for s in steps:
+        n_outer = max(s["cheby_steps"] - 1, 0)
+        per_step.append(t_init + s["cg_steps"] * t_cg + n_outer * t_outer)
     return {
         "value": statistics.mean(per_step),
         "unit": "s/timestep",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle on the {n}^2 chunk: {k} CG iterations ({t_cg * 1e3:.1f} ms each) and {m} PPCG "
-                   f"outer iterations ({t_outer * 1e3:.1f} ms each), scaled by the timed steps' iteration counts"),
+        "cpu_model": info["model"],
+        "physical_cores_available": info["physical_cores"],
+        "sample": (f"oracle (FP64 PPCG) on the deck's {n}^2 chunk, {threads} OpenMP threads: {k} CG iterations "
+                   f"({t_cg * 1e3:.1f} ms each) and {m} PPCG outer iterations ({t_outer * 1e3:.1f} ms each), "
+                   f"scaled by the timed steps' iteration counts"),
+        "per_iteration_s": {"init": t_init, "cg": t_cg, "ppcg_outer": t_outer},
     }
 
 
@@ -266,46 +307,76 @@ def mixed_form(ref: dict, mix: dict) -> dict:
     }
 
 
-def learned_workload(args, n: int) -> dict:
-    """The learned-FCG configuration: train the BF16 U-Net on (residual,
-    PPCG correction) pairs from FP64 PPCG solves of seeded random layouts of
-    8-64 rectangles and circles (training seeds from 1000), then solve
-    held-out layouts (seeds 0..) with flexible CG (network + 4 weighted Jacobi
-    sweeps, FP64 PPCG as fallback) and with FP64 PPCG.  Returns the bench
-    line's fields: `value` is the FP64 PPCG time, `forms.learned_fcg` the
-    learned solve against it, plus the flexible CG without the network."""
+def train_learner(args, n_train: int, dt_train: float):
+    """The BF16 U-Net trained on (residual, PPCG correction) pairs from FP64
+    PPCG solves of seeded random layouts of 8-64 rectangles and circles
+    (training seeds 1000..): crops of the iterates of the CG presteps and
+    the PPCG outer iterations, each pair's input residual the crop
+    operator applied to its correction, the A-norm loss measured after the
+    flexible CG's 4 Jacobi sweeps.  n_train/dt_train set the training
+    meshes; the network sees the operator per cell, whose face coefficients
+    scale with dt/dx^2, so a smaller mesh with dt scaled by the square of
+    the size ratio trains it for a larger one."""
     learner = tealeaf_hip.Learner(seed=args.seed + 17)
     t0 = time.perf_counter()
     for k in range(args.train_decks):
         s = 1000 + k
-        print(f"bench: training pairs from layout seed {s}", file=sys.stderr, flush=True)
-        learner.add_deck(decks.random_shapes_2d(n, s, steps=1), snapshots=args.snapshots, crops=args.crops,
-                         crop=args.crop, seed=s)
+        print(f"bench: training pairs from layout seed {s} ({n_train}^2, dt {dt_train:g})", file=sys.stderr,
+              flush=True)
+        learner.add_deck(decks.random_shapes_2d(n_train, s, steps=1, dt=dt_train), snapshots=args.snapshots,
+                         crops=args.crops, crop=min(args.crop, n_train), seed=s)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     losses = []
     chunks = max(1, args.train_steps // 100)
+    crop = min(args.crop, n_train)
     for k in range(chunks):
         lr = 2e-3 * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * k / chunks)))
-        losses += list(learner.train(args.train_steps // chunks, batch=max(2, 32 * 64 * 64 // (args.crop * args.crop)),
-                                     lr=lr))
-        print(f"bench: training step {len(losses)}, loss {statistics.mean(losses[-20:]):.4f}", file=sys.stderr,
-              flush=True)
+        losses += list(learner.train(args.train_steps // chunks, batch=max(1, 32 * 64 * 64 // (crop * crop)), lr=lr))
+        if k % 4 == 3 or k == chunks - 1:
+            print(f"bench: training step {len(losses)}, loss {statistics.mean(losses[-20:]):.4f}", file=sys.stderr,
+                  flush=True)
     t_train = time.perf_counter() - t0
+    stats = {"pairs": int(learner.pairs), "decks": args.train_decks, "mesh": n_train, "dt": dt_train,
+             "crop": crop, "steps": len(losses), "loss_first": float(statistics.mean(losses[:20])),
+             "loss_last": float(statistics.mean(losses[-20:])), "data_s": round(t_data, 2),
+             "train_s": round(t_train, 2),
+             "loss": "A-norm of the error after 4 weighted Jacobi sweeps, relative to the sweeps alone"}
+    return learner, stats
+
+
+def unet_roofline(learner, n: int) -> dict:
     fwd_ms, macs = learner.time_forward(1, n, n, reps=3)
     tflops = 2.0 * macs / (fwd_ms * 1e-3) / 1e12
     fwd_bytes = learner.forward_bytes(1, n, n)
     fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
+    return {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(tflops / 2500.0, 5), "forward_ms": round(fwd_ms, 3), "macs_per_forward": macs,
+            "dtype": "bf16 (fp32 accumulate)", "bytes_per_forward": fwd_bytes, "hbm_achieved_gbs": round(fwd_gbs, 1),
+            "hbm_frac": round(fwd_gbs / HBM_PEAK_GBS, 4)}
 
+
+def learned_workload(args, n: int) -> dict:
+    """The learned-FCG configuration: train the network (train_learner), then
+    solve held-out layouts (seeds 0..) with flexible CG (network + 4
+    weighted Jacobi sweeps, FP64 PPCG as fallback), with the same flexible
+    CG without the network, and with FP64 PPCG.  Returns the bench line's
+    fields: `value` is the FP64 PPCG time, `forms.learned_fcg` the learned
+    solve against it."""
+    learner, training = train_learner(args, args.train_n or n, 0.004 * ((n / (args.train_n or n)) ** 2))
+    roof = unet_roofline(learner, n)
     steps = args.warmup + args.steps
     per_seed, fcg_s, ref_s, sweep_s = [], [], [], []
+    ref0 = None
     for s in range(args.eval_decks):
         deck = decks.random_shapes_2d(n, s, steps=steps, extra=("tl_check_result",))
         print(f"bench: held-out layout seed {s}: FP64 PPCG", file=sys.stderr, flush=True)
         ref = run_solve(deck, 2, args.warmup, timing=False)
+        if ref0 is None:
+            ref0 = (deck, ref)
         learner.use()
         print(f"bench: held-out layout seed {s}: learned FCG", file=sys.stderr, flush=True)
-        got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}"), 2,
+        got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"), 2,
                         args.warmup, timing=False)
         learner.use(False)
         print(f"bench: held-out layout seed {s}: flexible CG, sweeps only", file=sys.stderr, flush=True)
@@ -323,50 +394,173 @@ def learned_workload(args, n: int) -> dict:
                 "ppcg_iters": b["iters"], "true_rrn": a["exact_error"],
                 "max_summary_rel_err": max(abs(a[k] - b[k]) / abs(b[k]) for k in ("vol", "mass", "ie", "temp")),
             })
-    pairs = learner.pairs
     learner.close()
     return {
-        "ref_s": statistics.mean(ref_s), "cells": n * n,
+        "ref_s": statistics.mean(ref_s), "cells": n * n, "ref0": ref0,
         "learned_fcg": {
             "s_per_step": round(statistics.mean(fcg_s), 6),
             "speedup_vs_fp64": round(statistics.mean(ref_s) / statistics.mean(fcg_s), 3),
             "sweeps_only_s_per_step": round(statistics.mean(sweep_s), 6),
+            "gain_vs_sweeps_only": round(statistics.mean(sweep_s) / statistics.mean(fcg_s), 3),
+            "net_every": args.net_every,
             "per_step": per_seed,
-            "training": {"pairs": int(pairs), "decks": args.train_decks,
-                         "steps": len(losses), "loss_first": float(statistics.mean(losses[:20])),
-                         "loss_last": float(statistics.mean(losses[-20:])), "data_s": round(t_data, 2),
-                         "train_s": round(t_train, 2)},
-            "unet_roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
-                              "frac": round(tflops / 2500.0, 5), "forward_ms": round(fwd_ms, 3),
-                              "macs_per_forward": macs, "dtype": "bf16 (fp32 accumulate)",
-                              "bytes_per_forward": fwd_bytes, "hbm_achieved_gbs": round(fwd_gbs, 1),
-                              "hbm_frac": round(fwd_gbs / HBM_PEAK_GBS, 4)},
+            "training": training,
+            "unet_roofline": roof,
         },
     }
+
+
+def true_residual_form(ref: dict, tr: dict) -> dict:
+    """FP64 PPCG accepted on the true residual (tl_ppcg_true_residual)
+    against the reference-faithful FP64 PPCG, which stops on its recurrence
+    residual."""
+    return {
+        "s_per_step": round(tr["s_per_step"], 6),
+        "vs_fp64_ppcg": round(ref["s_per_step"] / tr["s_per_step"], 3),
+        "per_step": [{"step": a["step"], "iters": a["iters"], "fp64_iters": b["iters"], "true_rrn": a["exact_error"],
+                      "fp64_true_rrn": b["exact_error"], "replacements": a["replacements"],
+                      "max_summary_rel_err": max(abs(a[k] - b[k]) / abs(b[k]) for k in ("vol", "mass", "ie", "temp"))}
+                     for a, b in zip(tr["timed"], ref["timed"])],
+    }
+
+
+def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None) -> dict:
+    """Learned flexible CG on the benchmark deck: the network is trained first
+    (train_learner, meshes of --train-n cells per side with dt scaled to
+    the benchmark's face coefficients), then applied every --net-every
+    iterations."""
+    n_train = args.train_n or min(n, 2048)
+    learner, training = train_learner(args, n_train, 0.004 * (n / n_train) ** 2)
+    learner.use()
+    print("bench: learned flexible CG on the same deck", file=sys.stderr, flush=True)
+    got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"), 2,
+                    args.warmup, timing=False)
+    learner.use(False)
+    roof = unet_roofline(learner, n)
+    learner.close()
+    per_step = []
+    for k, (a, b) in enumerate(zip(got["timed"], ref["timed"])):
+        row = {"step": a["step"], "iters": a["fcg_iters"], "fallback": a["fcg_fallback"], "fp64_iters": b["iters"],
+               "true_rrn": a["exact_error"], "fp64_true_rrn": b["exact_error"],
+               "max_summary_rel_err": max(abs(a[q] - b[q]) / abs(b[q]) for q in ("vol", "mass", "ie", "temp"))}
+        if sweeps:
+            row["sweeps_only_iters"] = sweeps["per_step"][k]["iters"]
+        per_step.append(row)
+    out = {
+        "s_per_step": round(got["s_per_step"], 6),
+        "speedup_vs_fp64": round(ref["s_per_step"] / got["s_per_step"], 3),
+        "net_every": args.net_every,
+        "net_iters": args.net_iters,
+        "per_step": per_step,
+        "training": training,
+        "unet_roofline": roof,
+    }
+    if sweeps:
+        out["gain_vs_sweeps_only"] = round(sweeps["s_per_step"] / got["s_per_step"], 3)
+    return out
+
+
+METRIC = "solve s/timestep @eps 1e-10 (learned-FCG, mixed PPCG vs FP64 PPCG); HBM GB/s"
+
+
+def line(value_s: float, args, workload: str, cells: int, **extra) -> dict:
+    out = {
+        "metric": METRIC, "value": round(value_s, 6), "unit": "s/timestep", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * value_s, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "cells": cells, "parallelism": "single chunk, one GPU"},
+    }
+    out.update(extra)
+    return out
+
+
+def copy_rate() -> float:
+    import ctypes
+    L = tealeaf_hip.lib(2)
+    L.tl_copy_probe.restype = ctypes.c_double
+    return round(L.tl_copy_probe(1 << 27, 10), 1)
+
+
+def north_star(args, n: int) -> dict:
+    """C4: FP64 PPCG against mixed PPCG, learned FCG, the flexible CG without
+    the network and FP64 PPCG accepted on the true residual, on the same
+    deck and timesteps."""
+    total = args.warmup + args.steps
+    deck = decks.random_rects_2d(n, args.seed, steps=total, solver="ppcg", presteps=30, inner=10,
+                                 max_iters=10000, extra=("tl_check_result",))
+    what = (f"2D {n}x{n}, 32 random rectangles (splitmix64 seed {args.seed}), eps 1e-10, dt 0.004; "
+            f"FP64 PPCG (30 presteps, 10 inner steps) vs mixed PPCG vs learned FCG")
+    print(f"bench: {what}, {args.warmup} + {args.steps} timesteps", file=sys.stderr, flush=True)
+    first = run_solve(deck, 2, 0, max_steps=1)
+    top = max(first["kernels"].items(), key=lambda kv: kv[1]["ms"])[0]
+    print(f"bench: FP64 PPCG, kernel with the most time: {top}", file=sys.stderr, flush=True)
+    ref = run_solve(deck, 2, args.warmup, only=top)
+    forms = {}
+    print("bench: mixed-precision PPCG", file=sys.stderr, flush=True)
+    mix = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg_mixed"), 2, args.warmup, only="ppcg_outer_mixed")
+    forms["mixed_ppcg"] = mixed_form(ref, mix)
+    print("bench: FP64 PPCG accepted on the true residual", file=sys.stderr, flush=True)
+    tr = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg\ntl_ppcg_true_residual"), 2, args.warmup, timing=False)
+    forms["fp64_ppcg_true_residual"] = true_residual_form(ref, tr)
+    print("bench: flexible CG, Jacobi sweeps only", file=sys.stderr, flush=True)
+    sw = run_solve(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network"), 2, args.warmup,
+                   timing=False)
+    forms["fcg_sweeps_only"] = fcg_form(ref, sw)
+    if not args.no_learned:
+        forms["learned_fcg"] = learned_form(args, deck, n, ref, forms["fcg_sweeps_only"])
+    t_ref, t_true = ref["s_per_step"], tr["s_per_step"]
+    ns = {"target_speedup": 1.5, "fp64_ppcg_s": round(t_ref, 6), "fp64_ppcg_true_residual_s": round(t_true, 6)}
+    for key in ("mixed_ppcg", "learned_fcg"):
+        if key in forms:
+            t = forms[key]["s_per_step"]
+            ns[key] = {"s": t, "vs_fp64_ppcg": round(t_ref / t, 3), "vs_fp64_ppcg_true_residual": round(t_true / t, 3)}
+    ns["met"] = any(ns[k]["vs_fp64_ppcg"] >= 1.5 for k in ("mixed_ppcg", "learned_fcg") if k in ns)
+    if "learned_fcg" in forms:
+        # what the trained network itself adds: the learned solve against the
+        # same flexible CG with the network switched off
+        ns["learned_fcg"]["network_gain_vs_sweeps_only"] = forms["learned_fcg"].get("gain_vs_sweeps_only")
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle sample)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(deck, n, args.cpu_threads, ref["timed"])
+    timed = ref["timed"]
+    return line(t_ref, args, what, ref["cells"],
+                median_s_per_step=round(statistics.median(s["solve_s"] for s in timed), 6),
+                iterations=[s["iters"] for s in timed], temp=[s["temp"] for s in timed],
+                true_rrn=[s["exact_error"] for s in timed], step1_s=ref["steps"][0]["solve_s"] if args.warmup else None,
+                wall_s=round(ref["wall_s"], 3), kernels=kernel_table(first),
+                kernels_note="every kernel, HIP events over the first timestep (a run of its own); roofline: "
+                             "events around the roofline kernel only, over the timed steps",
+                copy_gbs=copy_rate(), roofline=roofline_of(ref, n), north_star=ns, forms=forms, cpu_baseline=cpu)
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=9, help="timed timesteps")
+    ap.add_argument("--steps", type=int, default=None, help="timed timesteps (default 3 for c4, 9 otherwise)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps before them")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2: 2D 4096^2 two rectangles; c3: 3D 256^3 cube; c4: 2D 8192^2, 32 random rectangles; "
-                         "c5: 2D 4096^2 learned FCG on random rectangles and circles")
-    ap.add_argument("--train-decks", type=int, default=8, help="c5: layouts that give training pairs")
-    ap.add_argument("--snapshots", type=int, default=16, help="c5: PPCG iterations sampled per layout")
-    ap.add_argument("--crops", type=int, default=160, help="c5: 64x64 pairs cut from each sample")
-    ap.add_argument("--train-steps", type=int, default=1500, help="c5: Adam steps (batch 32)")
+    ap.add_argument("--workload", default="c4", choices=["c2", "c3", "c4", "c5"],
+                    help="c4: 2D 8192^2, 32 random rectangles (the target's configuration); c2: 2D 4096^2 two "
+                         "rectangles; c3: 3D 256^3 cube; c5: 2D 4096^2 learned FCG on random rectangles and circles")
+    ap.add_argument("--train-decks", type=int, default=8, help="learned form: layouts that give training pairs")
+    ap.add_argument("--train-n", type=int, default=None, help="learned form: training mesh (default min(n, 2048))")
+    ap.add_argument("--snapshots", type=int, default=16, help="learned form: iterates sampled per solve phase")
+    ap.add_argument("--crops", type=int, default=160, help="learned form: pairs cut from each iterate")
+    ap.add_argument("--train-steps", type=int, default=1600, help="learned form: Adam steps")
     ap.add_argument("--eval-decks", type=int, default=2, help="c5: held-out layouts")
-    ap.add_argument("--net-every", type=int, default=1, help="c5: apply the network every m-th FCG iteration")
-    ap.add_argument("--crop", type=int, default=64, help="c5: training pair size (cells per side)")
-    ap.add_argument("--n", type=int, default=None, help="cells per side (default 4096, 256 for c3, 8192 for c4)")
+    ap.add_argument("--net-every", type=int, default=1, help="learned form: network every m-th FCG iteration")
+    ap.add_argument("--net-iters", type=int, default=1,
+                    help="learned form: network only in the first m FCG iterations (0: throughout)")
+    ap.add_argument("--crop", type=int, default=64, help="learned form: training pair size (cells per side)")
+    ap.add_argument("--no-learned", action="store_true", help="c4: skip the learned form")
+    ap.add_argument("--n", type=int, default=None, help="cells per side (default 8192 for c4, 256 for c3, else 4096)")
     ap.add_argument("--solver", default="ppcg", choices=["ppcg", "cg", "chebyshev", "jacobi", "fcg"],
-                    help="fcg: flexible CG with Jacobi-sweep preconditioning on the PPCG deck")
+                    help="c2/c3; fcg: flexible CG with Jacobi-sweep preconditioning on the PPCG deck")
     ap.add_argument("--seed", type=int, default=0, help="c4 layout seed")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision PPCG solve")
-    ap.add_argument("--no-fcg", action="store_true", help="skip the flexible CG (Jacobi sweeps) solve")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--no-mixed", action="store_true", help="c2: skip the mixed-precision PPCG solve")
+    ap.add_argument("--no-fcg", action="store_true", help="c2: skip the flexible CG (Jacobi sweeps) solve")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: physical cores, "
+                                                                 "capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (counter calibration)")
@@ -374,6 +568,8 @@ def main() -> int:
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     args = ap.parse_args()
     n = args.n or {"c2": 4096, "c3": 256, "c4": 8192, "c5": 4096}[args.workload]
+    if args.steps is None:
+        args.steps = 3 if args.workload == "c4" else 9
 
     if args.copy_probe:
         L = tealeaf_hip.lib(2)
@@ -394,19 +590,19 @@ def main() -> int:
         print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
         return 2
 
+    if args.workload == "c4":
+        print(json.dumps(north_star(args, n)), flush=True)
+        return 0
+
     if args.workload == "c5":
         r = learned_workload(args, n)
-        out = {
-            "metric": "solve s/timestep @eps 1e-10 (learned-FCG, mixed PPCG vs FP64 PPCG); HBM GB/s",
-            "value": round(r["ref_s"], 6), "unit": "s/timestep", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(1e3 * r["ref_s"], 3), "higher_is_better": False,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-            "config": {"workload": f"2D {n}x{n} random rectangles and circles (held-out seeds 0-{args.eval_decks - 1}),"
-                                   f" FP64 PPCG vs learned FCG (BF16 U-Net), eps 1e-10, dt 0.004",
-                       "cells": r["cells"], "parallelism": "single chunk, one GPU"},
-            "forms": {"learned_fcg": r["learned_fcg"]},
-            "roofline": r["learned_fcg"]["unet_roofline"], "cpu_baseline": None,
-        }
+        deck0, ref0 = r["ref0"]
+        cpu = None if args.no_cpu_baseline else cpu_baseline(deck0, n, args.cpu_threads, ref0["timed"])
+        out = line(r["ref_s"], args,
+                   f"2D {n}x{n} random rectangles and circles (held-out seeds 0-{args.eval_decks - 1}), FP64 PPCG "
+                   f"vs learned FCG (BF16 U-Net), eps 1e-10, dt 0.004", r["cells"],
+                   forms={"learned_fcg": r["learned_fcg"]}, roofline=r["learned_fcg"]["unet_roofline"],
+                   cpu_baseline=cpu)
         print(json.dumps(out), flush=True)
         return 0
 
@@ -421,13 +617,9 @@ def main() -> int:
         deck = decks.two_rect_2d(n, solver, steps=total, presteps=30, inner=10, max_iters=max_iters,
                                  extra=extra)
         what = f"2D {n}x{n} {args.solver.upper()} two-rectangle deck"
-    elif args.workload == "c3":
+    else:
         deck = decks.cube_3d(n, solver, steps=total, presteps=30, inner=10, max_iters=max_iters, extra=extra)
         what = f"3D {n}^3 {args.solver.upper()} hot-box/cuboid deck"
-    else:
-        deck = decks.random_rects_2d(n, args.seed, steps=total, solver=solver, presteps=30, inner=10,
-                                     max_iters=max_iters, extra=extra)
-        what = f"2D {n}x{n} {args.solver.upper()} 32 random rectangles (splitmix64 seed {args.seed})"
     if args.solver == "fcg":
         if dim != 2:
             print("bench: the flexible CG is 2D", file=sys.stderr)
@@ -459,46 +651,20 @@ def main() -> int:
         fcg = run_solve(fdeck, dim, args.warmup, timing=False)
         forms["fcg_sweeps"] = fcg_form(ref, fcg)
 
-    import ctypes
-    L = tealeaf_hip.lib(2)
-    L.tl_copy_probe.restype = ctypes.c_double
-    copy_gbs = L.tl_copy_probe(1 << 27, 10)
-
     cpu = None
     if not args.no_cpu_baseline and args.workload == "c2" and args.solver == "ppcg":
-        cpu = cpu_baseline(n, args.cpu_threads, ref["timed"])
+        cpu = cpu_baseline(deck, n, args.cpu_threads, ref["timed"])
 
     timed = ref["timed"]
-    s_per_step = ref["s_per_step"]
-    out = {
-        "metric": "solve s/timestep @eps 1e-10 (learned-FCG, mixed PPCG vs FP64 PPCG); HBM GB/s",
-        "value": round(s_per_step, 6),
-        "unit": "s/timestep",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * s_per_step, 3),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{what}, FP64, eps 1e-10, dt 0.004, presteps 30, inner steps 10",
-                   "cells": ref["cells"], "parallelism": "single chunk, one GPU"},
-        "median_s_per_step": round(statistics.median(s["solve_s"] for s in timed), 6),
-        "iterations": [s["iters"] for s in timed],
-        "temp": [s["temp"] for s in timed],
-        "true_rrn": [s["exact_error"] for s in timed],
-        "step1_s": ref["steps"][0]["solve_s"] if args.warmup else None,
-        "wall_s": round(ref["wall_s"], 3),
-        "kernels": kernel_table(first),
-        "kernels_note": "every kernel, HIP events over the first timestep (a run of its own); roofline: "
-                        "events around the roofline kernel only, over the timed steps",
-        "copy_gbs": round(copy_gbs, 1),
-        "roofline": roofline_of(ref, n),
-        "forms": forms,
-        "cpu_baseline": cpu,
-    }
+    out = line(ref["s_per_step"], args, f"{what}, FP64, eps 1e-10, dt 0.004, presteps 30, inner steps 10",
+               ref["cells"], median_s_per_step=round(statistics.median(s["solve_s"] for s in timed), 6),
+               iterations=[s["iters"] for s in timed], temp=[s["temp"] for s in timed],
+               true_rrn=[s["exact_error"] for s in timed],
+               step1_s=ref["steps"][0]["solve_s"] if args.warmup else None, wall_s=round(ref["wall_s"], 3),
+               kernels=kernel_table(first),
+               kernels_note="every kernel, HIP events over the first timestep (a run of its own); roofline: "
+                            "events around the roofline kernel only, over the timed steps",
+               copy_gbs=copy_rate(), roofline=roofline_of(ref, n), forms=forms, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -292,6 +292,7 @@ Learner* active_learner();
 struct FcgOptions {
     bool network = true;
     int net_every = 1;
+    int net_iters = 0;  // > 0: the network only in the first net_iters iterations, sweeps alone after
     int smooth = 4;
     double omega = 0.8;
 };

@@ -57,10 +57,15 @@ struct Deck {
     // steps, the FP64 residual replaced once ||r||^2 fell by this factor.
     bool ppcg_mixed = false;
     double mixed_replace = 1.0e-2;
+    // FP64 PPCG under the same residual replacement, accepting only the true
+    // residual (no counterpart in the reference): the FP64 solve at the
+    // accuracy the mixed and learned forms are held to.
+    bool ppcg_true_residual = false;
     // Flexible CG with the learned preconditioner (no counterpart in the
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
-    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_net_every=<m>, tl_fcg_no_network
+    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_net_every=<m>, tl_fcg_net_iters=<m>,
+                     // tl_fcg_no_network
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -214,6 +219,7 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_use_fcg_learned") { d.use_cheby = d.use_cg = d.use_jacobi = d.ppcg_mixed = false; d.use_ppcg = d.use_fcg = true; }
             else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = d.use_fcg = false; d.use_ppcg = d.ppcg_mixed = true; }
             else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
+            else if (k == "tl_ppcg_true_residual") d.ppcg_true_residual = true;
             else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
             else if (k == "tl_fcg_omega") {
                 d.fcg.omega = R();
@@ -222,6 +228,7 @@ Deck parse_deck(const std::string& text)
             }
             else if (k == "tl_fcg_no_network") d.fcg.network = false;
             else if (k == "tl_fcg_net_every") d.fcg.net_every = I();
+            else if (k == "tl_fcg_net_iters") d.fcg.net_iters = I();
             else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();
@@ -505,8 +512,9 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     // true-residual checks that failed without halving the true rrn
     // mixed_form: the solve uses residual replacement and accepts only the
     // true residual; mixed: the FP32 inner steps are (still) in use
-    const bool mixed_form = d.use_ppcg && d.ppcg_mixed && ppcg_mixed_supported(c, d.ppcg_inner_steps);
-    bool mixed = mixed_form;
+    const bool mixed_ok = d.use_ppcg && d.ppcg_mixed && ppcg_mixed_supported(c, d.ppcg_inner_steps);
+    const bool mixed_form = mixed_ok || (d.use_ppcg && !d.use_fcg && d.ppcg_true_residual);
+    bool mixed = mixed_ok;
     double rrn_replaced = 0.0, last_true = HUGE_VAL;
     int stalls = 0;
     std::vector<double> cg_alphas, cg_betas;
@@ -613,10 +621,8 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     }
                     op_calc_residual(c);
                     switch_step = n;
-                    if (mixed) {
-                        op_mixed_prepare(c, &ch_alphas[1], &ch_betas[1], d.ppcg_inner_steps, theta);
-                        rrn_replaced = error;
-                    }
+                    if (mixed) op_mixed_prepare(c, &ch_alphas[1], &ch_betas[1], d.ppcg_inner_steps, theta);
+                    if (mixed_form) rrn_replaced = error;
                 }
                 // rro on the device is the last rrn (= error), as at tea_solve.f90:187,304
                 op_cg_calc_w_dev(c, p_pending);

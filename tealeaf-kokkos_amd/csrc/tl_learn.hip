@@ -526,10 +526,15 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         }
         return 0;
     };
-    // Sweeps only (2D): iteration k's r -= alpha w and its ||r||^2 ride in
-    // iteration k + 1's preconditioner pass, and the host reads them while
-    // that iteration's strip runs (which also brings u up to iteration k).
-    const bool defer = !opt.network && TL_DIM == 2 && fcg_smooth_tb_supported(c, opt.smooth);
+    // Before an iteration without the network (2D): iteration k's r -= alpha w
+    // and its ||r||^2 ride in iteration k + 1's preconditioner pass, and the
+    // host reads them while that iteration's strip runs (which also brings u
+    // up to iteration k).  The network reads r first, so the iteration before
+    // one that applies it keeps r current.
+    const bool sweeps_tb = TL_DIM == 2 && fcg_smooth_tb_supported(c, opt.smooth);
+    auto uses_net = [&](int j) {
+        return opt.network && (j - 1) % std::max(1, opt.net_every) == 0 && (opt.net_iters <= 0 || j <= opt.net_iters);
+    };
     bool r_pending = false;  // r -= alpha w of the last step not yet applied
     bool first = true;
     for (int it = 1; it <= max_iters; ++it) {
@@ -542,7 +547,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             std::swap(c.f[F_Z], c.f[F_SD]);
             std::swap(c.f[F_R], c.r_alt);
             r_pending = false;
-        } else if (opt.network && (it - 1) % std::max(1, opt.net_every) == 0) {
+        } else if (uses_net(it)) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
             launched("FCG input");
@@ -557,7 +562,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
         }
         // beta, p, w, alpha, u[, r] with the scalars on the device; one read
-        const bool defer_r = defer && it < max_iters;
+        const bool defer_r = sweeps_tb && !uses_net(it + 1) && it < max_iters;
         op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r, !pr_next);
         pr_next = false;
         first = false;

@@ -173,6 +173,49 @@ def random_shapes_2d(n: int, seed: int, nmin: int = 8, nmax: int = 64, solver: s
     return _finish(lines + _common(solver, steps, eps, dt, max_iters, presteps, inner, list(extra)))
 
 
+GEOMETRY = {"rectangle": 1, "circle": 2, "point": 3}
+
+
+def states(deck: str) -> dict:
+    """The 2D states of a deck as ext_generate_chunk_ takes them (lists
+    indexed by state; geometry 0 for the ambient state 1), with the extents
+    of states 2..n moved inwards by a hundredth of a cell
+    (tealeaf_2d/read_input.f90:284-297; dx from the single-precision cell
+    count, read_input.f90:290)."""
+    import numpy as np
+
+    grid = {"xmin": 0.0, "xmax": 100.0, "ymin": 0.0, "ymax": 100.0, "x_cells": 10, "y_cells": 10}
+    st = {}
+    for ln in deck.splitlines():
+        words = ln.replace("=", " ").split()
+        if not words:
+            continue
+        if words[0] == "state":
+            k = int(words[1])
+            kv = dict(zip(words[2::2], words[3::2]))
+            st[k] = {"density": float(kv["density"]), "energy": float(kv["energy"]),
+                     "geometry": GEOMETRY.get(kv.get("geometry", ""), 0),
+                     **{key: float(kv.get(key, 0.0)) for key in ("xmin", "xmax", "ymin", "ymax", "radius")}}
+        elif words[0] in grid and len(words) > 1:
+            grid[words[0]] = type(grid[words[0]])(float(words[1]) if words[0].endswith("cells") else words[1])
+    dx = (grid["xmax"] - grid["xmin"]) / float(np.float32(grid["x_cells"]))
+    dy = (grid["ymax"] - grid["ymin"]) / float(np.float32(grid["y_cells"]))
+    out = {key: [] for key in ("density", "energy", "xmin", "xmax", "ymin", "ymax", "radius", "geometry")}
+    for k in sorted(st):
+        s = dict(st[k])
+        if k >= 2:
+            s["xmin"] += dx / 100.0
+            s["ymin"] += dy / 100.0
+            s["xmax"] -= dx / 100.0
+            s["ymax"] -= dy / 100.0
+        if k == 1:
+            s["geometry"] = 0
+        for key in out:
+            out[key].append(s[key])
+    out.update(dx=dx, dy=dy, grid_xmin=grid["xmin"], grid_ymin=grid["ymin"])
+    return out
+
+
 def cells(deck: str) -> int:
     """Interior cell count of a deck (x_cells * y_cells [* z_cells])."""
     n = 1

@@ -368,3 +368,43 @@ def test_pack_message_regions(hip, oracle, dim, depth):
             want[dst] = vals[k][src]
             np.testing.assert_array_equal(got, want)
     ch.close()
+
+
+@pytest.mark.parametrize("kind,seed", [("rects", 3), ("shapes", 5), ("two_rect", 0)])
+def test_deck_states_reader(hip, oracle, kind, seed):
+    """decks.states() (the CPU baseline's way of rebuilding a deck's layout
+    through ext_generate_chunk_) paints the same cells as the oracle's deck
+    reader does for the whole deck."""
+    import ctypes
+    import decks
+    n = 72
+    deck = {"rects": lambda: decks.random_rects_2d(n, seed, steps=1),
+            "shapes": lambda: decks.random_shapes_2d(n, seed, steps=1),
+            "two_rect": lambda: decks.two_rect_2d(n, "cg", steps=1)}[kind]()
+    L = oracle.lib(2)
+    L.tl_oracle_keep_chunk.argtypes = [ctypes.c_int]
+    L.tl_oracle_keep_chunk(1)
+    try:
+        oracle.run_deck(deck.replace("end_step=1", "end_step=1\ntl_max_iters=1"), dim=2, reduce_parts=4,
+                        threads=4)
+        chunk = ctypes.c_int(1)
+        dims = (ctypes.c_int * 6)()
+        assert L.tl_chunk_dims(ctypes.byref(chunk), dims) == 0
+        want = {}
+        for name in ("density", "energy0"):
+            a = np.empty((dims[1], dims[0]))
+            assert L.tl_get_field(ctypes.byref(chunk), oracle.FIELD_ID[name],
+                                  a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+            want[name] = a
+    finally:
+        L.tl_oracle_keep_chunk(0)
+        L.ext_finalise_()
+    st = decks.states(deck)
+    ch = hip.ExtChunk(L, 2, n, n)
+    ch.initialise_chunk(st["grid_xmin"], st["grid_ymin"], st["dx"], st["dy"])
+    ch.generate_chunk(density=st["density"], energy=st["energy"], xmin=st["xmin"], xmax=st["xmax"],
+                      ymin=st["ymin"], ymax=st["ymax"], radius=st["radius"], geometry=st["geometry"])
+    got = {name: ch.get(name)[0] for name in ("density", "energy0")}
+    ch.close()
+    for name in got:
+        np.testing.assert_array_equal(got[name][2:-2, 2:-2], want[name][2:-2, 2:-2])
