@@ -566,6 +566,10 @@ def main() -> int:
                     help="only run the plain copy kernel (counter calibration)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only run the streaming probes of the attainable HBM rate")
+    ap.add_argument("--counter-deck", action="store_true",
+                    help="only run the c4 deck capped at 100 iterations (counter runs)")
     args = ap.parse_args()
     n = args.n or {"c2": 4096, "c3": 256, "c4": 8192, "c5": 4096}[args.workload]
     if args.steps is None:
@@ -580,6 +584,31 @@ def main() -> int:
         return 0
     if args.kernel_probe:
         print(json.dumps(kernel_probe(n, 20)), flush=True)
+        return 0
+    if args.stream_probe:
+        import ctypes
+        L = tealeaf_hip.lib(2)
+        L.tl_stream_probe.restype = ctypes.c_double
+        L.tl_stream_probe.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_int]
+        names = {0: "copy, grid-stride (tl_copy_probe)", 1: "copy, one element per thread",
+                 2: "copy, nontemporal", 3: "read only", 4: "write only",
+                 5: "4 reads + 2 writes (p, w strip mix)", 6: "6 reads + 2 writes (PPCG outer mix)"}
+        for kind, name in names.items():
+            for m2 in (1 << 28, 1 << 26):
+                if kind >= 5 and m2 == 1 << 28:
+                    continue  # eight 2 GiB arrays: the 512 MiB streams of the 8192^2 fields suffice
+                gbs = L.tl_stream_probe(kind, m2, 10)
+                print(json.dumps({"probe": name, "kind": kind, "doubles_per_array": m2, "gbs": round(gbs, 1),
+                                  "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 3)}), flush=True)
+        return 0
+    if args.counter_deck:
+        # the 8192^2 deck, one timestep capped at 100 iterations, switching to
+        # PPCG after 20: the CG strips and the fused PPCG outer kernel, short
+        # enough for hardware-counter runs
+        deck = decks.random_rects_2d(n, args.seed, steps=1, solver="ppcg", max_iters=100,
+                                     extra=("tl_ch_cg_errswitch", "tl_ch_cg_epslim=1.0e30"))
+        rep = tealeaf_hip.run_deck(deck, dim=2)
+        print(json.dumps({"probe": "counter deck", "n": n, "iters": rep["steps"][0]["iters"]}), flush=True)
         return 0
 
     if args.gpus != 1:

@@ -187,6 +187,15 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
  * calibrates HBM byte counters.  Negative on error. */
 double tl_copy_probe(long n, int reps);
 
+/* GB/s of a streaming probe over arrays of n doubles, repeated reps times;
+ * one 16-byte element per thread unless stated.  kind 0: the plain copy of
+ * tl_copy_probe (grid-stride); 1: copy; 2: copy with nontemporal loads and
+ * stores; 3: read only; 4: write only; 5: 4 reads + 2 writes (the p, w
+ * strip's stream mix); 6: 6 reads + 2 writes (the PPCG outer iteration's).
+ * The attainable rate the kernels' HBM figures are compared with; no
+ * counterpart in the reference.  Negative on error. */
+double tl_stream_probe(int kind, long n, int reps);
+
 /* No counterpart in the reference: ext_cg_calc_w_ for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  p, kx, ky, w: caller-owned device arrays

@@ -213,6 +213,8 @@ void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in 
 double op_error(Chunk& c);        // waits and returns ds->error
 void op_finish_rrn(Chunk& c, long nblocks);  // second phase of an rrn reduction, PPCG recurrences
 double stream_copy_probe(long n, int reps);   // GB/s of a plain copy of n doubles
+// GB/s of a streaming probe over arrays of n doubles (kinds in tl_stream_probe)
+double stream_probe(int kind, long n, int reps);
 // w = A p and *pw = sum w.p on caller device arrays in the reference's flat
 // layout, with no device allocation (tl_noalloc.hip)
 void cg_calc_w_noalloc(int nx, int ny, int nz, const double* p, const double* kx, const double* ky,

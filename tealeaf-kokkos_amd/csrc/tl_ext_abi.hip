@@ -427,6 +427,14 @@ double tl_copy_probe(long n, int reps)
     return gbs;
 }
 
+// GB/s of a streaming probe over arrays of n doubles (see the header).
+double tl_stream_probe(int kind, long n, int reps)
+{
+    double gbs = -1.0;
+    guarded([&] { gbs = stream_probe(kind, n, reps); });
+    return gbs;
+}
+
 // ext_cg_calc_w_ on caller-owned device arrays with no device allocation
 // (the work partials live in a static device array); 0, or -1 on error.
 #if TL_DIM == 3

@@ -15,6 +15,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <cstdlib>
 
 namespace tl {
 
@@ -419,6 +420,21 @@ struct PWStrip {
     int tiles_x, rows;
     long nwaves;
 };
+
+// Strip geometry of the p, w sweeps: strips of 64 rows (fewer strip starts,
+// each re-reading two rows of p and r), shorter only while there would be
+// under two waves per SIMD, 128 where that still leaves four or more.
+// Heights of 1 to 256 rows move the sweep by 1-3% at 4096^2 and 8192^2.
+inline PWStrip pw_strip(const Chunk& c)
+{
+    PWStrip t;
+    t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
+    t.rows = 64;
+    while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
+    if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
+    t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
+    return t;
+}
 
 // With PR (the flexible CG), p' . r is summed as well (a second partial),
 // r read beside the other inputs.
@@ -990,6 +1006,47 @@ __global__ __launch_bounds__(COPY_BLOCK) void k_stream_copy(const double2* __res
     for (long i = (long)blockIdx.x * COPY_BLOCK + threadIdx.x; i < n2; i += (long)gridDim.x * COPY_BLOCK) b[i] = a[i];
 }
 
+// Streaming probes of the attainable HBM rate: one 16-byte element per
+// thread over the whole array (the launch shape that streams fastest), plain
+// or with nontemporal loads and stores; read-only, write-only; and the stream
+// mixes of the solver's sweeps: 4 reads + 2 writes (p, w strip), 6 reads +
+// 2 writes (PPCG outer iteration).
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <bool NT>
+__global__ __launch_bounds__(256) void k_probe_copy(const d2v* __restrict__ a, d2v* __restrict__ b, long n2)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n2) return;
+    if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+    else b[i] = a[i];
+}
+__global__ __launch_bounds__(256) void k_probe_read(const d2v* __restrict__ a, long n2, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n2) return;
+    const d2v v = a[i];
+    if (v.x + v.y == -1.0) out[0] = v.x;  // keeps the load; the arrays hold zeros
+}
+__global__ __launch_bounds__(256) void k_probe_write(d2v* __restrict__ b, long n2)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n2) b[i] = d2v{1.0, 2.0};
+}
+struct ProbeStreams {
+    const d2v* r[6];
+    d2v* w[2];
+};
+template <int NR>
+__global__ __launch_bounds__(256) void k_probe_mix(ProbeStreams s, long n2)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n2) return;
+    d2v v[NR];
+    for (int k = 0; k < NR; ++k) v[k] = s.r[k][i];
+    s.w[0][i] = v[0] + v[1] * v[2];
+    s.w[1][i] = NR > 4 ? v[3] + v[4] * v[5] : v[3] + v[1];
+}
+
 // Second phase of a reduction whose first phase wrote `nblocks` partials.
 template <int NV>
 void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
@@ -1174,14 +1231,10 @@ Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
         TL_HIP(hipMalloc(&f[k], bytes));
         TL_HIP(hipMemsetAsync(f[k], 0, bytes, stream));
     }
-    TL_HIP(hipMalloc(&u_alt, bytes));
-    TL_HIP(hipMemsetAsync(u_alt, 0, bytes, stream));
-    TL_HIP(hipMalloc(&sd_alt, bytes));
-    TL_HIP(hipMemsetAsync(sd_alt, 0, bytes, stream));
-    TL_HIP(hipMalloc(&p_alt, bytes));
-    TL_HIP(hipMemsetAsync(p_alt, 0, bytes, stream));
-    TL_HIP(hipMalloc(&r_alt, bytes));
-    TL_HIP(hipMemsetAsync(r_alt, 0, bytes, stream));
+    for (double** q : {&u_alt, &sd_alt, &p_alt, &r_alt}) {
+        TL_HIP(hipMalloc(q, bytes));
+        TL_HIP(hipMemsetAsync(*q, 0, bytes, stream));
+    }
     TL_HIP(hipMalloc(&vertexX, sizeof(double) * (g.X + 1)));
     TL_HIP(hipMalloc(&vertexY, sizeof(double) * (g.Y + 1)));
     TL_HIP(hipMalloc(&vertexZ, sizeof(double) * (g.Z + 1)));
@@ -1547,16 +1600,7 @@ void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
     if (fold_u && (!fold || TL_DIM != 2)) throw Error("the pending u update folds only into the 2D p, w strip");
     if (fold) {
 #if TL_DIM == 2
-        PWStrip t;
-        t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
-        // strips of 64 rows (fewer strip starts, each re-reading two rows of
-        // p and r), shorter only while there would be under two waves per
-        // SIMD, 128 where that still leaves four or more (8192^2: 705 against
-        // 720 us)
-        t.rows = 64;
-        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
-        if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
-        t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
+        const PWStrip t = pw_strip(c);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         if (fold_u) {
             TIMED(c, K_CG_WPU, {
@@ -1670,6 +1714,60 @@ double stream_copy_probe(long n, int reps)
     (void)hipFree(a);
     (void)hipFree(b);
     return 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;  // GB/s, read + write
+}
+
+double stream_probe(int kind, long n, int reps)
+{
+    if (kind == 0) return stream_copy_probe(n, reps);
+    if (kind < 0 || kind > 6) throw Error("stream probe kinds are 0-6");
+    if (n < 2 || reps < 1) throw Error("stream probe needs n >= 2 and reps >= 1");
+    n &= ~1L;
+    const long n2 = n / 2;
+    const int nbuf = kind >= 5 ? 8 : 2;
+    std::vector<double*> buf(nbuf, nullptr);
+    for (double*& q : buf) {
+        TL_HIP(hipMalloc(&q, sizeof(double) * n));
+        TL_HIP(hipMemset(q, 0, sizeof(double) * n));
+    }
+    ProbeStreams st{};
+    for (int k = 0; k < 6 && k < nbuf; ++k) st.r[k] = reinterpret_cast<const d2v*>(buf[k]);
+    if (kind >= 5) {
+        st.w[0] = reinterpret_cast<d2v*>(buf[6]);
+        st.w[1] = reinterpret_cast<d2v*>(buf[7]);
+    }
+    const d2v* a = reinterpret_cast<const d2v*>(buf[0]);
+    d2v* b = reinterpret_cast<d2v*>(buf[1]);
+    const unsigned grid = (unsigned)((n2 + 255) / 256);
+    double per = 16.0;  // bytes per element and launch
+    auto launch = [&] {
+        switch (kind) {
+        case 1: hipLaunchKernelGGL(k_probe_copy<false>, dim3(grid), dim3(256), 0, 0, a, b, n2); break;
+        case 2: hipLaunchKernelGGL(k_probe_copy<true>, dim3(grid), dim3(256), 0, 0, a, b, n2); break;
+        case 3: hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, 0, a, n2, buf[1]); break;
+        case 4: hipLaunchKernelGGL(k_probe_write, dim3(grid), dim3(256), 0, 0, b, n2); break;
+        case 5: hipLaunchKernelGGL(k_probe_mix<4>, dim3(grid), dim3(256), 0, 0, st, n2); break;
+        default: hipLaunchKernelGGL(k_probe_mix<6>, dim3(grid), dim3(256), 0, 0, st, n2); break;
+        }
+    };
+    if (kind == 1 || kind == 2) per = 32.0;
+    if (kind == 5) per = 96.0;
+    if (kind == 6) per = 128.0;
+    hipEvent_t e0, e1;
+    TL_HIP(hipEventCreate(&e0));
+    TL_HIP(hipEventCreate(&e1));
+    launch();
+    check_launch();
+    TL_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r) launch();
+    TL_HIP(hipEventRecord(e1, 0));
+    TL_HIP(hipEventSynchronize(e1));
+    check_launch();
+    float ms = 0.0f;
+    TL_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (double* q : buf) (void)hipFree(q);
+    return per * (double)n2 * reps / (ms * 1e-3) / 1e9;
 }
 
 void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b)
@@ -1846,12 +1944,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
 #if TL_DIM == 2
     if (!first) {
         // p = z + beta p, w = A p, p.w and p.r in one streaming sweep
-        PWStrip t;
-        t.tiles_x = (c.g.nx + 2 * WAVE - 1) / (2 * WAVE);
-        t.rows = 64;
-        while (t.rows > 4 && (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows) < 2L * c.simds) t.rows /= 2;
-        if ((long)t.tiles_x * ((c.g.ny + 127) / 128) >= 4L * c.simds) t.rows = 128;
-        t.nwaves = (long)t.tiles_x * ((c.g.ny + t.rows - 1) / t.rows);
+        const PWStrip t = pw_strip(c);
         const long nb = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
         // with u_pending the previous iteration's u += alpha p rides along
         // (alpha and the old p are still in place).  With beta_rrn the
