@@ -568,6 +568,9 @@ def main() -> int:
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     ap.add_argument("--stream-probe", action="store_true",
                     help="only run the streaming probes of the attainable HBM rate")
+    ap.add_argument("--cpu-check", action="store_true",
+                    help="only check the CPU baseline's sampled estimate against one fully timed oracle timestep "
+                         "(2D n^2 two-rectangle PPCG deck; takes minutes)")
     ap.add_argument("--counter-deck", action="store_true",
                     help="only run the c4 deck capped at 100 iterations (counter runs)")
     args = ap.parse_args()
@@ -600,6 +603,24 @@ def main() -> int:
                 gbs = L.tl_stream_probe(kind, m2, 10)
                 print(json.dumps({"probe": name, "kind": kind, "doubles_per_array": m2, "gbs": round(gbs, 1),
                                   "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 3)}), flush=True)
+        return 0
+    if args.cpu_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        info = cpu_info()
+        threads = args.cpu_threads or info["threads"]
+        deck = decks.two_rect_2d(n, "ppcg", steps=1, presteps=30, inner=10, max_iters=10000)
+        t0 = time.perf_counter()
+        full = oracle.run_deck(deck, dim=2, reduce_parts=threads, threads=threads, max_steps=1)
+        wall = time.perf_counter() - t0
+        step = full["steps"][0]
+        est = cpu_baseline(deck, n, threads, [step])
+        print(json.dumps({"probe": "cpu baseline check", "n": n, "threads": threads, "cpu_model": info["model"],
+                          "full_step_solve_s": step["solve_s"], "full_run_wall_s": round(wall, 1),
+                          "iters": step["iters"], "cg_steps": step["cg_steps"], "cheby_steps": step["cheby_steps"],
+                          "sampled_estimate_s": round(est["value"], 3),
+                          "ratio_estimate_to_full": round(est["value"] / step["solve_s"], 4),
+                          "sample": est["sample"]}), flush=True)
         return 0
     if args.counter_deck:
         # the 8192^2 deck, one timestep capped at 100 iterations, switching to
