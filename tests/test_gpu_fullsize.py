@@ -13,10 +13,11 @@ differ by more than that (the eps test compares an unnormalised sum with
 lie within the band the two oracle runs span, widened by their own
 difference: the reference's result is then only defined up to that spread.
 
-C4 (8192^2, 32 seeded rectangles) is checked by properties that hold at any
-size: the mixed-precision form ends every timestep with the true residual
-||u0 - A u||^2 below eps (it accepts only on it), and its field summaries
-agree with the FP64 PPCG solve of the same deck.
+C4 (8192^2, 32 seeded rectangles) and C5 (4096^2 rectangles and circles)
+are checked by properties that hold at any size: the mixed-precision PPCG,
+the flexible CG and the learned flexible CG end every timestep with the true
+residual ||u0 - A u||^2 below eps (they accept only on it), and their field
+summaries agree with the FP64 PPCG solve of the same deck.
 """
 import glob
 import json
@@ -77,20 +78,62 @@ def test_fullsize_vs_oracle_fixture(hip, path):
     assert ok, rows
 
 
-def test_c4_mixed_ppcg_properties(hip):
-    """8192^2, 32 seeded random rectangles, one timestep: mixed PPCG meets
-    eps on the true residual; its summaries agree with FP64 PPCG's."""
+def test_c4_forms_properties(hip):
+    """8192^2, 32 seeded random rectangles, one timestep: mixed PPCG and the
+    flexible CG with Jacobi sweeps meet eps on the true residual (both accept
+    only on it); their field summaries agree with FP64 PPCG's (which stops on
+    its recurrence residual, true ||r||^2 ~1e-8: summaries within 1e-10)."""
     import decks
     extra = ("tl_check_result",)
+    deck = decks.random_rects_2d(8192, 0, solver="ppcg", steps=1, extra=extra)
     print("c4 8192^2 seed 0: FP64 PPCG", flush=True)
-    fp64 = hip.run_deck(decks.random_rects_2d(8192, 0, solver="ppcg", steps=1, extra=extra), dim=2)
-    print("c4 8192^2 seed 0: mixed PPCG", flush=True)
-    mixed = hip.run_deck(decks.random_rects_2d(8192, 0, solver="ppcg_mixed", steps=1, extra=extra), dim=2)
-    eps = mixed["eps"]
-    for sm, sf in zip(mixed["steps"], fp64["steps"]):
-        print(json.dumps({"step": sm["step"], "mixed_iters": sm["iters"], "fp64_iters": sf["iters"],
-                          "mixed_exact": sm["exact_error"], "fp64_exact": sf["exact_error"],
-                          "temp_rel": abs(sm["temp"] - sf["temp"]) / abs(sf["temp"])}), flush=True)
-        assert sm["exact_error"] < eps
-        for key in KEYS:
-            assert abs(sm[key] - sf[key]) <= 1e-10 * abs(sf[key]), (key, sm[key], sf[key])
+    fp64 = hip.run_deck(deck, dim=2)
+    forms = {"mixed": deck.replace("tl_use_ppcg", "tl_use_ppcg_mixed"),
+             "fcg_sweeps": deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network")}
+    for name, d in forms.items():
+        print(f"c4 8192^2 seed 0: {name}", flush=True)
+        got = hip.run_deck(d, dim=2)
+        eps = got["eps"]
+        for sg, sf in zip(got["steps"], fp64["steps"]):
+            print(json.dumps({"form": name, "step": sg["step"], "iters": sg["iters"], "fp64_iters": sf["iters"],
+                              "true_rrn": sg["exact_error"], "fp64_true_rrn": sf["exact_error"],
+                              "temp_rel": abs(sg["temp"] - sf["temp"]) / abs(sf["temp"])}), flush=True)
+            assert sg["exact_error"] < eps
+            for key in KEYS:
+                assert abs(sg[key] - sf[key]) <= 1e-10 * abs(sf[key]), (name, key, sg[key], sf[key])
+
+
+def test_c5_learned_fcg_properties(hip):
+    """4096^2 held-out layout of rectangles and circles (seed 0), after a
+    short training on two 1024^2 layouts with dt x 16 (the same face
+    coefficients per cell): the learned flexible CG (network in the first
+    iteration) meets eps on the true residual without falling back, and its
+    summaries agree with FP64 PPCG's to 1e-10."""
+    import decks
+    learner = hip.Learner(seed=3)
+    try:
+        for s in (1000, 1001):
+            print(f"c5 training pairs, seed {s}", flush=True)
+            learner.add_deck(decks.random_shapes_2d(1024, s, steps=1, dt=0.064), snapshots=8, crops=64, crop=64,
+                             seed=s)
+        losses = learner.train(300, batch=32, lr=2e-3)
+        print(f"c5 training loss {losses[:10].mean():.3f} -> {losses[-10:].mean():.3f}", flush=True)
+        deck = decks.random_shapes_2d(4096, 0, steps=1, extra=("tl_check_result",))
+        print("c5 4096^2 seed 0: FP64 PPCG", flush=True)
+        ref = hip.run_deck(deck, dim=2)
+        learner.use()
+        try:
+            print("c5 4096^2 seed 0: learned FCG", flush=True)
+            got = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_net_iters=1"), dim=2)
+        finally:
+            learner.use(False)
+    finally:
+        learner.close()
+    sg, sf = got["steps"][0], ref["steps"][0]
+    print(json.dumps({"fcg_iters": sg["fcg_iters"], "fallback": sg["fcg_fallback"], "true_rrn": sg["exact_error"],
+                      "ppcg_iters": sf["iters"], "temp_rel": abs(sg["temp"] - sf["temp"]) / abs(sf["temp"])}),
+          flush=True)
+    assert sg["fcg_fallback"] == 0
+    assert sg["exact_error"] < got["eps"]
+    for key in KEYS:
+        assert abs(sg[key] - sf[key]) <= 1e-10 * abs(sf[key]), (key, sg[key], sf[key])
