@@ -376,7 +376,8 @@ def learned_workload(args, n: int) -> dict:
             ref0 = (deck, ref)
         learner.use()
         print(f"bench: held-out layout seed {s}: learned FCG", file=sys.stderr, flush=True)
-        got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"), 2,
+        got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"
+                                        + ("\ntl_fcg_net_restart" if args.net_restart else "")), 2,
                         args.warmup, timing=False)
         learner.use(False)
         print(f"bench: held-out layout seed {s}: flexible CG, sweeps only", file=sys.stderr, flush=True)
@@ -433,7 +434,8 @@ def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None) -> dic
     learner, training = train_learner(args, n_train, 0.004 * (n / n_train) ** 2)
     learner.use()
     print("bench: learned flexible CG on the same deck", file=sys.stderr, flush=True)
-    got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"), 2,
+    got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"
+                                        + ("\ntl_fcg_net_restart" if args.net_restart else "")), 2,
                     args.warmup, timing=False)
     learner.use(False)
     roof = unet_roofline(learner, n)
@@ -451,6 +453,7 @@ def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None) -> dic
         "speedup_vs_fp64": round(ref["s_per_step"] / got["s_per_step"], 3),
         "net_every": args.net_every,
         "net_iters": args.net_iters,
+        "net_restart": bool(args.net_restart),
         "per_step": per_step,
         "training": training,
         "unet_roofline": roof,
@@ -549,6 +552,8 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=1600, help="learned form: Adam steps")
     ap.add_argument("--eval-decks", type=int, default=2, help="c5: held-out layouts")
     ap.add_argument("--net-every", type=int, default=1, help="learned form: network every m-th FCG iteration")
+    ap.add_argument("--net-restart", type=int, default=1,
+                    help="learned form: 1 starts a new flexible-CG recurrence after the network iterations")
     ap.add_argument("--net-iters", type=int, default=1,
                     help="learned form: network only in the first m FCG iterations (0: throughout)")
     ap.add_argument("--crop", type=int, default=64, help="learned form: training pair size (cells per side)")

@@ -295,6 +295,7 @@ struct FcgOptions {
     bool network = true;
     int net_every = 1;
     int net_iters = 0;  // > 0: the network only in the first net_iters iterations, sweeps alone after
+    bool net_restart = false;  // start a new recurrence (p = z) after the last network iteration
     int smooth = 4;
     double omega = 0.8;
 };

@@ -229,6 +229,7 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_fcg_no_network") d.fcg.network = false;
             else if (k == "tl_fcg_net_every") d.fcg.net_every = I();
             else if (k == "tl_fcg_net_iters") d.fcg.net_iters = I();
+            else if (k == "tl_fcg_net_restart") d.fcg.net_restart = true;
             else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();

@@ -562,10 +562,13 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
         }
         // beta, p, w, alpha, u[, r] with the scalars on the device; one read
-        const bool defer_r = sweeps_tb && !uses_net(it + 1) && it < max_iters;
+        // after the last network iteration a restart begins a new recurrence
+        // from the improved iterate (p = z), which needs r current
+        const bool restart_next = opt.net_restart && uses_net(it) && !uses_net(it + 1);
+        const bool defer_r = sweeps_tb && !uses_net(it + 1) && !restart_next && it < max_iters;
         op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r, !pr_next);
         pr_next = false;
-        first = false;
+        first = restart_next;
         if (beta_rrn) {
             // iteration it - 1 is complete: r updated by the pass, u by the
             // strip; what is pending now belongs to iteration it

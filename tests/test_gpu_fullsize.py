@@ -107,8 +107,9 @@ def test_c5_learned_fcg_properties(hip):
     """4096^2 held-out layout of rectangles and circles (seed 0), after a
     short training on two 1024^2 layouts with dt x 16 (the same face
     coefficients per cell): the learned flexible CG (network in the first
-    iteration) meets eps on the true residual without falling back, and its
-    summaries agree with FP64 PPCG's to 1e-10."""
+    iteration, then a new recurrence from that iterate) meets eps on the true
+    residual without falling back, and its summaries agree with FP64 PPCG's
+    to 1e-10."""
     import decks
     learner = hip.Learner(seed=3)
     try:
@@ -124,7 +125,7 @@ def test_c5_learned_fcg_properties(hip):
         learner.use()
         try:
             print("c5 4096^2 seed 0: learned FCG", flush=True)
-            got = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_net_iters=1"), dim=2)
+            got = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_net_iters=1\ntl_fcg_net_restart"), dim=2)
         finally:
             learner.use(False)
     finally:
