@@ -505,9 +505,9 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
     int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
     bool p_pending = false;
-    // CG iterations leave u += alpha p to the next iteration's p, w strip
-    // (2D): u_pending marks an update not yet applied
-    const bool defer_u = TL_DIM == 2;
+    // CG iterations leave u += alpha p to the next iteration's p, w sweep:
+    // u_pending marks an update not yet applied
+    const bool defer_u = true;
     bool u_pending = false;
     // mixed-precision PPCG state: rrn at the last residual replacement, and
     // true-residual checks that failed without halving the true rrn

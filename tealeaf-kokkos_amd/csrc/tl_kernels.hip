@@ -353,14 +353,19 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_rro(Geom g, KPtr k, int pre, 
 // applied on the fly: every stencil point of p is rebuilt from the old p and
 // r (z) with the same expression, the new p goes to p_out (a second buffer,
 // since neighbours still read the old one) and A is applied to it.
-template <bool FOLD>
+// With U (and FOLD) the previous iteration's CGCalcU, u += alpha p with that
+// iteration's alpha and p (both still in place), rides along: CGCalcUr's
+// sweep then updates r only (ext_cg_kernel.hpp:244-264, same expression).
+template <bool FOLD, bool U = false>
 __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const double* __restrict__ p,
                                                      const double* __restrict__ q, const DevScalars* __restrict__ ds,
-                                                     double* __restrict__ p_out, double* __restrict__ w, Reduce red)
+                                                     double* __restrict__ p_out, double* __restrict__ w, Reduce red,
+                                                     double* __restrict__ u)
 {
     Strip s = make_strip(g);
     double v[1] = {0.0};
     const double beta = FOLD ? ds->beta : 0.0;
+    const double alpha = U ? ds->alpha : 0.0;
     for (int n = 0; n < s.count; ++n, strip_advance(s)) {
         const long i = s.i;
         double sm, pc;
@@ -381,6 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const doubl
             sm = (1.0 + (kxe + kxc) + (kyn + kyc)) * pc - (kxe * pe + kxc * pw_) - (kyn * pn + kyc * ps);
 #endif
             p_out[i] = pc;
+            if (U) u[i] = u[i] + alpha * p[i];
         } else {
             pc = p[i];
             sm = smvp(g, p, k, i, s.kk, s.jj, s.ii);
@@ -1574,7 +1580,7 @@ double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt)
 double op_cg_calc_w(Chunk& c)
 {
     TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                  c.reducer()));
+                                  c.reducer(), nullptr));
     finish<1>(c, c.g.nblocks, S_PW, EP_STORE);
     return c.result(S_PW);
 }
@@ -1597,7 +1603,7 @@ void op_cg_calc_p(Chunk& c, double beta)
 void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
 {
     const double* q = c.preconditioner ? c.f[F_Z] : c.f[F_R];
-    if (fold_u && (!fold || TL_DIM != 2)) throw Error("the pending u update folds only into the 2D p, w strip");
+    if (fold_u && !fold) throw Error("the pending u update folds only into the p, w sweep");
     if (fold) {
 #if TL_DIM == 2
         const PWStrip t = pw_strip(c);
@@ -1619,13 +1625,18 @@ void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
         finish<1>(c, nb, S_PW, EP_PW);
         return;
 #else
-        TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_calc_w<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
-                                      c.reducer()));
+        if (fold_u) {
+            TIMED(c, K_CG_WPU, STRIP_LAUNCH((k_cg_calc_w<true, true>), c, kptr(c), c.f[F_P], q, c.ds, c.p_alt,
+                                            c.f[F_W], c.reducer(), c.f[F_U]));
+        } else {
+            TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_calc_w<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
+                                          c.reducer(), nullptr));
+        }
         std::swap(c.f[F_P], c.p_alt);
 #endif
     } else {
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                      c.reducer()));
+                                      c.reducer(), nullptr));
     }
     finish<1>(c, c.g.nblocks, S_PW, EP_PW);
 }
@@ -1992,7 +2003,7 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         }
         STRIP_LAUNCH(k_fcg_p, c, 0.0, c.ds, first ? 1 : 0, c.f[F_Z], c.f[F_P]);
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                      c.reducer()));
+                                      c.reducer(), nullptr));
         finish<1>(c, c.g.nblocks, S_PW, EP_FCG_PW);
         STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer());
         finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_ALPHA);
