@@ -112,6 +112,10 @@ struct Chunk {
     double* sd_alt = nullptr;  // ping-pong partner of sd (PPCG inner steps)
     double* p_alt = nullptr;   // ping-pong partner of p (CGCalcP folded into CGCalcW)
     double* r_alt = nullptr;   // ping-pong partner of r (temporal-blocked PPCG)
+    // p of the mesh's boundary cells as the depth-1 halo of p holds it
+    // after the last CG iteration before the Chebyshev switch (per face:
+    // west, east, south, north[, back, front]); allocated at first use
+    double* p_faces = nullptr;
 
     // Mixed-precision PPCG: FP32 copies of kx, ky (same layout, float
     // elements) and of the inner-step coefficients, made at the switch.
@@ -208,6 +212,12 @@ void op_cg_calc_ur_dev(Chunk& c, int n);
 void op_cg_calc_r_dev(Chunk& c, int n);
 void op_cg_calc_u_dev(Chunk& c);
 void op_cg_calc_p_dev(Chunk& c);
+// Chebyshev switch: keep the boundary cells' p (with the pending CGCalcP
+// applied when `pending`) for op_cg_calc_w_faces_dev.
+void op_capture_p_faces(Chunk& c, bool pending);
+// CGCalcW whose out-of-mesh neighbours come from the captured faces (a CG
+// iteration after Chebyshev steps reads that stale halo of p).
+void op_cg_calc_w_faces_dev(Chunk& c);
 void op_ppcg_calc_ur_dev(Chunk& c);  // u += alpha p, r -= alpha w; its r.r is not used
 void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in the PPCG loop
 double op_error(Chunk& c);        // waits and returns ds->error

@@ -505,6 +505,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     double eigmin = 0.0, eigmax = 0.0, theta = 0.0, cn = 0.0;
     int est_itc = 0, max_cheby_iters = 0, switch_step = 0;
     bool p_pending = false;
+    // the next CG iteration is the first since the Chebyshev switch: the
+    // reference has refreshed only u's halo since (tea_solve.f90:203-204),
+    // so its CGCalcW reads p's depth-1 halo of the last CG iteration
+    bool stale_p_halo = false;
     // CG iterations leave u += alpha p to the next iteration's p, w sweep:
     // u_pending marks an update not yet applied
     const bool defer_u = true;
@@ -583,9 +587,19 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                               "\n";
             }
             if (d.use_cheby) {
+                if (cheby_calc_steps > 0 && p_pending) {
+                    // a CG iteration ran since the last Chebyshev step (the
+                    // error was above 1): ChebyIterate reads its new p
+                    op_cg_calc_p_dev(c);
+                    p_pending = false;
+                }
                 if (cheby_calc_steps == 0) {
                     // tea_leaf_cheby_first_step (tea_solve.f90:658-750); the
-                    // pending p update is dead: ChebyInit overwrites p.
+                    // pending p update is dead in the mesh (ChebyInit
+                    // overwrites p) but lives on in the halo of p, which a
+                    // CG iteration after this step would read
+                    op_capture_p_faces(c, p_pending);
+                    stale_p_halo = true;
                     p_pending = false;
                     const double bb = op_calc_2norm(c, false);
                     op_cheby_init(c, &ch_alphas[1], &ch_betas[1], max_cheby_iters, theta, d.preconditioner, true);
@@ -671,7 +685,13 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
             ++cg_calc_steps;
-            if (defer_u) {
+            if (stale_p_halo) {
+                // p and u are up to date here: Chebyshev steps ran since the
+                // last CG iteration
+                op_cg_calc_w_faces_dev(c);
+                stale_p_halo = false;
+                op_cg_calc_ur_dev(c, want_history ? n : 0);
+            } else if (defer_u) {
                 // CGCalcUr's u += alpha p rides on the next iteration's p, w
                 // strip, which reads p anyway (bitwise the same update)
                 op_cg_calc_w_dev(c, p_pending, u_pending);

@@ -397,7 +397,6 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const doubl
     block_partial<1>(v, red);
 }
 
-#if TL_DIM == 2
 // Lane i receives lane i-1 / i+1 through DPP row shifts (wave_shr:1 /
 // wave_shl:1); the lane at the wave's edge keeps its own value.
 __device__ __forceinline__ double dpp_from_west(double v)
@@ -413,6 +412,285 @@ __device__ __forceinline__ double dpp_from_east(double v)
     return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
                             __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
 }
+
+#if TL_DIM == 3
+// 3D sweeps along z with a register window.  A lane owns column kk, a wave
+// row jj, and the lane walks the block's strip of planes (the tiling of
+// make_strip) keeping the stencil operand of planes ii-1, ii, ii+1 in
+// registers: x-neighbours come from the adjacent lanes (DPP; at the wave's
+// edges one extra load), y-neighbours are loaded, and kz of a plane's back
+// face is the previous plane's front face.  Raw inputs of the next plane
+// are loaded while the current plane is computed.  The loop count is the
+// same for every lane of a wave (lanes past the last column run on the
+// clamped column and store nothing), so the DPP sources are always live.
+// Same expressions, summation order and partials as the per-cell kernels
+// they replace.
+struct ZWalk {
+    long i;        // device index of the lane's cell in the current plane
+    long dys, dyn; // offsets of the y-neighbours (0 on the mesh faces)
+    long de;       // offset of the x-neighbour beyond the wave's edge
+    int i0, count; // first plane and planes in the strip (wave-uniform)
+    int lane;
+    bool xok;      // the lane's column is in the mesh (stores, sums)
+    bool edge;     // lane 0 or lane 63
+    bool east;     // the column is the last one (east neighbour = itself)
+};
+
+__device__ inline ZWalk make_zwalk(const Geom& g)
+{
+    const long t = xcd_tile(blockIdx.x, g.nblocks);
+    ZWalk z;
+    z.lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int tx = (int)(t % g.tiles_x);
+    const long rest = t / g.tiles_x;
+    const int ty = (int)(rest % g.tiles_y);
+    const int ts = (int)(rest / g.tiles_y);
+    const int kk = HALO + tx * WAVE + z.lane;
+    const int jj = HALO + ty * WAVES_PER_BLOCK + wave;
+    z.i0 = HALO + ts * g.strip;
+    z.count = min(g.strip, g.nz + HALO - z.i0);
+    if (jj > g.ny + 1 || z.count < 0) z.count = 0;
+    z.xok = kk <= g.nx + 1;
+    const int kc = min(kk, g.nx + 1);
+    z.east = kc == g.nx + 1;
+    z.edge = z.lane == 0 || z.lane == WAVE - 1;
+    const int ce = z.lane == 0 ? max(kc - 1, HALO) : min(kc + 1, g.nx + 1);
+    z.de = z.edge ? (long)(ce - kc) : 0;  // other lanes load their own cell (no branch around the load)
+    z.dys = jj > HALO ? -g.sx : 0;
+    z.dyn = jj < g.ny + 1 ? g.sx : 0;
+    z.i = gidx(g, kc, jj, z.i0);
+    return z;
+}
+
+// Index of plane ii + 1 (the lane's column), clamped to the last plane.
+__device__ inline long z_next(const Geom& g, long i, int ii)
+{
+    return ii < g.nz + 1 ? i + g.sy : i;
+}
+
+// x-neighbours of the centre value c: adjacent lanes, the extra load ec at
+// the wave's edges, the cell itself on the mesh faces (ec is the cell's own
+// value at the west face).
+__device__ inline void z_xnb(const ZWalk& z, double c, double ec, double& w, double& e)
+{
+    const double dw = dpp_from_west(c), de = dpp_from_east(c);
+    w = z.lane == 0 ? ec : dw;
+    e = z.east ? c : (z.lane == WAVE - 1 ? ec : de);
+}
+
+// Face coefficients of the current plane, and of the next one in flight.
+struct ZCoef {
+    double kxc, kxe, kyc, kyn, kzc, kzf;
+    double kxc1, kxe1, kyc1, kyn1, kzf1;
+    __device__ void load(const Geom& g, const KPtr& k, long i)
+    {
+        kxc = k.kx[i];
+        kxe = k.kx[i + 1];
+        kyc = k.ky[i];
+        kyn = k.ky[i + g.sx];
+        kzc = k.kz[i];
+        kzf = k.kz[i + g.sy];
+    }
+    __device__ void prefetch(const Geom& g, const KPtr& k, long i1)
+    {
+        kxc1 = k.kx[i1];
+        kxe1 = k.kx[i1 + 1];
+        kyc1 = k.ky[i1];
+        kyn1 = k.ky[i1 + g.sx];
+        kzf1 = k.kz[i1 + g.sy];
+    }
+    __device__ void advance()
+    {
+        kxc = kxc1;
+        kxe = kxe1;
+        kyc = kyc1;
+        kyn = kyn1;
+        kzc = kzf;
+        kzf = kzf1;
+    }
+    // the 7-point SMVP of smvp() from the neighbour values
+    __device__ double apply(double c, double w, double e, double s, double n, double b, double f) const
+    {
+        return (1.0 + (kxe + kxc) + (kyn + kyc) + (kzf + kzc)) * c - (kxe * e + kxc * w) - (kyn * n + kyc * s)
+             - (kzf * f + kzc * b);
+    }
+};
+
+// CGCalcP folded into CGCalcW (k_cg_calc_w<true, U>) along z: the operand
+// is the new p, beta p + r (z), rebuilt from the raw p and r (z) of every
+// point it is needed at; the old p of the centre feeds the folded u update.
+template <bool U>
+__global__ __launch_bounds__(BLOCK) void k_cg_pw_z(Geom g, KPtr k, const double* __restrict__ p,
+                                                   const double* __restrict__ q, const DevScalars* __restrict__ ds,
+                                                   double* __restrict__ p_out, double* __restrict__ w,
+                                                   double* __restrict__ u, Reduce red)
+{
+    ZWalk z = make_zwalk(g);
+    const double beta = ds->beta;
+    const double alpha = U ? ds->alpha : 0.0;
+    double acc = 0.0;
+    if (z.count > 0) {
+        long i = z.i;
+        // window: new p of planes ii-1 (b), ii (c), ii+1 (f); old p of ii, ii+1
+        const long ib = z.i0 > HALO ? i - g.sy : i;
+        double pb = beta * p[ib] + q[ib];
+        double poc = p[i];
+        double pc = beta * poc + q[i];
+        const long if0 = z_next(g, i, z.i0);
+        double pof = p[if0];
+        double pf = beta * pof + q[if0];
+        if (z.i0 == HALO) pb = pc;
+        double ps = beta * p[i + z.dys] + q[i + z.dys];
+        double pn = beta * p[i + z.dyn] + q[i + z.dyn];
+        double ec = beta * p[i + z.de] + q[i + z.de];
+        ZCoef kc;
+        kc.load(g, k, i);
+        double uc = U ? u[i] : 0.0;
+        for (int n = 0; n < z.count; ++n) {
+            const int ii = z.i0 + n;
+            // raw inputs of plane ii + 1 (whose front neighbour is plane ii + 2)
+            const long i1 = z_next(g, i, ii);
+            const long i2 = z_next(g, i1, ii + 1);
+            const double pp2 = p[i2], qq2 = q[i2];
+            const double ps1 = beta * p[i1 + z.dys] + q[i1 + z.dys];
+            const double pn1 = beta * p[i1 + z.dyn] + q[i1 + z.dyn];
+            const double pe1 = p[i1 + z.de], qe1 = q[i1 + z.de];
+            kc.prefetch(g, k, i1);
+            const double u1 = U ? u[i1] : 0.0;
+
+            double pw_, pe;
+            z_xnb(z, pc, ec, pw_, pe);
+            const double sm = kc.apply(pc, pw_, pe, ps, pn, pb, ii < g.nz + 1 ? pf : pc);
+            if (z.xok) {
+                p_out[i] = pc;
+                w[i] = sm;
+                if (U) u[i] = uc + alpha * poc;
+                acc += sm * pc;
+            }
+            pb = pc;
+            pc = pf;
+            poc = pof;
+            pof = pp2;
+            pf = beta * pp2 + qq2;
+            ps = ps1;
+            pn = pn1;
+            ec = beta * pe1 + qe1;
+            kc.advance();
+            uc = u1;
+            i = i1;
+        }
+    }
+    double v[1] = {acc};
+    block_partial<1>(v, red);
+}
+
+// k_ppcg_inner_fused along z: r -= A sd, u += sd, sd' = a sd + b M r.
+__global__ __launch_bounds__(BLOCK) void k_ppcg_inner_z(Geom g, KPtr k, int pre, double a, double b,
+                                                        const double* __restrict__ sd, double* __restrict__ r,
+                                                        double* __restrict__ u, const double* __restrict__ mi,
+                                                        double* __restrict__ sd_out)
+{
+    ZWalk z = make_zwalk(g);
+    if (z.count == 0) return;
+    long i = z.i;
+    double sb = sd[z.i0 > HALO ? i - g.sy : i];
+    double sc = sd[i];
+    double sf = sd[z_next(g, i, z.i0)];
+    if (z.i0 == HALO) sb = sc;
+    double ss = sd[i + z.dys], sn = sd[i + z.dyn];
+    double ec = sd[i + z.de];
+    ZCoef kc;
+    kc.load(g, k, i);
+    double rc = r[i], uc = u[i], mc = pre ? mi[i] : 0.0;
+    for (int n = 0; n < z.count; ++n) {
+        const int ii = z.i0 + n;
+        const long i1 = z_next(g, i, ii);
+        const long i2 = z_next(g, i1, ii + 1);
+        const double sf2 = sd[i2];
+        const double ss1 = sd[i1 + z.dys], sn1 = sd[i1 + z.dyn];
+        const double ec1 = sd[i1 + z.de];
+        kc.prefetch(g, k, i1);
+        const double r1 = r[i1], u1 = u[i1], m1 = pre ? mi[i1] : 0.0;
+
+        double sw, se;
+        z_xnb(z, sc, ec, sw, se);
+        const double sm = kc.apply(sc, sw, se, ss, sn, sb, ii < g.nz + 1 ? sf : sc);
+        if (z.xok) {
+            const double rv = rc - sm;
+            r[i] = rv;
+            u[i] = uc + sc;
+            sd_out[i] = a * sc + b * (pre ? mc * rv : rv);
+        }
+        sb = sc;
+        sc = sf;
+        sf = sf2;
+        ss = ss1;
+        sn = sn1;
+        ec = ec1;
+        kc.advance();
+        rc = r1;
+        uc = u1;
+        mc = m1;
+        i = i1;
+    }
+}
+
+// k_cheby<false, true> along z: r = u0 - A u, p = a p + b M r, u' = u + p.
+__global__ __launch_bounds__(BLOCK) void k_cheby_z(Geom g, KPtr k, int pre, double a, double b,
+                                                   const double* __restrict__ u, const double* __restrict__ u0,
+                                                   const double* __restrict__ mi, double* __restrict__ r,
+                                                   double* __restrict__ p, double* __restrict__ u_out)
+{
+    ZWalk z = make_zwalk(g);
+    if (z.count == 0) return;
+    long i = z.i;
+    double vb = u[z.i0 > HALO ? i - g.sy : i];
+    double vc = u[i];
+    double vf = u[z_next(g, i, z.i0)];
+    if (z.i0 == HALO) vb = vc;
+    double vs = u[i + z.dys], vn = u[i + z.dyn];
+    double ec = u[i + z.de];
+    ZCoef kc;
+    kc.load(g, k, i);
+    double u0c = u0[i], pc = p[i], mc = pre ? mi[i] : 0.0;
+    for (int n = 0; n < z.count; ++n) {
+        const int ii = z.i0 + n;
+        const long i1 = z_next(g, i, ii);
+        const long i2 = z_next(g, i1, ii + 1);
+        const double vf2 = u[i2];
+        const double vs1 = u[i1 + z.dys], vn1 = u[i1 + z.dyn];
+        const double ec1 = u[i1 + z.de];
+        kc.prefetch(g, k, i1);
+        const double u01 = u0[i1], p1 = p[i1], m1 = pre ? mi[i1] : 0.0;
+
+        double vw, ve;
+        z_xnb(z, vc, ec, vw, ve);
+        const double sm = kc.apply(vc, vw, ve, vs, vn, vb, ii < g.nz + 1 ? vf : vc);
+        if (z.xok) {
+            const double rv = u0c - sm;
+            r[i] = rv;
+            const double zr = pre ? mc * rv : rv;
+            const double pv = a * pc + b * zr;
+            p[i] = pv;
+            u_out[i] = vc + pv;
+        }
+        vb = vc;
+        vc = vf;
+        vf = vf2;
+        vs = vs1;
+        vn = vn1;
+        ec = ec1;
+        kc.advance();
+        u0c = u01;
+        pc = p1;
+        mc = m1;
+        i = i1;
+    }
+}
+#endif
+
+#if TL_DIM == 2
 
 // CGCalcP folded into CGCalcW as a streaming 2D sweep: each lane owns two
 // adjacent columns, a wave 128 columns and a strip of `rows` rows; the new p
@@ -651,6 +929,108 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_p(Geom g, double beta_v, cons
         const long i = s.i;
         p[i] = beta * p[i] + q[i];
     }
+}
+
+// Boundary cells' p for a CG iteration that follows Chebyshev steps.  One
+// value per boundary cell and face, faces in the order west, east
+// (ny [x nz] cells), south, north (nx [x nz]) and, in 3D, back, front
+// (nx x ny); within a face the slower of its two axes is the outer one.
+struct FaceOffsets {
+    long w, e, s, n, b, f, count;
+};
+
+__host__ __device__ inline FaceOffsets face_offsets(const Geom& g)
+{
+    FaceOffsets o;
+#if TL_DIM == 3
+    const long yz = (long)g.ny * g.nz, xz = (long)g.nx * g.nz, xy = (long)g.nx * g.ny;
+#else
+    const long yz = g.ny, xz = g.nx, xy = 0;
+#endif
+    o.w = 0;
+    o.e = yz;
+    o.s = 2 * yz;
+    o.n = 2 * yz + xz;
+    o.b = 2 * yz + 2 * xz;
+    o.f = o.b + xy;
+    o.count = o.f + xy;
+    return o;
+}
+
+// faces[t] = p (or beta p + q with the pending CGCalcP) of the boundary cell
+// that face entry t belongs to.
+__global__ void k_capture_faces(Geom g, FaceOffsets o, int pending, const DevScalars* __restrict__ ds,
+                                const double* __restrict__ p, const double* __restrict__ q,
+                                double* __restrict__ faces)
+{
+    const long t = (long)blockIdx.x * FLAT_BLOCK + threadIdx.x;
+    if (t >= o.count) return;
+    int kk, jj, ii = 0;
+    if (t < o.s) {  // west, east: t' = (ii - 2) ny + (jj - 2)
+        const long u = t < o.e ? t : t - o.e;
+        kk = t < o.e ? HALO : g.nx + 1;
+        jj = HALO + (int)(u % g.ny);
+#if TL_DIM == 3
+        ii = HALO + (int)(u / g.ny);
+#endif
+    } else if (t < o.b) {  // south, north: t' = (ii - 2) nx + (kk - 2)
+        const long u = t < o.n ? t - o.s : t - o.n;
+        jj = t < o.n ? HALO : g.ny + 1;
+        kk = HALO + (int)(u % g.nx);
+#if TL_DIM == 3
+        ii = HALO + (int)(u / g.nx);
+#endif
+    } else {  // back, front (3D): t' = (jj - 2) nx + (kk - 2)
+        const long u = t < o.f ? t - o.b : t - o.f;
+        ii = t < o.f ? HALO : g.nz + 1;
+        kk = HALO + (int)(u % g.nx);
+        jj = HALO + (int)(u / g.nx);
+    }
+    const long i = gidx(g, kk, jj, ii);
+    faces[t] = pending ? ds->beta * p[i] + q[i] : p[i];
+}
+
+// CGCalcW (:178-189) whose neighbours outside the mesh are the captured
+// faces instead of the reflected cell (the reference's SMVP reads a depth-1
+// halo of p that is still the one of the last CG iteration before the
+// Chebyshev switch).  Same expression, tiling and partials as k_cg_calc_w.
+__global__ __launch_bounds__(BLOCK) void k_cg_calc_w_faces(Geom g, FaceOffsets o, KPtr k,
+                                                           const double* __restrict__ p,
+                                                           const double* __restrict__ faces,
+                                                           double* __restrict__ w, Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        const long i = s.i;
+        const int kk = s.kk, jj = s.jj;
+#if TL_DIM == 3
+        const int ii = s.ii;
+        const long yz = (long)(ii - HALO) * g.ny + (jj - HALO), xz = (long)(ii - HALO) * g.nx + (kk - HALO);
+#else
+        const long yz = jj - HALO, xz = kk - HALO;
+#endif
+        const double ac = p[i];
+        const double aw = kk > HALO ? p[i - 1] : faces[o.w + yz];
+        const double ae = kk < g.nx + 1 ? p[i + 1] : faces[o.e + yz];
+        const double as = jj > HALO ? p[i - g.sx] : faces[o.s + xz];
+        const double an = jj < g.ny + 1 ? p[i + g.sx] : faces[o.n + xz];
+        const double kxc = k.kx[i], kxe = k.kx[i + 1];
+        const double kyc = k.ky[i], kyn = k.ky[i + g.sx];
+#if TL_DIM == 3
+        const long xy = (long)(jj - HALO) * g.nx + (kk - HALO);
+        const double ab = ii > HALO ? p[i - g.sy] : faces[o.b + xy];
+        const double af = ii < g.nz + 1 ? p[i + g.sy] : faces[o.f + xy];
+        const double kzc = k.kz[i], kzf = k.kz[i + g.sy];
+        const double sm = (1.0 + (kxe + kxc) + (kyn + kyc) + (kzf + kzc)) * ac - (kxe * ae + kxc * aw)
+                        - (kyn * an + kyc * as) - (kzf * af + kzc * ab);
+#else
+        const double sm = (1.0 + (kxe + kxc) + (kyn + kyc)) * ac - (kxe * ae + kxc * aw) - (kyn * an + kyc * as);
+#endif
+        w[i] = sm;
+        v[0] += sm * ac;
+    }
+    block_partial<1>(v, red);
 }
 
 // Second phase of every reduction: one workgroup adds the block partials in
@@ -1277,6 +1657,7 @@ Chunk::~Chunk()
     (void)hipFree(sd_alt);
     (void)hipFree(p_alt);
     (void)hipFree(r_alt);
+    (void)hipFree(p_faces);
     (void)hipFree(kxy32);
     (void)hipFree(d_rowtab);
     (void)hipFree(d_pack);
@@ -1626,11 +2007,11 @@ void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
         return;
 #else
         if (fold_u) {
-            TIMED(c, K_CG_WPU, STRIP_LAUNCH((k_cg_calc_w<true, true>), c, kptr(c), c.f[F_P], q, c.ds, c.p_alt,
-                                            c.f[F_W], c.reducer(), c.f[F_U]));
+            TIMED(c, K_CG_WPU, STRIP_LAUNCH(k_cg_pw_z<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
+                                            c.f[F_U], c.reducer()));
         } else {
-            TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_calc_w<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
-                                          c.reducer(), nullptr));
+            TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_pw_z<false>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
+                                          nullptr, c.reducer()));
         }
         std::swap(c.f[F_P], c.p_alt);
 #endif
@@ -1671,6 +2052,24 @@ void op_ppcg_calc_ur_dev(Chunk& c)
 void op_cg_calc_p_dev(Chunk& c)
 {
     TIMED(c, K_CG_P, STRIP_LAUNCH(k_cg_calc_p, c, 0.0, c.ds, c.f[F_P], c.preconditioner ? c.f[F_Z] : c.f[F_R]));
+}
+
+void op_capture_p_faces(Chunk& c, bool pending)
+{
+    const FaceOffsets o = face_offsets(c.g);
+    if (!c.p_faces) TL_HIP(hipMalloc(&c.p_faces, sizeof(double) * (size_t)o.count));
+    const unsigned nb = (unsigned)((o.count + FLAT_BLOCK - 1) / FLAT_BLOCK);
+    hipLaunchKernelGGL(k_capture_faces, dim3(nb), dim3(FLAT_BLOCK), 0, c.stream, c.g, o, (int)pending, c.ds,
+                       c.f[F_P], c.preconditioner ? c.f[F_Z] : c.f[F_R], c.p_faces);
+    check_launch();
+}
+
+void op_cg_calc_w_faces_dev(Chunk& c)
+{
+    if (!c.p_faces) throw Error("no boundary faces of p were captured");
+    TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w_faces, c, face_offsets(c.g), kptr(c), c.f[F_P], c.p_faces, c.f[F_W],
+                                  c.reducer()));
+    finish<1>(c, c.g.nblocks, S_PW, EP_PW);
 }
 
 void op_2norm_rrn_dev(Chunk& c)
@@ -1847,8 +2246,13 @@ void op_cheby_iterate(Chunk& c, int step1, bool fused)
     const double a = c.alphas[s], b = c.betas[s];
     const int pre = c.preconditioner;
     if (fused) {
+#if TL_DIM == 3
+        TIMED(c, K_CHEBY, STRIP_LAUNCH(k_cheby_z, c, kptr(c), pre, a, b, c.f[F_U], c.f[F_U0], c.f[F_MI], c.f[F_R],
+                                       c.f[F_P], c.u_alt));
+#else
         TIMED(c, K_CHEBY, STRIP_LAUNCH((k_cheby<false, true>), c, kptr(c), pre, 0.0, a, b, c.f[F_U], c.f[F_U0],
                                        c.f[F_MI], c.f[F_W], c.f[F_R], c.f[F_P], c.u_alt));
+#endif
         std::swap(c.f[F_U], c.u_alt);
     } else {
         STRIP_LAUNCH((k_cheby<false, false>), c, kptr(c), pre, 0.0, a, b, c.f[F_U], c.f[F_U0], c.f[F_MI], c.f[F_W],
@@ -1881,8 +2285,13 @@ void op_ppcg_inner(Chunk& c, int step1, bool fused)
     const double a = c.alphas[s], b = c.betas[s];
     const int pre = c.preconditioner;
     if (fused) {
+#if TL_DIM == 3
+        TIMED(c, K_PPCG_INNER, STRIP_LAUNCH(k_ppcg_inner_z, c, kptr(c), pre, a, b, c.f[F_SD], c.f[F_R], c.f[F_U],
+                                            c.f[F_MI], c.sd_alt));
+#else
         TIMED(c, K_PPCG_INNER, STRIP_LAUNCH(k_ppcg_inner_fused, c, kptr(c), pre, a, b, c.f[F_SD], c.f[F_R],
                                             c.f[F_U], c.f[F_MI], c.sd_alt));
+#endif
         std::swap(c.f[F_SD], c.sd_alt);
     } else {
         STRIP_LAUNCH(k_ppcg_calc_u, c, kptr(c), c.f[F_SD], c.f[F_R], c.f[F_U]);

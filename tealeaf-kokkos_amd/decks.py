@@ -74,16 +74,17 @@ def two_rect_2d(n: int, solver: str = "cg", steps: int = 10, eps: float = 1e-10,
 
 
 def cube_3d(n: int, solver: str = "cg", steps: int = 5, eps: float = 1e-10, dt: float = 0.004,
-            max_iters: int = 10000, presteps: int = 30, inner: int = 10,
-            extra: tuple[str, ...] = ()) -> str:
+            max_iters: int = 10000, presteps: int = 30, inner: int = 10, ny: int | None = None,
+            nz: int | None = None, extra: tuple[str, ...] = ()) -> str:
     """3D layout: ambient density 100 / energy 1e-4, a hot box over the first
     10% of each axis (density 0.1, energy 25) and a cuboid over 10-60% of
-    each axis (density 0.1, energy 0.1) on 10x10x10."""
+    each axis (density 0.1, energy 0.1) on 10x10x10; n x ny x nz cells
+    (ny, nz default to n)."""
     lines = [
         "state 1 density=100.0 energy=0.0001",
         "state 2 density=0.1 energy=25. geometry=cuboid xmin=0. xmax=1. ymin=0. ymax=1. zmin=0. zmax=1.",
         "state 3 density=0.1 energy=0.1 geometry=cuboid xmin=1. xmax=6. ymin=1. ymax=6. zmin=1. zmax=6.",
-        f"x_cells={n}", f"y_cells={n}", f"z_cells={n}",
+        f"x_cells={n}", f"y_cells={ny or n}", f"z_cells={nz or n}",
         "xmin=0.0", "ymin=0.0", "zmin=0.0", "xmax=10.0", "ymax=10.0", "zmax=10.0",
     ]
     return _finish(lines + _common(solver, steps, eps, dt, max_iters, presteps, inner, list(extra)))

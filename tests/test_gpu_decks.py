@@ -66,6 +66,11 @@ CASES_2D = [
     # meshes a few cells wide: every strip and tile is an edge one, reflections repeat
     ("chebyshev", 3, 2, {"ny": 40, "presteps": 5}), ("ppcg", 2, 2, {"ny": 17, "presteps": 5}),
     ("chebyshev", 17, 2, {"ny": 2, "presteps": 3}),
+    # preconditioned Chebyshev whose ||r||^2 is still above 1 after the first step: a CG
+    # iteration follows it (reading the halo of p the reference left from before the switch),
+    # then Chebyshev resumes on that iteration's p
+    ("chebyshev", 64, 2, {"dt": 0.4, "presteps": 20, "extra": ("tl_preconditioner_on",)}),
+    ("chebyshev", 96, 2, {"ny": 40, "dt": 0.4, "presteps": 20, "extra": ("tl_preconditioner_on",)}),
 ]
 
 
@@ -81,6 +86,27 @@ def test_decks_2d(hip, oracle, solver, n, steps, kw):
 def test_decks_3d(hip, oracle, solver, n, steps):
     import decks
     deck = decks.cube_3d(n, solver, steps=steps, max_iters=3000)
+    gpu, ref = run_both(hip, oracle, deck, 3)
+    compare(gpu, ref)
+
+
+# Ragged 3D meshes for the sweeps along z: a second, partial 64-column tile
+# (wave-edge neighbours across tiles, lanes past the last column), a partial
+# 4-row tile and a partial strip of planes, with and without the diagonal
+# preconditioner.
+CASES_3D_RAGGED = [
+    ("cg", 70, 9, 37, 0.04, 20, ()), ("cg", 70, 9, 37, 0.04, 20, ("tl_preconditioner_on",)),
+    ("ppcg", 67, 6, 21, 0.4, 5, ()), ("ppcg", 67, 6, 21, 0.04, 20, ("tl_preconditioner_on",)),
+    ("chebyshev", 65, 7, 19, 0.4, 5, ()), ("chebyshev", 65, 7, 19, 0.4, 20, ("tl_preconditioner_on",)),
+    # a CG iteration after the first Chebyshev step (see the 2D cases)
+    ("chebyshev", 24, 24, 24, 0.4, 20, ("tl_preconditioner_on",)),
+]
+
+
+@pytest.mark.parametrize("solver,nx,ny,nz,dt,presteps,extra", CASES_3D_RAGGED)
+def test_decks_3d_ragged(hip, oracle, solver, nx, ny, nz, dt, presteps, extra):
+    import decks
+    deck = decks.cube_3d(nx, solver, steps=2, max_iters=3000, ny=ny, nz=nz, dt=dt, presteps=presteps, extra=extra)
     gpu, ref = run_both(hip, oracle, deck, 3)
     compare(gpu, ref)
 
