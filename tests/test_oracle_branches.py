@@ -16,7 +16,11 @@ models share no code with the oracle and are written in other forms:
     evaluated on the eigendecomposition of A;
   - Jacobi as the matrix splitting A = D' - N' the reference's kernel
     iterates (D' includes the boundary faces' coefficients, whose mirrored
-    neighbour is the cell itself, ext_jacobi_kernel.hpp:85-117).
+    neighbour is the cell itself, ext_jacobi_kernel.hpp:85-117);
+  - preconditioned Chebyshev in explicit p form with the closed-form
+    coefficients, including the CG iterations the switch test lets run
+    between Chebyshev steps and the stale halo of p the first of them
+    reads.
 
 Each test runs the oracle's whole driver on a deck (capped or to
 convergence), reads its final fields, and compares u, the iteration count,
@@ -373,3 +377,119 @@ def test_jacobi_l1_stop(oracle):
     eps = math.sqrt(hist[n_stop - 1][0] * hist[n_stop - 2][0])
     rep, _ = kept_run(oracle, deck("jacobi", 1000, eps, dt=0.5))
     assert rep["steps"][0]["iters"] == n_stop - 1
+
+
+def pre_chebyshev_model(sys_, presteps, max_iters, eps, stale=True, p_after_cg=True):
+    """Preconditioned Chebyshev (tl_preconditioner_on, M = 1 / diagonal of
+    the SMVP including the boundary faces, ext_cg_kernel.hpp:119-121) in
+    explicit p form: p = a_k p + b_k M r, u += p, with a_k = w_{k+1} - 1 and
+    b_k = w_{k+1} / theta from the closed-form omega.  While ||r||^2 after a
+    Chebyshev step is above 1 the reference's switch test fails and a CG
+    iteration runs on the Chebyshev state (tea_solve.f90:169-180, 378-433).
+    The first such iteration's SMVP reads the depth-1 halo of p that the
+    last CG iteration before the switch left (only u's halo is refreshed in
+    Chebyshev mode, tea_solve.f90:203-204): A p + bsum (p - p_stale).
+    Planted mistakes: stale=False uses the mirrored halo there;
+    p_after_cg=False lets Chebyshev resume on the p from before that CG
+    iteration's p update."""
+    A, b = sys_.A, sys_.b
+    Mi = 1.0 / (np.diag(A) + sys_.bsum)
+    u = b.copy()
+    r = b - A @ u
+    p = Mi * r
+    rro = r @ p
+    error = 1e10
+    cg_a, cg_b = [], []
+    steps = 0
+    out = {"switch_step": 0, "est_itc": 0, "hybrid": 0}
+    p_stale = None
+    n_done = max_iters
+    for n in range(1, max_iters + 1):
+        if n >= presteps and error <= 1.0:
+            if steps == 0:
+                M = max_iters - n + 2
+                rro = error
+                lo, hi = lanczos_eigs(cg_a, cg_b)
+                eigmin, eigmax = lo * F95, hi * F105
+                theta, delta = (eigmax + eigmin) / 2, (eigmax - eigmin) / 2
+                sigma = theta / delta
+                cn = eigmax / eigmin
+                out.update(switch_step=n, eigmin=eigmin, eigmax=eigmax)
+                p_stale = p.copy()
+                bb = b @ b
+                r = b - A @ u
+                p = Mi * r / theta
+                u = u + p
+                r = b - A @ u
+                w = omega(M, sigma)
+                p = (w - 1.0) * p + (w / theta) * (Mi * r)
+                u = u + p
+                error = r @ r
+                gamm = (math.sqrt(cn) - 1.0) / (math.sqrt(cn) + 1.0)
+                out["est_itc"] = int(math.floor(math.log(EPS64 * bb / (4.0 * error)) / (2.0 * math.log(gamm)) + 0.5))
+                steps = 1
+            else:
+                r = b - A @ u
+                w = omega(steps, sigma)
+                p = (w - 1.0) * p + (w / theta) * (Mi * r)
+                u = u + p
+                if n >= out["est_itc"] and n % 10 == 0:
+                    error = r @ r
+            steps += 1
+        else:
+            wv = A @ p
+            if p_stale is not None and stale:
+                wv = wv + sys_.bsum * (p - p_stale)
+            if steps > 0:
+                out["hybrid"] += 1
+                p_stale = None  # the CG iteration's halo update refreshes p's halo from now on
+            alpha = rro / (p @ wv)
+            u = u + alpha * p
+            r = r - alpha * wv
+            z = Mi * r
+            rrn = r @ z
+            beta = rrn / rro
+            if steps == 0:
+                cg_a.append(alpha)
+                cg_b.append(beta)
+            p_new = z + beta * p
+            if steps == 0 or p_after_cg:
+                p = p_new
+            error = rro = rrn
+        if abs(error) < eps:
+            n_done = n - 1
+            break
+    out["iters"] = n_done
+    out["u"] = u
+    return out
+
+
+@pytest.mark.parametrize("extra", [1, 3, 6, 12])
+def test_preconditioned_chebyshev_with_cg_iterations(oracle, extra):
+    """Preconditioned Chebyshev on a deck whose ||r||^2 stays above 1 after
+    the first Chebyshev step, so CG iterations run between Chebyshev steps:
+    capped solves give the model's u; a mirrored halo in the first of those
+    CG iterations, or Chebyshev resuming on the p from before the CG
+    iteration's update, does not."""
+    import decks
+
+    def pdeck(cap, eps):
+        return decks.two_rect_2d(14, "chebyshev", steps=1, eps=eps, dt=2.0, max_iters=cap, presteps=8, ny=11,
+                                 extra=("tl_preconditioner_on",))
+
+    full, _ = kept_run(oracle, pdeck(10000, 1e-16))
+    st = full["steps"][0]
+    ns = st["switch_step"]
+    assert st["cg_steps"] > ns  # CG iterations after the switch
+    cap = ns + extra
+    rep, f = kept_run(oracle, pdeck(cap, 0.0))
+    s = System(f)
+    m = pre_chebyshev_model(s, 8, cap, 0.0)
+    assert m["switch_step"] == rep["steps"][0]["switch_step"] == ns
+    assert m["hybrid"] >= 1
+    assert rel(interior(f, "u"), m["u"]) < TOL
+    # a planted mistake may also diverge (nan): anything but agreement counts
+    with np.errstate(all="ignore"):
+        assert not rel(interior(f, "u"), pre_chebyshev_model(s, 8, cap, 0.0, stale=False)["u"]) <= MISS
+        if extra >= 3:
+            assert not rel(interior(f, "u"), pre_chebyshev_model(s, 8, cap, 0.0, p_after_cg=False)["u"]) <= MISS
