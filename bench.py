@@ -164,6 +164,29 @@ def cpu_baseline(deck: str, n: int, threads: int, steps: list[dict], budget_s: f
     }
 
 
+def inner_probe(n: int, launches: int) -> dict:
+    """A short program for hardware-counter runs of the 3D PPCG inner steps:
+    an n^3 chunk through the ext ABI with seeded random sd, r, u and face
+    coefficients, then `launches` two-step sweeps (tl_ppcg_inner_pair, as the
+    3D driver runs them); the kernel trace of the run times them."""
+    import numpy as np
+
+    L = tealeaf_hip.lib(3)
+    ch = tealeaf_hip.ExtChunk(L, 3, n, n, n)
+    rng = np.random.default_rng(1)
+    for name in ("sd", "r", "u"):
+        ch.set(name, rng.uniform(-1.0, 1.0, size=ch.shape))
+    for name in ("kx", "ky", "kz"):
+        ch.set(name, rng.uniform(0.1, 4.0, size=ch.shape))
+    ch.ppcg_init([0.3] * 10, [0.3] * 10, 1.0)
+    for _ in range(launches):
+        if L.tl_ppcg_inner_pair(ch.c, 1) != 0:
+            raise RuntimeError(L.tl_last_error().decode())
+    ch.get("u")  # waits for the sweeps
+    ch.close()
+    return {"probe": "ppcg_inner2 (3D two-step sweep)", "n": n, "launches": launches}
+
+
 def kernel_probe(n: int, launches: int) -> dict:
     """A short program for hardware-counter runs: the 2D n^2 two-rectangle
     chunk set up through the ext ABI (generate, CG init, 40 CG iterations),
@@ -571,6 +594,8 @@ def main() -> int:
                     help="only run the plain copy kernel (counter calibration)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
+    ap.add_argument("--inner-probe", action="store_true",
+                    help="only run 3D two-step PPCG inner sweeps on an n^3 chunk (counter runs)")
     ap.add_argument("--stream-probe", action="store_true",
                     help="only run the streaming probes of the attainable HBM rate")
     ap.add_argument("--cpu-check", action="store_true",
@@ -592,6 +617,9 @@ def main() -> int:
         return 0
     if args.kernel_probe:
         print(json.dumps(kernel_probe(n, 20)), flush=True)
+        return 0
+    if args.inner_probe:
+        print(json.dumps(inner_probe(args.n or 256, 9)), flush=True)
         return 0
     if args.stream_probe:
         import ctypes

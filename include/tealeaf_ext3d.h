@@ -147,6 +147,13 @@ void tl_set_progress(int on);
 int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const double* alphas, const double* betas,
                         int nsteps, double* rrn);
 
+/* PPCG inner steps step1 and step1 + 1 (1-based) in one sweep, as the 3D
+ * driver runs them (no counterpart in the reference; the two
+ * ext_ppcg_inner_ calls, ext_core.cpp:554-567, fused), on a chunk after
+ * ext_ppcg_init_: r, u and sd as after those calls.  3D only; otherwise
+ * returns -1 with tl_last_error() set. */
+int tl_ppcg_inner_pair(const int* chunk, int step1);
+
 /* Chebyshev iterations step1 .. step1 + count - 1 (1-based) as the driver
  * runs them (no counterpart in the reference; the ext_cheby_solver_iterate_
  * calls of those steps, ext_core.cpp:501-515, in temporal-blocked passes of

@@ -52,7 +52,7 @@ enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
 enum KernelId {
     K_CG_INIT = 0, K_CG_W, K_CG_WP, K_CG_UR, K_CG_P, K_CHEBY, K_PPCG_SD, K_PPCG_INNER, K_PPCG_OUTER,
     K_JACOBI, K_RESIDUAL, K_2NORM, K_COPY_U, K_FINALISE, K_SUMMARY, K_HALO, K_PPCG_MIXED, K_CHEBY_TB, K_JACOBI_TB,
-    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_FCG_SMOOTH_R, K_FCG_STRIP_ZR,
+    K_CG_WPU, K_CG_R, K_CG_U, K_FCG_SMOOTH, K_FCG_STRIP, K_FCG_SMOOTH_R, K_FCG_STRIP_ZR, K_PPCG_INNER2,
     K_COUNT
 };
 const char* kernel_name(int k);
@@ -366,6 +366,9 @@ void op_cheby_iterate(Chunk& c, int step1, bool fused);  // 1-based step
 void op_ppcg_init(Chunk& c, const double* alphas, const double* betas, int n, double theta);
 void op_ppcg_init_sd(Chunk& c);
 void op_ppcg_inner(Chunk& c, int step1, bool fused);  // 1-based step
+// 3D: inner steps step1 and step1 + 1 in one sweep (r and sd change
+// buffers); throws in 2D.
+void op_ppcg_inner2(Chunk& c, int step1);
 
 void op_jacobi_init(Chunk& c, int coefficient, double dt);
 double op_jacobi_solve(Chunk& c, bool fused);

@@ -651,7 +651,14 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     op_ppcg_calc_ur_dev(c);
                     // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
                     op_ppcg_init_sd(c);
-                    for (int step = 1; step <= d.ppcg_inner_steps; ++step) op_ppcg_inner(c, step, true);
+                    for (int step = 1; step <= d.ppcg_inner_steps;) {
+                        if (TL_DIM == 3 && step < d.ppcg_inner_steps) {
+                            op_ppcg_inner2(c, step);  // two inner steps per sweep
+                            step += 2;
+                        } else {
+                            op_ppcg_inner(c, step++, true);
+                        }
+                    }
                     op_2norm_rrn_dev(c);
                 }
                 p_pending = true;

@@ -127,6 +127,19 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
     return rc;
 }
 
+// PPCG inner steps step1 and step1 + 1 in one 3D sweep, as the driver runs
+// them, on a chunk after ext_ppcg_init_: r, u and sd as after the two
+// ext_ppcg_inner_ calls.
+int tl_ppcg_inner_pair(const int* chunk, int step1)
+{
+    int rc = -1;
+    guarded([&] {
+        op_ppcg_inner2(*stencil_chunk(chunk), step1);
+        rc = 0;
+    });
+    return rc;
+}
+
 // Chebyshev iterations step1 .. step1 + count - 1 on a chunk after
 // ext_cheby_solver_init_, as the driver runs them: temporal-blocked passes
 // of up to 10, the last writing r and the first phase of ||r||^2.  u, p and

@@ -636,6 +636,119 @@ __global__ __launch_bounds__(BLOCK) void k_ppcg_inner_z(Geom g, KPtr k, int pre,
     }
 }
 
+// Two PPCG inner steps (PPCGCalcU + PPCGCalcSd twice) in one 3D sweep
+// along z.  A workgroup of 16 waves owns a tile of 64 columns (lane l:
+// column kk0 - 2 + l) by 16 rows (wave w: row jj0 - 2 + w) and writes the
+// 60 x 12 cells in its middle: the first step's sd is needed one cell
+// beyond them, the input sd two.  At step t the input sd of plane
+// a = i0 - 2 + t arrives, the first step runs at plane a - 1 and the second
+// at a - 2; row neighbours go through LDS (two buffers per level, one
+// barrier per step), column neighbours through DPP, plane neighbours stay in
+// registers, and the next step's inputs are loaded while this one computes.
+// Cells outside the mesh run on the clamped cell and are never used: every
+// neighbour across a mesh face is replaced by the cell itself, as after the
+// reflective halo update.  Neighbouring tiles read the input r and sd at
+// their halo, so r goes to r_out and sd to sd_out; u is read only at the
+// cells this tile writes and is updated in place.  Same expressions as
+// k_ppcg_inner_z: bitwise two of its launches.
+constexpr int PZ_WAVES = 16;
+constexpr int PZ_TX = WAVE - 4, PZ_TY = PZ_WAVES - 4;
+
+struct PZGeom {
+    int tiles_x, tiles_y, strip;
+    long nblocks;
+};
+
+__global__ __launch_bounds__(WAVE * PZ_WAVES) void k_ppcg_inner2_z(
+    Geom g, PZGeom pz, KPtr k, int pre, double a1, double b1, double a2, double b2, const double* __restrict__ sd,
+    const double* __restrict__ r, double* __restrict__ u, const double* __restrict__ mi, double* __restrict__ sd_out,
+    double* __restrict__ r_out)
+{
+    __shared__ double l0[2][PZ_WAVES][WAVE];  // input sd of one plane, per row
+    __shared__ double l1[2][PZ_WAVES][WAVE];  // first-step sd of one plane, per row
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const long blk = xcd_tile(blockIdx.x, pz.nblocks);
+    const int tx = (int)(blk % pz.tiles_x);
+    const long rest = blk / pz.tiles_x;
+    const int ty = (int)(rest % pz.tiles_y);
+    const int ts = (int)(rest / pz.tiles_y);
+    const int kk = HALO + tx * PZ_TX - 2 + lane;
+    const int jj = HALO + ty * PZ_TY - 2 + w;
+    const int i0 = HALO + ts * pz.strip;
+    const int i1 = min(i0 + pz.strip, g.nz + HALO);  // output planes [i0, i1)
+    const int nz1 = g.nz + 1;
+    const int kc = min(max(kk, HALO), g.nx + 1), jc = min(max(jj, HALO), g.ny + 1);
+    const bool out = lane >= 2 && lane < WAVE - 2 && w >= 2 && w < PZ_WAVES - 2 && kk <= g.nx + 1 && jj <= g.ny + 1;
+    const bool fw = kk > HALO, fe = kk < g.nx + 1, fs = jj > HALO, fn = jj < g.ny + 1;
+    const int ws = max(w - 1, 0), wn = min(w + 1, PZ_WAVES - 1);
+    const long col = gidx(g, kc, jc, 0);
+    auto at = [&](int ii) { return col + (long)min(max(ii, HALO), nz1) * g.sy; };
+
+    // inputs of the coming step: sd of plane a, r, u, mi and faces of plane a - 1
+    double nsd = sd[at(i0 - 2)];
+    long ib = at(i0 - 3);
+    double nr = r[ib], nu = u[ib], nm = pre ? mi[ib] : 0.0;
+    ZCoef nk;
+    nk.load(g, k, ib);
+    double s0b = 0.0, s0c = 0.0, s0f = 0.0;  // input sd of planes b - 1, b, b + 1 (b = a - 1)
+    double s1b = 0.0, s1c = 0.0, s1f = 0.0;  // first-step sd of planes c - 1, c, c + 1 (c = a - 2)
+    double r1c = 0.0, u1c = 0.0, m1c = 0.0;  // first-step r, u and mi of plane c
+    ZCoef kcp = nk;                          // faces of plane c
+    const int steps = i1 - i0 + 4;
+    for (int t = 0; t < steps; ++t) {
+        const int a = i0 - 2 + t, b = a - 1, c = a - 2;
+        const double sda = nsd, r0b = nr, u0b = nu, mib = nm;
+        const ZCoef kb = nk;
+        // the next step's inputs
+        nsd = sd[at(a + 1)];
+        ib = at(a);
+        nr = r[ib];
+        nu = u[ib];
+        nm = pre ? mi[ib] : 0.0;
+        nk.load(g, k, ib);
+
+        s0b = s0c;
+        s0c = s0f;
+        s0f = sda;
+        const int rb = (t + 1) & 1;  // buffers written at step t - 1
+        const double s0s = l0[rb][ws][lane], s0n = l0[rb][wn][lane];
+        const double s1s = l1[rb][ws][lane], s1n = l1[rb][wn][lane];
+        // first step at plane b
+        double sd1b = 0.0, r1b = 0.0, u1b = 0.0;
+        if (t >= 2) {
+            const double dw = dpp_from_west(s0c), de = dpp_from_east(s0c);
+            const double sm = kb.apply(s0c, fw ? dw : s0c, fe ? de : s0c, fs ? s0s : s0c, fn ? s0n : s0c,
+                                       b > HALO ? s0b : s0c, b < nz1 ? s0f : s0c);
+            r1b = r0b - sm;
+            u1b = u0b + s0c;
+            sd1b = a1 * s0c + b1 * (pre ? mib * r1b : r1b);
+        }
+        // second step at plane c
+        s1b = s1c;
+        s1c = s1f;
+        s1f = sd1b;
+        if (t >= 4) {
+            const double dw = dpp_from_west(s1c), de = dpp_from_east(s1c);
+            const double sm = kcp.apply(s1c, fw ? dw : s1c, fe ? de : s1c, fs ? s1s : s1c, fn ? s1n : s1c,
+                                        c > HALO ? s1b : s1c, c < nz1 ? s1f : s1c);
+            const double rv = r1c - sm;
+            if (out) {
+                const long i = gidx(g, kc, jc, c);
+                r_out[i] = rv;
+                u[i] = u1c + s1c;
+                sd_out[i] = a2 * s1c + b2 * (pre ? m1c * rv : rv);
+            }
+        }
+        r1c = r1b;
+        u1c = u1b;
+        m1c = mib;
+        kcp = kb;
+        l0[t & 1][w][lane] = sda;
+        l1[t & 1][w][lane] = sd1b;
+        __syncthreads();
+    }
+}
+
 // k_cheby<false, true> along z: r = u0 - A u, p = a p + b M r, u' = u + p.
 __global__ __launch_bounds__(BLOCK) void k_cheby_z(Geom g, KPtr k, int pre, double a, double b,
                                                    const double* __restrict__ u, const double* __restrict__ u0,
@@ -1470,7 +1583,7 @@ const char* kernel_name(int k)
                                          "jacobi_solve", "calc_residual", "calc_2norm", "copy_u", "finalise",
                                          "field_summary", "update_halo", "ppcg_outer_mixed", "cheby_tb",
                                          "jacobi_tb", "cg_calc_p_w_u", "cg_calc_r", "cg_calc_u", "fcg_smooth",
-                                         "fcg_p_w", "fcg_smooth_r", "fcg_p_w_zr"};
+                                         "fcg_p_w", "fcg_smooth_r", "fcg_p_w_zr", "ppcg_inner2"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -1486,6 +1599,7 @@ double kernel_bytes_per_cell(int k)
     case K_CHEBY: return 48.0 + kb;        // u, u0, p, coefficients -> r, p, u'
     case K_PPCG_SD: return 16.0;           // r -> sd
     case K_PPCG_INNER: return 48.0 + kb;   // sd, r, u, coefficients -> r, u, sd'
+    case K_PPCG_INNER2: return 48.0 + kb;  // the same for two inner steps
     case K_JACOBI: return 24.0 + kb;       // u, u0, coefficients -> u'
     case K_RESIDUAL: return 24.0 + kb;     // u, u0, coefficients -> r
     case K_2NORM: return 8.0;
@@ -2297,6 +2411,47 @@ void op_ppcg_inner(Chunk& c, int step1, bool fused)
         STRIP_LAUNCH(k_ppcg_calc_u, c, kptr(c), c.f[F_SD], c.f[F_R], c.f[F_U]);
         STRIP_LAUNCH(k_ppcg_calc_sd, c, pre, a, b, c.f[F_R], c.f[F_MI], c.f[F_SD]);
     }
+}
+
+void op_ppcg_inner2(Chunk& c, int step1)
+{
+#if TL_DIM == 3
+    const int s = step1 - 1;
+    if (s < 0 || s + 1 >= (int)c.alphas.size())
+        throw Error("PPCG steps " + std::to_string(step1) + ", " + std::to_string(step1 + 1) +
+                    " outside the stored coefficients");
+    PZGeom pz;
+    pz.tiles_x = (c.g.nx + PZ_TX - 1) / PZ_TX;
+    pz.tiles_y = (c.g.ny + PZ_TY - 1) / PZ_TY;
+    // strip length: fewest rounds of resident workgroups (one per CU) times
+    // the steps each walks (its planes + 4)
+    const long cus = std::max(1, c.simds / 4), cols = (long)pz.tiles_x * pz.tiles_y;
+    long best = -1;
+    pz.strip = c.g.nz;
+    for (int S : {64, 48, 32, 24, 16, 12, 8}) {
+        const int len = std::min(S, c.g.nz);
+        const long blocks = cols * ((c.g.nz + len - 1) / len);
+        const long cost = ((blocks + cus - 1) / cus) * (len + 4);
+        if (best < 0 || cost < best) {
+            best = cost;
+            pz.strip = len;
+        }
+    }
+    pz.nblocks = cols * ((c.g.nz + pz.strip - 1) / pz.strip);
+    const int pre = c.preconditioner;
+    TIMED(c, K_PPCG_INNER2, {
+        hipLaunchKernelGGL(k_ppcg_inner2_z, dim3((unsigned)pz.nblocks), dim3(WAVE * PZ_WAVES), 0, c.stream, c.g, pz,
+                           kptr(c), pre, c.alphas[s], c.betas[s], c.alphas[s + 1], c.betas[s + 1], c.f[F_SD],
+                           c.f[F_R], c.f[F_U], c.f[F_MI], c.sd_alt, c.r_alt);
+        check_launch();
+    });
+    std::swap(c.f[F_SD], c.sd_alt);
+    std::swap(c.f[F_R], c.r_alt);
+#else
+    (void)c;
+    (void)step1;
+    throw Error("the two-step PPCG sweep is 3D only");
+#endif
 }
 
 // ext_core.cpp:340-358

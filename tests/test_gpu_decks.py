@@ -111,6 +111,16 @@ def test_decks_3d_ragged(hip, oracle, solver, nx, ny, nz, dt, presteps, extra):
     compare(gpu, ref)
 
 
+@pytest.mark.parametrize("inner", [1, 3, 4])
+def test_decks_3d_ppcg_inner_counts(hip, oracle, inner):
+    """3D PPCG runs its inner steps two per sweep: odd counts end with a
+    single step, and one step is a single sweep."""
+    import decks
+    deck = decks.cube_3d(67, "ppcg", steps=2, max_iters=3000, ny=6, nz=21, dt=0.4, presteps=5, inner=inner)
+    gpu, ref = run_both(hip, oracle, deck, 3)
+    compare(gpu, ref)
+
+
 def test_random_rect_deck(hip, oracle):
     import decks
     deck = decks.random_rects_2d(192, seed=3, nrect=32, steps=2)

@@ -92,6 +92,55 @@ def test_setup_and_field_summary(hip, oracle, dim):
     o.close()
 
 
+# one tile; x, y and z ragged; three tiles each way with a 5-column and
+# 2-row remainder; the smallest mesh
+PAIR_SHAPES = [(13, 11, 9), (70, 9, 37), (125, 26, 40), (2, 2, 2)]
+
+
+@pytest.mark.parametrize("shape", PAIR_SHAPES)
+@pytest.mark.parametrize("pre", [False, True])
+def test_ppcg_inner_pair_3d(hip, oracle, shape, pre):
+    """The 3D driver's two-step PPCG sweep (tl_ppcg_inner_pair) against two
+    ext_ppcg_inner_ calls each (with the depth-1 halo update of sd before
+    each, as tea_solve.f90 makes it) on the oracle, for 10 inner steps:
+    sd, u and r bitwise on the interior."""
+    nx, ny, nz = shape
+    g = hip.ExtChunk(hip.lib(3), 3, nx, ny, nz)
+    o = hip.ExtChunk(oracle.lib(3), 3, nx, ny, nz)
+    cg_prelude(g, o, 3, pre)
+    a, b, theta = oracle.calc_ch_coefs(0.9, 40.0, 10)
+    for ch in (g, o):
+        ch.ppcg_init(a, b, theta)
+        ch.calculate_residual()
+        ch.update_halo([FIELD_U, FIELD_P], 1)
+    pw = o.cg_calc_w()
+    g.cg_calc_w()
+    alpha = 0.5 / pw
+    for ch in (g, o):
+        ch.cg_calc_ur(alpha)
+        ch.ppcg_init_sd()
+    for step in range(1, 11, 2):
+        rc = g.L.tl_ppcg_inner_pair(g.c, step)
+        assert rc == 0, g.L.tl_last_error()
+        for s in (step, step + 1):
+            o.update_halo([FIELD_SD], 1)
+            o.ppcg_inner(s)
+        assert_same(g, o, ["sd", "u", "r"], region="interior")
+    g.close()
+    o.close()
+
+
+def test_ppcg_inner_pair_refused_in_2d(hip):
+    """The two-step sweep is 3D only: the 2D library refuses it."""
+    g = hip.ExtChunk(hip.lib(2), 2, 16, 16)
+    a = [0.5] * 4
+    g.ppcg_init(a, a, 1.0)
+    assert g.L.tl_ppcg_inner_pair(g.c, 1) == -1
+    assert b"3D only" in g.L.tl_last_error()
+    g.L.tl_clear_error()
+    g.close()
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("coef", [1, 2])
 @pytest.mark.parametrize("pre", [False, True])
