@@ -45,6 +45,7 @@ struct Deck {
     double end_time = 10.0;
     int end_step = 640000;  // g_ibig (data.f90:28)
     int summary_frequency = 10;
+    int visit_frequency = 0;
     int coefficient = CONDUCTIVITY;
     bool errswitch = false;
     int presteps = 30;
@@ -205,7 +206,7 @@ Deck parse_deck(const std::string& text)
             else if (k == "zmax") d.zmax = R();
             else if (k == "z_cells") d.z_cells = I();
 #endif
-            else if (k == "visit_frequency") (void)I();
+            else if (k == "visit_frequency") d.visit_frequency = I();
             else if (k == "summary_frequency") d.summary_frequency = I();
             else if (k == "tl_ch_cg_presteps") d.presteps = I();
             else if (k == "tl_ppcg_inner_steps") d.ppcg_inner_steps = I();
@@ -433,6 +434,60 @@ std::string edit_E(double v, int w, int d)
 }
 
 }  // namespace fortran
+
+// ---- VTK dumps (visit.f90:20-112) -----------------------------------------
+//
+// At every visit_frequency-th step, and again when the run completes, the
+// reference appends the chunk's file name to tea.visit (started with an
+// "!NBLOCKS" line on the first call) and writes tea.<chunk>.<step>.vtk: a
+// rectilinear-grid header, the vertex coordinates in E12.4 and the headers
+// of two fields whose values it never writes.  The 2D reference leaves the
+// third DIMENSIONS entry uninitialised (tealeaf_2d/visit.f90:95); 1 is
+// written there.  Files go to the working directory, as the reference's do.
+std::string visit_tag(int n)
+{
+    std::string t = fortran::edit_I(n + 100000, 6);  // i6, the leading digit replaced by '.'
+    t[0] = '.';
+    return t;
+}
+
+void write_visit(const Deck& d, double dx, double dy, double dz, int step, bool& first_call)
+{
+    if (first_call) {
+        FILE* f = std::fopen("tea.visit", "w");
+        if (!f) throw Error("cannot write tea.visit");
+        std::fprintf(f, "!NBLOCKS %s\n", fortran::edit_I(1, 5).c_str());
+        std::fclose(f);
+        first_call = false;
+    }
+    const std::string name = "tea" + visit_tag(1) + visit_tag(step) + ".vtk";
+    FILE* f = std::fopen("tea.visit", "a");
+    if (!f) throw Error("cannot append to tea.visit");
+    std::fprintf(f, "%s\n", name.c_str());
+    std::fclose(f);
+    const long nxv = d.x_cells + 1, nyv = d.y_cells + 1, nzv = TL_DIM == 3 ? d.z_cells + 1 : 1;
+    std::string t = "# vtk DataFile Version 3.0\nvtk output\nASCII\nDATASET RECTILINEAR_GRID\n";
+    t += "DIMENSIONS" + fortran::edit_I(nxv, 12) + fortran::edit_I(nyv, 12) + fortran::edit_I(nzv, 12) + "\n";
+    auto axis = [&](const char* label, long n, double lo, double h) {
+        t += std::string(label) + fortran::edit_I(n, 5) + " double\n";
+        for (long j = 0; j < n; ++j) t += fortran::edit_E(lo + h * (double)j, 12, 4) + "\n";  // vertexx(j)
+    };
+    axis("X_COORDINATES ", nxv, d.xmin, dx);
+    axis("Y_COORDINATES ", nyv, d.ymin, dy);
+#if TL_DIM == 3
+    axis("Z_COORDINATES ", nzv, d.zmin, dz);
+#else
+    (void)dz;
+#endif
+    const long nv = nxv * nyv * nzv;
+    t += "FIELD FieldData 3\n";
+    t += "x_vel 1 " + fortran::edit_I(nv, 20) + " double\n";
+    t += "y_vel 1 " + fortran::edit_I(nv, 20) + " double\n";
+    FILE* v = std::fopen(name.c_str(), "w");
+    if (!v) throw Error("cannot write " + name);
+    std::fwrite(t.data(), 1, t.size(), v);
+    std::fclose(v);
+}
 
 // One summary in tea.out (field_summary.f90:44-45,81): the column heads and
 // the ' step:' line; the list-directed 'Time' line is left out.
@@ -841,6 +896,7 @@ std::string run_deck(const std::string& text, int max_steps)
     double time = 0.0, dt = d.dtinit, total = 0.0;
     const double g_small = (double)1.0e-16f;
     int step = 0;
+    bool visit_first = true;
     for (;;) {
         ++step;
         c->timer.on = kernel_timing_from() > 0 && step >= kernel_timing_from();
@@ -867,7 +923,10 @@ std::string run_deck(const std::string& text, int max_steps)
         else
             o.f("\"step\": %d, \"time\": %.17g", step, time);
         o.f("}");
-        if (time + g_small > d.end_time || step >= d.end_step || (max_steps > 0 && step >= max_steps)) break;
+        if (d.visit_frequency != 0 && step % d.visit_frequency == 0) write_visit(d, dx, dy, dz, step, visit_first);
+        const bool complete = time + g_small > d.end_time || step >= d.end_step;
+        if (complete && d.visit_frequency != 0) write_visit(d, dx, dy, dz, step, visit_first);  // diffuse.f90:93
+        if (complete || (max_steps > 0 && step >= max_steps)) break;
     }
     o.f("], \"final\": {");
     tea_out_qa(tea_out, d.test_problem, summary(o, tea_out, *c, step, time));

@@ -264,3 +264,36 @@ def test_tea_out_qa_verdict(hip):
     qa = _num(lines[-2][len("Test problem  11 is within"):len("Test problem  11 is within") + 16])
     assert qa < 0.001
     assert lines[-1] == " This test is considered PASSED"
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_visit_files(hip, tmp_path, monkeypatch, dim):
+    """visit_frequency (diffuse.f90:62-63,93; visit.f90): tea.visit starts
+    with '!NBLOCKS     1' and lists tea.00001.<step>.vtk for every
+    visit_frequency-th step and again for the last one; each file is the
+    reference's header-only rectilinear grid with the vertex coordinates in
+    E12.4 (no field values: the reference writes none)."""
+    import decks
+    monkeypatch.chdir(tmp_path)
+    if dim == 2:
+        n, nz = 16, 1
+        deck = decks.two_rect_2d(n, "cg", steps=5, extra=("visit_frequency=2",))
+        visits = [2, 4, 5]
+    else:
+        n, nz = 8, 8
+        deck = decks.cube_3d(n, "cg", steps=2, extra=("visit_frequency=1",))
+        visits = [1, 2, 2]
+    hip.run_deck(deck, dim=dim)
+    listed = (tmp_path / "tea.visit").read_text().splitlines()
+    assert listed == ["!NBLOCKS     1"] + [f"tea.00001.{s:05d}.vtk" for s in visits]
+    lines = (tmp_path / f"tea.00001.{visits[-1]:05d}.vtk").read_text().splitlines()
+    nv = n + 1
+    exp = ["# vtk DataFile Version 3.0", "vtk output", "ASCII", "DATASET RECTILINEAR_GRID",
+           f"DIMENSIONS{nv:12d}{nv:12d}{(nz + 1) if dim == 3 else 1:12d}"]
+    dx = 10.0 / n
+    for axis in ("X", "Y") + (("Z",) if dim == 3 else ()):
+        exp.append(f"{axis}_COORDINATES {nv:5d} double")
+        exp += [hip.fortran_edit("e12.4", dx * j, dim) for j in range(nv)]
+    total = nv * nv * (nz + 1 if dim == 3 else 1)
+    exp += ["FIELD FieldData 3", f"x_vel 1 {total:20d} double", f"y_vel 1 {total:20d} double"]
+    assert lines == exp
