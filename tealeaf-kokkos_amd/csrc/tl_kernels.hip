@@ -843,10 +843,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
                                                        const double* __restrict__ rin = nullptr)
 {
     const int lane = threadIdx.x % WAVE;
-    const long wv = (long)blockIdx.x * WAVES_PER_BLOCK + threadIdx.x / WAVE;
+    // the XCD-aware order is taken per workgroup (workgroups, not waves, go
+    // round-robin over the XCDs); a workgroup's waves take consecutive
+    // tiles, adjacent column ranges of one strip row, so the column beyond
+    // each wave's edge is its neighbour's and stays in that XCD's L2
+    const long nblk = (t.nwaves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    const long tw = xcd_tile(blockIdx.x, nblk) * WAVES_PER_BLOCK + threadIdx.x / WAVE;
     double acc = 0.0, acc_pr = 0.0;
-    if (wv < t.nwaves) {
-        const long tw = xcd_tile(wv, t.nwaves);
+    if (tw < t.nwaves) {
         const int tx = (int)(tw % t.tiles_x), ty = (int)(tw / t.tiles_x);
         const int nx = g.nx, ny = g.ny;
         const int c0 = HALO + tx * 2 * WAVE + 2 * lane, c1 = c0 + 1;
