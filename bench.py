@@ -494,7 +494,8 @@ def line(value_s: float, args, workload: str, cells: int, **extra) -> dict:
         "metric": METRIC, "value": round(value_s, 6), "unit": "s/timestep", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * value_s, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": workload, "cells": cells, "parallelism": "single chunk, one GPU"},
+        "config": {"workload": workload, "cells": cells, "parallelism": "single chunk, one GPU",
+                   "reductions": "single pass" if args.single_pass else "separate finish launch"},
     }
     out.update(extra)
     return out
@@ -590,6 +591,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: physical cores, "
                                                                  "capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--single-pass", action="store_true",
+                    help="close every grid reduction inside its sweep (default: a separate one-workgroup launch)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the plain copy kernel (counter calibration)")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -605,6 +608,9 @@ def main() -> int:
                     help="only run the c4 deck capped at 100 iterations (counter runs)")
     args = ap.parse_args()
     n = args.n or {"c2": 4096, "c3": 256, "c4": 8192, "c5": 4096}[args.workload]
+    if args.single_pass:
+        for d in (2, 3):
+            tealeaf_hip.lib(d).tl_set_single_pass(1)
     if args.steps is None:
         args.steps = 3 if args.workload == "c4" else 9
 

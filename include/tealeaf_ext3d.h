@@ -139,6 +139,15 @@ void tl_set_kernel_timing(int from_step);
 int tl_set_kernel_timing_only(const char* name);
 /* A progress line on stderr after every timestep of tl_run_deck (0 = off). */
 void tl_set_progress(int on);
+/* How chunks created after the call close their grid reductions: on != 0,
+ * inside the sweep that computes the sum (single pass: the last workgroup
+ * to finish adds the partials and applies the solver recurrence); 0 (the
+ * default), with a separate one-workgroup launch after it, which measures
+ * faster on MI355X.  Both add the partials in the same order and give the
+ * same bits.  No counterpart in the reference (its Kokkos CUDA backend
+ * closes reductions in the last block,
+ * kokkos_src/core/src/Cuda/Kokkos_Cuda_ReduceScan.hpp:331-403). */
+void tl_set_single_pass(int on);
 
 /* The fused PPCG outer iteration the driver uses (2D, no preconditioner),
  * on a chunk made by ext_init_: with the given alpha, CGCalcUr + PPCGInitSd
@@ -192,6 +201,18 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
  * times: the attainable streaming rate, and the known-size kernel that
  * calibrates HBM byte counters.  Negative on error. */
 double tl_copy_probe(long n, int reps);
+
+/* The grid reduction of every sweep, on given values: workgroup b of a
+ * 1D grid of nblocks workgroups of nwaves waves (1 or 4) contributes x[b]
+ * as its partial.  out[0] and out[1]: two single-pass reductions in a row
+ * (the last workgroup to finish adds the partials, a device-wide counter
+ * finds it and is left at 0 for the next); out[2]: the same partials added
+ * by a separate one-workgroup launch.  The three are bitwise equal by
+ * construction.  No counterpart in the reference (its Kokkos CUDA backend
+ * closes reductions in the last block,
+ * kokkos_src/core/src/Cuda/Kokkos_Cuda_ReduceScan.hpp:331-403).  0, or -1
+ * with tl_last_error() set. */
+int tl_reduce_probe(long nblocks, int nwaves, const double* x, double* out);
 
 /* GB/s of a streaming probe over arrays of n doubles, repeated reps times;
  * one 16-byte element per thread unless stated.  kind 0: the plain copy of

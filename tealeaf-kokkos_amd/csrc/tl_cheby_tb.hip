@@ -18,7 +18,7 @@ bool cheby_tb_supported(const Chunk& c)
 // Iterations step1 .. step1 + count - 1 (ext_cheby_solver_iterate_,
 // ext_core.cpp:501-515, each), as passes of 10, 5, 2 and 1 iterations
 // that hand u and p over through memory.
-long op_cheby_tb(Chunk& c, int step1, int count, bool norm)
+void op_cheby_tb(Chunk& c, int step1, int count, bool norm)
 {
     if (!cheby_tb_supported(c)) throw Error("temporal-blocked Chebyshev is 2D, no preconditioner");
     if (count < 1) throw Error("temporal-blocked Chebyshev needs at least one iteration");
@@ -26,7 +26,6 @@ long op_cheby_tb(Chunk& c, int step1, int count, bool norm)
         throw Error("Chebyshev steps " + std::to_string(step1) + ".." + std::to_string(step1 + count - 1) +
                     " outside the stored coefficients");
     const std::vector<int> passes = tb_passes(count);
-    long nblocks = 0;
     int step = step1;
     for (size_t q = 0; q < passes.size(); ++q) {
         const int K = passes[q];
@@ -44,7 +43,7 @@ long op_cheby_tb(Chunk& c, int step1, int count, bool norm)
         a.cb = c.d_cheby + c.d_cheby_n + (step - 1);
         a.rowtab = tb_rowtab(c);
         a.row0 = TB_ROWTAB0;
-        a.partials = c.partials;
+        a.red = last && norm ? c.reducer(S_NORM, EP_STORE) : c.reducer();
         const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
         a.tiles_x = (c.g.nx + ow - 1) / ow;
         a.rows = tb_rows(c.g.ny, a.tiles_x, K, c.simds, tb::TB_UNROLL);
@@ -58,19 +57,17 @@ long op_cheby_tb(Chunk& c, int step1, int count, bool norm)
             default: ctb::ctb_launch_1(c, a, last); break;
             }
         });
+        if (last && norm) c.close(a.nblocks, 1);
         std::swap(c.f[F_U], c.u_alt);
         std::swap(c.f[F_P], c.p_alt);
         step += K;
-        nblocks = a.nblocks;
     }
-    (void)norm;
-    return nblocks;
 }
 
 #else  // the 7-point stencil runs one iteration per launch
 
 bool cheby_tb_supported(const Chunk&) { return false; }
-long op_cheby_tb(Chunk&, int, int, bool) { throw Error("temporal-blocked Chebyshev is implemented for 2D"); }
+void op_cheby_tb(Chunk&, int, int, bool) { throw Error("temporal-blocked Chebyshev is implemented for 2D"); }
 
 #endif
 

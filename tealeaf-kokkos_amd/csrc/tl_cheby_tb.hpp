@@ -47,7 +47,7 @@ struct CTBArgs {
     const double* cb;
     const uint2* rowtab;  // reflected row byte offsets (cell row, face row)
     int row0;
-    double* partials;
+    Reduce red;  // the last pass: ||r||^2 (closed when the driver checks it)
     int tiles_x;
     int rows;
     long nblocks;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void ctb_wave(const CTBArgs& A, const CTBCtx& c)
     for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL) ctb_rounds<K, LAST, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
     if (LAST) {
         double v[1] = {st.acc};
-        block_partial<1, 1>(v, Reduce{A.partials});
+        block_partial<1, 1>(v, A.red);
     }
 }
 

@@ -86,6 +86,10 @@ int kernel_timing_from();
 void set_kernel_timing_only(int kid);
 int kernel_timing_only();
 int kernel_id(const char* name);  // -1 for an unknown name
+// Whether chunks created from now on close their reductions inside the
+// sweeps (single pass) or with a k_finish launch (the default: faster here).
+void set_single_pass(bool on);
+bool single_pass_default();
 
 // Runs STMT between two timer events when the chunk's timer is on.
 #define TL_TIMED(c, kid, STMT)                               \
@@ -170,7 +174,22 @@ struct Chunk {
     Chunk(const Chunk&) = delete;
     Chunk& operator=(const Chunk&) = delete;
 
-    Reduce reducer() const { return Reduce{partials}; }
+    // Single-pass reductions (single_pass): reduce_counters(partials_cap)
+    // counters, 0 between reductions, and 4 * partials_cap slots, SLOT_EMPTY
+    // between them.  Otherwise each reduction is closed by a k_finish launch.
+    unsigned* d_counter = nullptr;
+    double* d_slots = nullptr;
+    bool single_pass = false;
+    Finish pending{};  // the reduction close() launches k_finish for
+    bool pending_open = false;
+
+    // Partials only: an explicit finish closes the reduction.
+    Reduce reducer() const { return Reduce{partials, Finish{}}; }
+    // The next reduction of `slot`, with epilogue `ep`: closed inside the
+    // sweep that takes it (single_pass), else by close() after its launch.
+    Reduce reducer(Slot slot, int ep, int hist_n = -1);
+    Finish finish_args(Slot slot, int ep, int hist_n = -1);
+    void close(long nblocks, int nv);  // launches k_finish unless the sweep closed the reduction
     double result(Slot s, int k = 0) const;  // waits for the slot's last reduction
     void ensure_history(int n);
     void sync() const;
@@ -221,8 +240,11 @@ void op_cg_calc_w_faces_dev(Chunk& c);
 void op_ppcg_calc_ur_dev(Chunk& c);  // u += alpha p, r -= alpha w; its r.r is not used
 void op_2norm_rrn_dev(Chunk& c);  // rrn = ||r||^2, then beta, error, rro as in the PPCG loop
 double op_error(Chunk& c);        // waits and returns ds->error
-void op_finish_rrn(Chunk& c, long nblocks);  // second phase of an rrn reduction, PPCG recurrences
 double stream_copy_probe(long n, int reps);   // GB/s of a plain copy of n doubles
+// Sum of x[0..nblocks) through the grid reduction with workgroups of
+// nwaves (1 or 4) waves: out[0], out[1] single pass twice in a row, out[2]
+// with the separate k_finish launch.
+void reduce_probe(long nblocks, int nwaves, const double* x, double* out);
 // GB/s of a streaming probe over arrays of n doubles (kinds in tl_stream_probe)
 double stream_probe(int kind, long n, int reps);
 // w = A p and *pw = sum w.p on caller device arrays in the reference's flat
@@ -231,8 +253,8 @@ void cg_calc_w_noalloc(int nx, int ny, int nz, const double* p, const double* kx
                        const double* kz, double* w, double* pw, hipStream_t stream);
 
 // Temporal-blocked PPCG outer iteration (tl_ppcg_tb.hip): after CGCalcW,
-// CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep.
-// Returns the number of rrn partials; op_finish_rrn closes the reduction.
+// CGCalcUr + PPCGInitSd + inner_steps inner steps + ||r||^2 in one sweep,
+// which also closes the rrn reduction (PPCG recurrences, rrn in S_RRN).
 // Output rows per wave of a streaming pass with L levels over tiles_x
 // column tiles, for `slots` resident waves and blocks of `unroll` rounds.
 int tb_rows(int ny, int tiles_x, int L, int slots, int unroll = 6);
@@ -240,7 +262,7 @@ int tb_rows(int ny, int tiles_x, int L, int slots, int unroll = 6);
 // with the given inner steps (each a compiled pass size).
 std::vector<int> tb_passes(int inner_steps);
 bool ppcg_tb_supported(const Chunk& c, int inner_steps);
-long op_ppcg_outer_tb(Chunk& c, int inner_steps);
+void op_ppcg_outer_tb(Chunk& c, int inner_steps);
 // First row of the reflected-row table of the temporal-blocked kernels
 // (2D): entry y - TB_ROWTAB0 holds the byte offsets of the cell row and the
 // face row that row y reads.
@@ -249,13 +271,12 @@ constexpr int TB_ROWTAB0 = HALO - 14;
 const uint2* tb_rowtab(Chunk& c);
 #endif
 // Temporal-blocked Chebyshev (2D, no preconditioner): `count` iterations
-// from the 1-based step `step1` in passes of at most 10; with `norm` the
-// last writes r and leaves the first phase of ||r||^2 (returns its block
-// count, closed by op_finish_norm).
+// from the 1-based step `step1` in passes of at most 10; the last writes r
+// and, with `norm`, closes ||r||^2 into S_NORM.
 bool cheby_tb_supported(const Chunk& c);
-long op_cheby_tb(Chunk& c, int step1, int count, bool norm);
-double op_finish_norm(Chunk& c, long nblocks);
-// The same for three sums per block (out[0..3)).
+void op_cheby_tb(Chunk& c, int step1, int count, bool norm);
+// Closes a reduction of three sums per block that an earlier pass left in
+// the partials (out[0..3)).
 void op_finish_sums3(Chunk& c, long nblocks, double* out);
 // Temporal-blocked Jacobi (2D): one pass of K (1, 2, 5 or 10) iterations,
 // K = jacobi_tb_pass(iterations left); returns each iteration's error sum;
@@ -347,7 +368,7 @@ UNet& learn_net(Learner& L);
 // op_ppcg_init; the outer op mirrors op_ppcg_outer_tb.
 bool ppcg_mixed_supported(const Chunk& c, int inner_steps);
 void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int inner_steps, double theta);
-long op_ppcg_outer_mixed(Chunk& c, int inner_steps);
+void op_ppcg_outer_mixed(Chunk& c, int inner_steps);
 // rrn = ||r||^2 of a replaced residual; beta from the rro before this iteration
 void op_2norm_replace_dev(Chunk& c);
 void op_read_history(Chunk& c, int n, std::vector<double>& a, std::vector<double>& b);  // 1..n

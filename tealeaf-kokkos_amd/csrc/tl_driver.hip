@@ -673,8 +673,8 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     int m = n;
                     while (m < max_iters && !((m >= est_itc) && (m % 10 == 0))) ++m;
                     const bool check = (m >= est_itc) && (m % 10 == 0);
-                    const long nb = op_cheby_tb(c, cheby_calc_steps, m - n + 1, check);
-                    if (check) error = op_finish_norm(c, nb);
+                    op_cheby_tb(c, cheby_calc_steps, m - n + 1, check);
+                    if (check) error = c.result(S_NORM);
                     cheby_calc_steps += m - n;
                     n = m;
                 } else {
@@ -697,11 +697,11 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                 // rro on the device is the last rrn (= error), as at tea_solve.f90:187,304
                 op_cg_calc_w_dev(c, p_pending);
                 if (mixed) {
-                    TL_TIMED(c, K_PPCG_MIXED, op_finish_rrn(c, op_ppcg_outer_mixed(c, d.ppcg_inner_steps)));
+                    TL_TIMED(c, K_PPCG_MIXED, op_ppcg_outer_mixed(c, d.ppcg_inner_steps));
                     ++si.mixed_steps;
                 } else if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
-                    TL_TIMED(c, K_PPCG_OUTER, op_finish_rrn(c, op_ppcg_outer_tb(c, d.ppcg_inner_steps)));
+                    TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps));
                 } else {
                     op_ppcg_calc_ur_dev(c);
                     // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
@@ -883,6 +883,7 @@ std::string run_deck(const std::string& text, int max_steps)
     Json o;
     o.f("{\"dim\": %d, \"x_cells\": %d, \"y_cells\": %d, \"z_cells\": %d, \"device\": \"hip\", ", TL_DIM, d.x_cells,
         d.y_cells, nz);
+    o.f("\"reductions\": \"%s\", ", c->single_pass ? "single-pass" : "separate finish");
     o.f("\"solver\": \"%s\", \"eps\": %.17g, \"dtinit\": %.17g, \"max_iters\": %d, ",
         d.use_cg ? "cg" : d.use_cheby ? "chebyshev" : d.use_ppcg ? (d.use_fcg ? "fcg_learned" : d.ppcg_mixed ? "ppcg_mixed" : "ppcg") : "jacobi",
         d.eps, d.dtinit, d.max_iters);
@@ -1030,5 +1031,10 @@ int tl_set_kernel_timing_only(const char* name)
 
 // A line on stderr after every timestep of tl_run_deck (0 turns it off).
 void tl_set_progress(int on) { tl::g_progress = on != 0; }
+
+// Chunks created after this call close every reduction inside the sweep
+// that computes it (on != 0) or with a separate one-workgroup launch (0, the
+// default); both give the same bits.
+void tl_set_single_pass(int on) { tl::set_single_pass(on != 0); }
 
 }  // extern "C"

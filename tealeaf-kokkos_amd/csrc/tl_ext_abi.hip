@@ -120,7 +120,7 @@ int tl_ppcg_outer_fused(const int* chunk, double alpha, double theta, const doub
         s.alpha = alpha;
         TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
         op_ppcg_init(*c, alphas, betas, nsteps, theta);
-        op_finish_rrn(*c, op_ppcg_outer_tb(*c, nsteps));
+        op_ppcg_outer_tb(*c, nsteps);
         *rrn = op_error(*c);
         rc = 0;
     });
@@ -150,7 +150,8 @@ int tl_cheby_iterate_fused(const int* chunk, int step1, int count, double* rrn)
     guarded([&] {
         Chunk* c = stencil_chunk(chunk);
         op_cheby_upload(*c);
-        *rrn = op_finish_norm(*c, op_cheby_tb(*c, step1, count, true));
+        op_cheby_tb(*c, step1, count, true);
+        *rrn = c->result(S_NORM);
         rc = 0;
     });
     return rc;
@@ -229,7 +230,7 @@ int tl_ppcg_outer_mixed(const int* chunk, double alpha, double theta, const doub
         TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
         op_ppcg_init(*c, alphas, betas, nsteps, theta);
         op_mixed_prepare(*c, alphas, betas, nsteps, theta);
-        op_finish_rrn(*c, op_ppcg_outer_mixed(*c, nsteps));
+        op_ppcg_outer_mixed(*c, nsteps);
         *rrn = op_error(*c);
         rc = 0;
     });
@@ -438,6 +439,18 @@ double tl_copy_probe(long n, int reps)
     double gbs = -1.0;
     guarded([&] { gbs = stream_copy_probe(n, reps); });
     return gbs;
+}
+
+// The grid reduction on given partials (see the header); 0, or -1 on error.
+int tl_reduce_probe(long nblocks, int nwaves, const double* x, double* out)
+{
+    int rc = -1;
+    guarded([&] {
+        if (!x || !out) throw Error("reduce probe: null pointer");
+        reduce_probe(nblocks, nwaves, x, out);
+        rc = 0;
+    });
+    return rc;
 }
 
 // GB/s of a streaming probe over arrays of n doubles (see the header).

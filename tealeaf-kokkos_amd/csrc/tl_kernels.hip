@@ -1150,109 +1150,18 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w_faces(Geom g, FaceOffsets o
     block_partial<1>(v, red);
 }
 
-// Second phase of every reduction: one workgroup adds the block partials in
-// block order and applies the scalar recurrence of tea_solve.f90 that
-// consumes the total, so the next kernel can read it from `ds`.
-enum Epilogue {
-    EP_STORE = 0, EP_RRO, EP_PW, EP_RRN, EP_RRN_REPLACE, EP_FCG_BETA, EP_FCG_PW, EP_FCG_ALPHA, EP_FCG_PWPR,
-    EP_FCG_RRN, EP_FCG_BETA_RRN, EP_FCG_PW_ZR
-};
-
+// Last phase of a reduction whose partials a previous launch left: one
+// workgroup adds them in block order and applies the epilogue (the sweeps
+// that close their own reduction run the same code in their last
+// workgroup).
 template <int NV>
-__global__ __launch_bounds__(BLOCK) void k_finish(const double* __restrict__ partials, long nblocks, int ep,
-                                                  DevScalars* __restrict__ ds, double* __restrict__ dout,
-                                                  double* __restrict__ hout, double* __restrict__ hist_a,
-                                                  double* __restrict__ hist_b, int hist_n, unsigned* hseq,
-                                                  unsigned seq)
+__global__ __launch_bounds__(BLOCK) void k_finish(double* __restrict__ partials, long nblocks, Finish f)
 {
+    DevScalars cur{};
+    if (threadIdx.x == 0 && f.ds) cur = *f.ds;
     double v[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-    for (long b = threadIdx.x; b < nblocks; b += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) v[k] += partials[b * NV + k];
-    }
-    block_fold<NV>(v);
-    if (threadIdx.x != 0) return;
-    double out[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = v[k];
-    switch (ep) {
-    case EP_RRO:  // rro of ext_cg_solver_init
-        ds->rro = v[0];
-        break;
-    case EP_PW: {  // alpha = rro / pw (tea_solve.f90:322,397)
-        const double alpha = ds->rro / v[0];
-        ds->pw = v[0];
-        ds->alpha = alpha;
-        out[1] = alpha;
-        break;
-    }
-    case EP_RRN: {  // beta = rrn / rro; error = rrn; rro = rrn (:358,:373-374,:417,:433-434)
-        const double beta = v[0] / ds->rro;
-        ds->rrn = v[0];
-        ds->beta = beta;
-        if (hist_n >= 0) {
-            hist_a[hist_n] = ds->alpha;
-            hist_b[hist_n] = beta;
-        }
-        ds->rro_prev = ds->rro;
-        ds->rro = v[0];
-        ds->error = v[0];
-        out[1] = beta;
-        break;
-    }
-    case EP_FCG_BETA:  // flexible CG: beta = -(z . w_prev) / (p . w)_prev
-        ds->beta = -v[0] / ds->pw;
-        break;
-    case EP_FCG_PW:  // flexible CG: p . w
-        ds->pw = v[0];
-        break;
-    case EP_FCG_ALPHA:  // flexible CG: alpha = (p . r) / (p . w); a pw <= 0 or non-finite p.r stops the solve
-        ds->alpha = v[0] / ds->pw;
-        ds->bad = (ds->pw > 0.0 && isfinite(v[0])) ? 0.0 : 1.0;
-        break;
-    case EP_FCG_PWPR:  // flexible CG: p . w and p . r in one reduction -> alpha
-        ds->pw = v[0];
-        ds->alpha = v[NV - 1] / v[0];
-        ds->bad = (v[0] > 0.0 && isfinite(v[NV - 1])) ? 0.0 : 1.0;
-        break;
-    case EP_FCG_RRN:  // flexible CG: rrn, and the stop flag of this iteration's alpha
-        out[1] = ds->bad;
-        break;
-    case EP_FCG_BETA_RRN:  // flexible CG: beta and z . r; the previous iteration's rrn and stop flag
-        ds->beta = -v[0] / ds->pw;
-        ds->zr = v[NV > 2 ? 2 : 0];
-        out[0] = v[NV > 1 ? 1 : 0];
-        out[1] = ds->bad;
-        break;
-    case EP_FCG_PW_ZR:  // flexible CG, sweeps only: alpha = (z . r) / (p . w)
-        // z . r > 0 for the SPD sweep preconditioner and r != 0: anything
-        // else is a breakdown and stops the solve
-        ds->pw = v[0];
-        ds->alpha = ds->zr / v[0];
-        ds->bad = (v[0] > 0.0 && ds->zr > 0.0 && isfinite(ds->zr)) ? 0.0 : 1.0;
-        break;
-    case EP_RRN_REPLACE: {  // rrn of a replaced residual: beta from the rro before this iteration's rrn
-        const double beta = v[0] / ds->rro_prev;
-        ds->rrn = v[0];
-        ds->beta = beta;
-        ds->rro = v[0];
-        ds->error = v[0];
-        out[1] = beta;
-        break;
-    }
-    default:
-        break;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        dout[k] = out[k];
-        hout[k] = out[k];
-    }
-    // the result reaches host memory before the number the host waits for
-    __threadfence_system();
-    *reinterpret_cast<volatile unsigned*>(hseq) = seq;
+    sum_partials<NV, WAVES_PER_BLOCK>(partials, nblocks, v);
+    if (threadIdx.x == 0) finish_scalars<NV, false>(v, f, cur);
 }
 
 // ----------------------------------------------------------------------------
@@ -1550,14 +1459,23 @@ __global__ __launch_bounds__(256) void k_probe_mix(ProbeStreams s, long n2)
     s.w[1][i] = NR > 4 ? v[3] + v[4] * v[5] : v[3] + v[1];
 }
 
-// Second phase of a reduction whose first phase wrote `nblocks` partials.
+// Probe of the grid reduction: workgroup b contributes x[b] as its partial
+// (thread 0 holds it, the other lanes 0), then the reduction closes in the
+// last workgroup (single pass) or waits for k_finish.
+template <int NWAVES>
+__global__ __launch_bounds__(NWAVES * WAVE) void k_reduce_probe(const double* __restrict__ x, Reduce red)
+{
+    double v[1] = {threadIdx.x == 0 ? x[blockIdx.x] : 0.0};
+    block_partial<1, NWAVES>(v, red);
+}
+
+// Last phase of a reduction whose first phase wrote `nblocks` partials.
 template <int NV>
 void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
 {
-    const unsigned seq = ++c.fin_seq[slot];
-    hipLaunchKernelGGL(k_finish<NV>, dim3(1), dim3(BLOCK), 0, c.stream, c.partials, nblocks, ep, c.ds,
-                       c.d_result + 4 * slot, c.h_result_dev + 4 * slot, c.hist_a, c.hist_b, hist_n,
-                       c.h_seq_dev + slot, seq);
+    Finish f = c.finish_args(slot, ep, hist_n);
+    f.counter = nullptr;
+    hipLaunchKernelGGL(k_finish<NV>, dim3(1), dim3(BLOCK), 0, c.stream, c.partials, nblocks, f);
     check_launch();
 }
 
@@ -1565,6 +1483,10 @@ void finish(Chunk& c, long nblocks, Slot slot, int ep, int hist_n = -1)
 
 // ============================================================================
 // Kernel timer
+
+static bool g_single_pass = false;
+void set_single_pass(bool on) { g_single_pass = on; }
+bool single_pass_default() { return g_single_pass; }
 
 static int g_timing_from = 0;
 static int g_timing_only = -1;
@@ -1753,6 +1675,11 @@ Chunk::Chunk(int nx, int ny, int nz, int rank_) : rank(rank_)
     if (cus > 0) simds = 4 * cus;
     TL_HIP(hipMalloc(&ds, sizeof(DevScalars)));
     TL_HIP(hipMemsetAsync(ds, 0, sizeof(DevScalars), stream));
+    single_pass = single_pass_default();
+    TL_HIP(hipMalloc(&d_counter, sizeof(unsigned) * reduce_counters(partials_cap)));
+    TL_HIP(hipMemsetAsync(d_counter, 0, sizeof(unsigned) * reduce_counters(partials_cap), stream));
+    TL_HIP(hipMalloc(&d_slots, sizeof(double) * 4 * (size_t)partials_cap));
+    TL_HIP(hipMemsetAsync(d_slots, 0xff, sizeof(double) * 4 * (size_t)partials_cap, stream));  // SLOT_EMPTY
     TL_HIP(hipMalloc(&d_result, sizeof(double) * 4 * S_COUNT));
     double* h = nullptr;
     TL_HIP(hipHostMalloc(&h, sizeof(double) * 4 * S_COUNT, hipHostMallocMapped | hipHostMallocCoherent));
@@ -1784,6 +1711,8 @@ Chunk::~Chunk()
     (void)hipFree(d_coef32);
     (void)hipFree(d_coef);
     (void)hipFree(ds);
+    (void)hipFree(d_counter);
+    (void)hipFree(d_slots);
     (void)hipFree(hist_a);
     (void)hipFree(hist_b);
     (void)hipFree(vertexX);
@@ -1825,6 +1754,40 @@ void Chunk::sync() const
 // stream synchronisation.  A wait longer than 50 ms, or more than 1024
 // pending timer events, falls back to synchronising the stream (which also
 // reports a failed launch).
+Finish Chunk::finish_args(Slot slot, int ep, int hist_n)
+{
+    const unsigned seq = ++fin_seq[slot];
+    return Finish{d_counter, d_slots, ds, d_result + 4 * slot, h_result_dev + 4 * slot, hist_a, hist_b,
+                  h_seq_dev + slot, ep, hist_n, seq, (unsigned)((partials_cap + REDUCE_GROUP - 1) / REDUCE_GROUP)};
+}
+
+Reduce Chunk::reducer(Slot slot, int ep, int hist_n)
+{
+    Finish f = finish_args(slot, ep, hist_n);
+    pending_open = !single_pass;
+    if (pending_open) {
+        pending = f;
+        f.counter = nullptr;
+    }
+    return Reduce{partials, f};
+}
+
+void Chunk::close(long nblocks, int nv)
+{
+    if (!pending_open) return;
+    pending_open = false;
+    Finish f = pending;
+    f.counter = nullptr;
+    switch (nv) {
+    case 1: hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLOCK), 0, stream, partials, nblocks, f); break;
+    case 2: hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(BLOCK), 0, stream, partials, nblocks, f); break;
+    case 3: hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(BLOCK), 0, stream, partials, nblocks, f); break;
+    case 4: hipLaunchKernelGGL(k_finish<4>, dim3(1), dim3(BLOCK), 0, stream, partials, nblocks, f); break;
+    default: throw Error("reductions carry 1 to 4 sums");
+    }
+    check_launch();
+}
+
 double Chunk::result(Slot s, int k) const
 {
     const unsigned want = fin_seq[s];
@@ -1926,8 +1889,8 @@ void op_set_field(Chunk& c)
 void op_field_summary(Chunk& c, double* vol, double* mass, double* ie, double* temp)
 {
     TIMED(c, K_SUMMARY, STRIP_LAUNCH(k_field_summary, c, c.f[F_VOLUME], c.f[F_DENSITY], c.f[F_ENERGY0], c.f[F_U],
-                                     c.reducer()));
-    finish<4>(c, c.g.nblocks, S_SUMMARY, EP_STORE);
+                                     c.reducer(S_SUMMARY, EP_STORE)));
+    c.close(c.g.nblocks, 4);
     c.sync();
     *vol = c.h_result[4 * S_SUMMARY + 0];
     *mass = c.h_result[4 * S_SUMMARY + 1];
@@ -2069,9 +2032,9 @@ double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt)
         FLAT_LAUNCH(k_cg_init_k, c, coefficient, rx, ry, rz, c.f[F_DENSITY], c.f[F_ENERGY1], c.f[F_U], c.f[F_P],
                     c.f[F_R], c.f[F_W], c.f[F_KX], c.f[F_KY], c.f[F_KZ]);
         STRIP_LAUNCH(k_cg_calc_rro, c, kptr(c), (int)preconditioner, c.f[F_U], c.f[F_W], c.f[F_R], c.f[F_P],
-                     c.f[F_MI], c.f[F_Z], c.reducer());
+                     c.f[F_MI], c.f[F_Z], c.reducer(S_RRO, EP_RRO));
     });
-    finish<1>(c, c.g.nblocks, S_RRO, EP_RRO);
+    c.close(c.g.nblocks, 1);
     return c.result(S_RRO);
 }
 
@@ -2079,8 +2042,8 @@ double op_cg_init(Chunk& c, int coefficient, bool preconditioner, double dt)
 double op_cg_calc_w(Chunk& c)
 {
     TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                  c.reducer(), nullptr));
-    finish<1>(c, c.g.nblocks, S_PW, EP_STORE);
+                                  c.reducer(S_PW, EP_STORE), nullptr));
+    c.close(c.g.nblocks, 1);
     return c.result(S_PW);
 }
 
@@ -2088,8 +2051,8 @@ double op_cg_calc_w(Chunk& c)
 double op_cg_calc_ur(Chunk& c, double alpha)
 {
     TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, alpha, nullptr, (int)c.preconditioner, c.f[F_U], c.f[F_R],
-                                   c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_RRN, EP_STORE);
+                                   c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_STORE)));
+    c.close(c.g.nblocks, 1);
     return c.result(S_RRN);
 }
 
@@ -2110,50 +2073,50 @@ void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u)
         if (fold_u) {
             TIMED(c, K_CG_WPU, {
                 hipLaunchKernelGGL(k_cg_pw_strip<true>, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t, kptr(c),
-                                   c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer());
+                                   c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer(S_PW, EP_PW));
                 check_launch();
             });
         } else {
             TIMED(c, K_CG_WP, {
                 hipLaunchKernelGGL(k_cg_pw_strip<false>, dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
-                                   kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer());
+                                   kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(S_PW, EP_PW));
                 check_launch();
             });
         }
         std::swap(c.f[F_P], c.p_alt);
-        finish<1>(c, nb, S_PW, EP_PW);
+        c.close(nb, 1);
         return;
 #else
         if (fold_u) {
             TIMED(c, K_CG_WPU, STRIP_LAUNCH(k_cg_pw_z<true>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
-                                            c.f[F_U], c.reducer()));
+                                            c.f[F_U], c.reducer(S_PW, EP_PW)));
         } else {
             TIMED(c, K_CG_WP, STRIP_LAUNCH(k_cg_pw_z<false>, c, kptr(c), c.f[F_P], q, c.ds, c.p_alt, c.f[F_W],
-                                          nullptr, c.reducer()));
+                                          nullptr, c.reducer(S_PW, EP_PW)));
         }
         std::swap(c.f[F_P], c.p_alt);
 #endif
     } else {
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                      c.reducer(), nullptr));
+                                      c.reducer(S_PW, EP_PW), nullptr));
     }
-    finish<1>(c, c.g.nblocks, S_PW, EP_PW);
+    c.close(c.g.nblocks, 1);
 }
 
 void op_cg_calc_ur_dev(Chunk& c, int n)
 {
     if (n > 0) c.ensure_history(n);
     TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R], c.f[F_P],
-                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN, n > 0 ? n : -1);
+                                   c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_RRN, n > 0 ? n : -1)));
+    c.close(c.g.nblocks, 1);
 }
 
 void op_cg_calc_r_dev(Chunk& c, int n)
 {
     if (n > 0) c.ensure_history(n);
     TIMED(c, K_CG_R, STRIP_LAUNCH(k_cg_calc_ur<false>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R],
-                                  c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN, n > 0 ? n : -1);
+                                  c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_RRN, n > 0 ? n : -1)));
+    c.close(c.g.nblocks, 1);
 }
 
 void op_cg_calc_u_dev(Chunk& c)
@@ -2186,25 +2149,17 @@ void op_cg_calc_w_faces_dev(Chunk& c)
 {
     if (!c.p_faces) throw Error("no boundary faces of p were captured");
     TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w_faces, c, face_offsets(c.g), kptr(c), c.f[F_P], c.p_faces, c.f[F_W],
-                                  c.reducer()));
-    finish<1>(c, c.g.nblocks, S_PW, EP_PW);
+                                  c.reducer(S_PW, EP_PW)));
+    c.close(c.g.nblocks, 1);
 }
 
 void op_2norm_rrn_dev(Chunk& c)
 {
-    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN);
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer(S_RRN, EP_RRN)));
+    c.close(c.g.nblocks, 1);
 }
 
 double op_error(Chunk& c) { return c.result(S_RRN); }
-
-void op_finish_rrn(Chunk& c, long nblocks) { finish<1>(c, nblocks, S_RRN, EP_RRN); }
-
-double op_finish_norm(Chunk& c, long nblocks)
-{
-    finish<1>(c, nblocks, S_NORM, EP_STORE);
-    return c.result(S_NORM);
-}
 
 void op_finish_sums3(Chunk& c, long nblocks, double* out)
 {
@@ -2214,8 +2169,73 @@ void op_finish_sums3(Chunk& c, long nblocks, double* out)
 
 void op_2norm_replace_dev(Chunk& c)
 {
-    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_RRN, EP_RRN_REPLACE);
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, c.f[F_R], c.reducer(S_RRN, EP_RRN_REPLACE)));
+    c.close(c.g.nblocks, 1);
+}
+
+void reduce_probe(long nblocks, int nwaves, const double* x, double* out)
+{
+    if (nblocks < 1 || nblocks > (1L << 22)) throw Error("reduce probe: 1 to 2^22 workgroups");
+    if (nwaves != 1 && nwaves != WAVES_PER_BLOCK) throw Error("reduce probe: workgroups of 1 or 4 waves");
+    hipStream_t s = nullptr;
+    double *dx = nullptr, *part = nullptr, *slots = nullptr, *res = nullptr;
+    unsigned *counter = nullptr, *seq = nullptr;
+    const long ncount = reduce_counters(nblocks);
+    auto release = [&] {
+        (void)hipFree(dx);
+        (void)hipFree(part);
+        (void)hipFree(slots);
+        (void)hipFree(res);
+        (void)hipFree(counter);
+        (void)hipFree(seq);
+    };
+    try {
+        TL_HIP(hipMalloc(&dx, sizeof(double) * nblocks));
+        TL_HIP(hipMalloc(&part, sizeof(double) * nblocks));
+        TL_HIP(hipMalloc(&slots, sizeof(double) * nblocks));
+        TL_HIP(hipMalloc(&res, sizeof(double) * 12));
+        TL_HIP(hipMalloc(&counter, sizeof(unsigned) * ncount));
+        TL_HIP(hipMalloc(&seq, sizeof(unsigned)));
+        TL_HIP(hipMemcpy(dx, x, sizeof(double) * nblocks, hipMemcpyHostToDevice));
+        TL_HIP(hipMemset(slots, 0xff, sizeof(double) * nblocks));  // SLOT_EMPTY
+        TL_HIP(hipMemset(counter, 0, sizeof(unsigned) * ncount));
+        TL_HIP(hipMemset(res, 0, sizeof(double) * 12));
+        // three reductions in a row: single pass twice (counters and slots
+        // must be back at their empty state after the first), then the
+        // separate k_finish launch
+        for (int k = 0; k < 3; ++k) {
+            Reduce red{part, Finish{k < 2 ? counter : nullptr, slots, nullptr, res + 4 * k, res + 4 * k, nullptr,
+                                    nullptr, seq, EP_STORE, -1, (unsigned)k + 1,
+                                    (unsigned)((nblocks + REDUCE_GROUP - 1) / REDUCE_GROUP)}};
+            if (nwaves == 1)
+                hipLaunchKernelGGL(k_reduce_probe<1>, dim3((unsigned)nblocks), dim3(WAVE), 0, s, dx, red);
+            else
+                hipLaunchKernelGGL(k_reduce_probe<WAVES_PER_BLOCK>, dim3((unsigned)nblocks), dim3(BLOCK), 0, s, dx,
+                                   red);
+            check_launch();
+            if (k == 2) {
+                hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(BLOCK), 0, s, part, nblocks, red.fin);
+                check_launch();
+            }
+        }
+        double h[12];
+        TL_HIP(hipMemcpy(h, res, sizeof h, hipMemcpyDeviceToHost));
+        std::vector<unsigned> left((size_t)ncount, 1u);
+        TL_HIP(hipMemcpy(left.data(), counter, sizeof(unsigned) * left.size(), hipMemcpyDeviceToHost));
+        for (unsigned v : left)
+            if (v != 0) throw Error("reduce probe: a counter was not reset");
+        std::vector<unsigned long long> empty((size_t)nblocks, 0ull);
+        TL_HIP(hipMemcpy(empty.data(), slots, sizeof(double) * nblocks, hipMemcpyDeviceToHost));
+        for (unsigned long long v : empty)
+            if (v != SLOT_EMPTY) throw Error("reduce probe: a partial slot was not left empty");
+        out[0] = h[0];
+        out[1] = h[4];
+        out[2] = h[8];
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
 }
 
 double stream_copy_probe(long n, int reps)
@@ -2477,13 +2497,13 @@ void op_jacobi_init(Chunk& c, int coefficient, double dt)
 double op_jacobi_solve(Chunk& c, bool fused)
 {
     if (fused) {
-        TIMED(c, K_JACOBI, STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_U], c.f[F_R], c.reducer()));
+        TIMED(c, K_JACOBI, STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_U], c.f[F_R], c.reducer(S_ERR, EP_STORE)));
         std::swap(c.f[F_U], c.f[F_R]);
     } else {
         TL_HIP(hipMemcpyAsync(c.f[F_R], c.f[F_U], sizeof(double) * c.g.alloc, hipMemcpyDeviceToDevice, c.stream));
-        STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_R], c.f[F_U], c.reducer());
+        STRIP_LAUNCH(k_jacobi_solve, c, kptr(c), c.f[F_U0], c.f[F_R], c.f[F_U], c.reducer(S_ERR, EP_STORE));
     }
-    finish<1>(c, c.g.nblocks, S_ERR, EP_STORE);
+    c.close(c.g.nblocks, 1);
     return c.result(S_ERR);
 }
 
@@ -2496,8 +2516,8 @@ void op_calc_residual(Chunk& c)
 
 double op_dot(Chunk& c, const double* a, const double* b)
 {
-    STRIP_LAUNCH(k_dot, c, a, b, c.reducer());
-    finish<1>(c, c.g.nblocks, S_NORM, EP_STORE);
+    STRIP_LAUNCH(k_dot, c, a, b, c.reducer(S_NORM, EP_STORE));
+    c.close(c.g.nblocks, 1);
     return c.result(S_NORM);
 }
 
@@ -2517,8 +2537,8 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
     } else if (!first && beta_blocks > 0) {
         finish<1>(c, beta_blocks, S_ERR, EP_FCG_BETA);
     } else if (!first) {  // beta = -(z . w) / pw_prev, w still A p_prev
-        STRIP_LAUNCH(k_dot, c, c.f[F_Z], c.f[F_W], c.reducer());
-        finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_BETA);
+        STRIP_LAUNCH(k_dot, c, c.f[F_Z], c.f[F_W], c.reducer(S_ERR, EP_FCG_BETA));
+        c.close(c.g.nblocks, 1);
     }
 #if TL_DIM == 2
     if (!first) {
@@ -2536,31 +2556,31 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
                 if (u_pending)
                     hipLaunchKernelGGL((k_cg_pw_strip<true, false>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream,
                                        c.g, t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U],
-                                       c.reducer());
+                                       c.reducer(S_PW, EP_FCG_PW_ZR));
                 else
                     hipLaunchKernelGGL((k_cg_pw_strip<false, false>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream,
                                        c.g, t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr,
-                                       c.reducer());
+                                       c.reducer(S_PW, EP_FCG_PW_ZR));
                 check_launch();
             });
             u_pending = false;
             std::swap(c.f[F_P], c.p_alt);
-            finish<1>(c, nb, S_PW, EP_FCG_PW_ZR);
+            c.close(nb, 1);
         } else {
         TIMED(c, K_FCG_STRIP, {
             if (u_pending)
                 hipLaunchKernelGGL((k_cg_pw_strip<true, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g, t,
-                                   kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer(),
+                                   kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], c.f[F_U], c.reducer(S_PW, EP_FCG_PWPR),
                                    c.f[F_R]);
             else
                 hipLaunchKernelGGL((k_cg_pw_strip<false, true>), dim3((unsigned)nb), dim3(BLOCK), 0, c.stream, c.g,
-                                   t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(),
+                                   t, kptr(c), c.f[F_P], c.f[F_Z], c.ds, c.p_alt, c.f[F_W], nullptr, c.reducer(S_PW, EP_FCG_PWPR),
                                    c.f[F_R]);
             check_launch();
         });
         u_pending = false;
         std::swap(c.f[F_P], c.p_alt);
-        finish<2>(c, nb, S_PW, EP_FCG_PWPR);
+        c.close(nb, 2);
         }
     } else
 #endif
@@ -2571,10 +2591,10 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
         }
         STRIP_LAUNCH(k_fcg_p, c, 0.0, c.ds, first ? 1 : 0, c.f[F_Z], c.f[F_P]);
         TIMED(c, K_CG_W, STRIP_LAUNCH(k_cg_calc_w<false>, c, kptr(c), c.f[F_P], nullptr, nullptr, nullptr, c.f[F_W],
-                                      c.reducer(), nullptr));
-        finish<1>(c, c.g.nblocks, S_PW, EP_FCG_PW);
-        STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer());
-        finish<1>(c, c.g.nblocks, S_ERR, EP_FCG_ALPHA);
+                                      c.reducer(S_PW, EP_FCG_PW), nullptr));
+        c.close(c.g.nblocks, 1);
+        STRIP_LAUNCH(k_dot, c, c.f[F_P], c.f[F_R], c.reducer(S_ERR, EP_FCG_ALPHA));
+        c.close(c.g.nblocks, 1);
     }
     if (TL_DIM == 2 && defer_r) {
         u_pending = true;
@@ -2583,19 +2603,19 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
     if (TL_DIM == 2) {
         // r only: u += alpha p is left to the next strip (or op_cg_calc_u_dev)
         TIMED(c, K_CG_R, STRIP_LAUNCH(k_cg_calc_ur<false>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U],
-                                      c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+                                      c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_FCG_RRN)));
         u_pending = true;
     } else {
         TIMED(c, K_CG_UR, STRIP_LAUNCH(k_cg_calc_ur<true>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U],
-                                       c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer()));
+                                       c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_FCG_RRN)));
     }
-    finish<1>(c, c.g.nblocks, S_RRN, EP_FCG_RRN);
+    c.close(c.g.nblocks, 1);
 }
 
 double op_calc_2norm(Chunk& c, bool norm_r)
 {
-    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, norm_r ? c.f[F_R] : c.f[F_U0], c.reducer()));
-    finish<1>(c, c.g.nblocks, S_NORM, EP_STORE);
+    TIMED(c, K_2NORM, STRIP_LAUNCH(k_2norm, c, norm_r ? c.f[F_R] : c.f[F_U0], c.reducer(S_NORM, EP_STORE)));
+    c.close(c.g.nblocks, 1);
     return c.result(S_NORM);
 }
 

@@ -63,7 +63,7 @@ struct MXArgs {
     const DevScalars* ds;
     double inv_theta;
     const float* coef;  // a[0..L) then b[0..L)
-    double* partials;
+    Reduce red;  // closes the rrn reduction
     int tiles_x;
     int rows;
     long nblocks;
@@ -313,7 +313,7 @@ __device__ __forceinline__ void mx_wave(const MXArgs& A, const MXCtx& c)
         __builtin_amdgcn_sched_barrier(0);
     }
     double v[1] = {st.acc};
-    block_partial<1, 1>(v, Reduce{A.partials});
+    block_partial<1, 1>(v, A.red);
 }
 
 template <int L>
@@ -421,9 +421,9 @@ void op_mixed_prepare(Chunk& c, const double* alphas, const double* betas, int i
 }
 
 // One mixed-precision PPCG outer iteration after CGCalcW (alpha on the
-// device): u and r updated in place of CGCalcUr .. Calculate2Norm, the rrn
-// reduction's first phase left in Chunk::partials (returns its block count).
-long op_ppcg_outer_mixed(Chunk& c, int inner_steps)
+// device): u and r updated in place of CGCalcUr .. Calculate2Norm, and the
+// rrn reduction closed (beta, error and rro on the device, rrn in S_RRN).
+void op_ppcg_outer_mixed(Chunk& c, int inner_steps)
 {
     if (c.mixed_steps != inner_steps || !c.kxy32) throw Error("mixed-precision PPCG used before op_mixed_prepare");
     MXArgs a{};
@@ -436,7 +436,7 @@ long op_ppcg_outer_mixed(Chunk& c, int inner_steps)
     a.ds = c.ds;
     a.inv_theta = c.inv_theta;
     a.coef = c.d_coef32;
-    a.partials = c.partials;
+    a.red = c.reducer(S_RRN, EP_RRN);
     const int L = inner_steps;
     const int ow = 2 * WAVE - 2 * (L + (L & 1));
     a.tiles_x = (c.g.nx + ow - 1) / ow;
@@ -450,8 +450,8 @@ long op_ppcg_outer_mixed(Chunk& c, int inner_steps)
 #undef TL_MX_CASE
     default: throw Error("mixed-precision PPCG pass needs 1, 3 or 10 levels");
     }
+    c.close(a.nblocks, 1);
     std::swap(c.f[F_R], c.r_alt);
-    return a.nblocks;
 }
 
 #else  // the 7-point stencil has no mixed-precision form
@@ -461,7 +461,7 @@ void op_mixed_prepare(Chunk&, const double*, const double*, int, double)
 {
     throw Error("mixed-precision PPCG is implemented for 2D");
 }
-long op_ppcg_outer_mixed(Chunk&, int) { throw Error("mixed-precision PPCG is implemented for 2D"); }
+void op_ppcg_outer_mixed(Chunk&, int) { throw Error("mixed-precision PPCG is implemented for 2D"); }
 
 #endif
 

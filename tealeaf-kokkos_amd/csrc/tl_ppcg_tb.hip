@@ -101,17 +101,16 @@ const uint2* tb_rowtab(Chunk& c)
 }
 
 // One PPCG outer iteration after CGCalcW: alpha is read from the device
-// scalars, u and r are updated, and the rrn reduction's first phase is left
-// in Chunk::partials (returns its block count).  The inner steps run as
+// scalars, u and r are updated, and the last pass closes the rrn reduction
+// (beta, error and rro on the device, rrn in S_RRN).  The inner steps run as
 // passes of 10, 5, 2 and 1 levels (largest first), handing sd over through
 // memory between passes; every cell's arithmetic is the same in any split.
-long op_ppcg_outer_tb(Chunk& c, int inner_steps)
+void op_ppcg_outer_tb(Chunk& c, int inner_steps)
 {
     if (!ppcg_tb_supported(c, inner_steps)) throw Error("temporal-blocked PPCG is 2D, no preconditioner");
     if ((int)c.alphas.size() < inner_steps) throw Error("PPCG coefficients shorter than the inner steps");
     const std::vector<int> passes = tb_passes(inner_steps);
     tb_coefs(c, inner_steps, passes);
-    long last_blocks = 0;
     int done = 0;
     for (size_t q = 0; q < passes.size(); ++q) {
         const int L = passes[q];
@@ -131,7 +130,7 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.coef = c.d_coef + 2 * done;
         a.rowtab = tb_rowtab(c);
         a.row0 = TB_ROWTAB0;
-        a.partials = c.partials;
+        a.red = last ? c.reducer(S_RRN, EP_RRN) : c.reducer();
         const int ow = 2 * WAVE - 2 * tb::tb_margin(L);
         a.tiles_x = (c.g.nx + ow - 1) / ow;
         a.rows = tb_rows(c.g.ny, a.tiles_x, L, c.simds, tb::TB_UNROLL);
@@ -144,19 +143,18 @@ long op_ppcg_outer_tb(Chunk& c, int inner_steps)
         case 2: tb::tb_launch_2(c, a, first, last); break;
         default: tb::tb_launch_1(c, a, first, last); break;
         }
+        if (last) c.close(a.nblocks, 1);
         std::swap(c.f[F_R], c.r_alt);
         if (!last) std::swap(c.f[F_SD], c.sd_alt);
         done += L;
-        last_blocks = a.nblocks;
     }
-    return last_blocks;
 }
 
 #else  // the 7-point stencil runs the PPCG steps as separate sweeps
 
 bool ppcg_tb_supported(const Chunk&, int) { return false; }
 std::vector<int> tb_passes(int) { return {}; }
-long op_ppcg_outer_tb(Chunk&, int) { throw Error("temporal-blocked PPCG is implemented for 2D"); }
+void op_ppcg_outer_tb(Chunk&, int) { throw Error("temporal-blocked PPCG is implemented for 2D"); }
 
 #endif
 

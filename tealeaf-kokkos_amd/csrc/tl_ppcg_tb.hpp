@@ -52,7 +52,7 @@ struct TBArgs {
     // reflected into the mesh) for rows row0 .. row0 + count - 1
     const uint2* rowtab;
     int row0;
-    double* partials;
+    Reduce red;  // the last pass closes the rrn reduction
     int tiles_x;
     int rows;  // output rows per wave
     long nblocks;
@@ -403,7 +403,7 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
     }
     if (LAST) {
         double v[1] = {st.acc};
-        block_partial<1, 1>(v, Reduce{A.partials});
+        block_partial<1, 1>(v, A.red);
     }
 }
 

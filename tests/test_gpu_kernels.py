@@ -14,6 +14,8 @@ update_halo; the HIP kernels read the mirrored interior value.  The tests
 therefore refresh the halos of stencil inputs exactly where the reference's
 driver does (tea_solve.f90) before comparing.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -581,3 +583,47 @@ def test_fcg_smooth_refused_in_3d(hip):
     assert b"2D only" in g.L.tl_last_error()
     g.L.tl_clear_error()
     g.close()
+
+
+def reduction_order(x):
+    """The grid reduction's summation order: thread t of a 256-lane
+    workgroup adds partials t, t + 256, ... in turn; each 64-lane wave folds
+    with the xor butterfly (offsets 32 .. 1); the four waves' sums are added
+    in wave order."""
+    lanes = np.zeros(256)
+    for t in range(256):
+        s = 0.0
+        for v in x[t::256]:
+            s = s + v
+        lanes[t] = s
+    idx = np.arange(64)
+    waves = []
+    for w in range(4):
+        v = lanes[64 * w:64 * (w + 1)].copy()
+        for off in (32, 16, 8, 4, 2, 1):
+            v = v + v[idx ^ off]
+        waves.append(v[0])
+    s = waves[0]
+    for w in waves[1:]:
+        s = s + w
+    return s
+
+
+@pytest.mark.parametrize("nwaves", [1, 4])
+@pytest.mark.parametrize("nblocks", [1, 3, 64, 255, 256, 257, 1000, 4097, 70000])
+def test_single_pass_reduction(hip, nwaves, nblocks):
+    """The last workgroup's sum (single pass, counter reset for the next
+    reduction) equals the separate k_finish launch and the stated order
+    bitwise, for workgroups of one wave (temporal-blocked passes) and four
+    (streaming sweeps), on values spread over 16 decades so that any other
+    order would round differently."""
+    rng = np.random.default_rng(nblocks * 7 + nwaves)
+    x = rng.standard_normal(nblocks) * 10.0 ** rng.uniform(-8, 8, nblocks)
+    out = np.zeros(3)
+    L = hip.lib(2)
+    ptr = ctypes.POINTER(ctypes.c_double)
+    rc = L.tl_reduce_probe(nblocks, nwaves, x.ctypes.data_as(ptr), out.ctypes.data_as(ptr))
+    assert rc == 0, L.tl_last_error().decode()
+    want = reduction_order(x)
+    assert out[0] == want and out[1] == want and out[2] == want, (out, want)
+    assert abs(want - np.sum(np.sort(x))) <= 1e-12 * np.sum(np.abs(x))
