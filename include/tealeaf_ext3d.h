@@ -251,6 +251,25 @@ void tl_learn_destroy(void* learner);
  * Returns the number of pairs added. */
 long tl_learn_add_deck(void* learner, const char* deck, int snapshots, int crops, int crop,
                        unsigned long long seed);
+/* Test hook of the two-grid preconditioner's coarse pieces on a chunk (2D;
+ * nc coarse cells a side, blocks of B = x_cells / nc cells): kxc, kyc the
+ * west and south faces of the coarse operator P^T A P / B^2 = I + K_c (the
+ * sums of the fine faces across each coarse face over B^2, 0 on the mesh
+ * boundary), fc the block mean of the density feature, rc the restricted
+ * residual P^T (r - A z) / B^2 of the chunk's r and z fields (of r alone
+ * without with_z); row-major (nc x y_cells / B), any output may be NULL.  No
+ * counterpart in the reference.  0, or -1 with tl_last_error() set. */
+int tl_coarse_probe(const int* chunk, int nc, int with_z, float* kxc, float* kyc, float* fc, double* rc);
+/* The network as the coarse-grid solve of a two-grid preconditioner (deck
+ * keyword tl_fcg_coarse with tl_use_fcg_learned): FP64 PPCG solve of the
+ * deck's first timestep, one pair per snapshot, each the whole nc x nc
+ * coarse grid of the mesh's (n / nc)^2-cell blocks (nc a multiple of 8
+ * dividing x_cells and y_cells, y_cells / (x_cells / nc) a multiple of 8
+ * too): target the snapshot's restricted error e_c, input the coarse
+ * residual (I + K_c) e_c of the Galerkin coarse operator scaled by 1 / B^2.
+ * A learner holds crops or coarse pairs, not both.  Returns the number of
+ * pairs added. */
+long tl_learn_add_deck_coarse(void* learner, const char* deck, int snapshots, int nc);
 long tl_learn_pairs(void* learner);
 /* See tealeaf_ext2d.h (the learned preconditioner is 2D only). */
 int tl_learn_consistent_pairs(void* learner, int on);

@@ -329,7 +329,33 @@ struct FcgOptions {
     bool net_restart = false;  // start a new recurrence (p = z) after the last network iteration
     int smooth = 4;
     double omega = 0.8;
+    // the network as the coarse-grid solve of a two-grid preconditioner
+    // (tl_coarse.hip), in every iteration: sweeps, restriction, network,
+    // prolongation, sweeps
+    bool coarse = false;
+    bool coarse_pre = true;  // smooth before the coarse correction too (else only after it)
 };
+
+// Coarse grid of the two-grid preconditioner: blocks of B x B cells, ncx x
+// ncy of them (B = 0: the mesh does not divide into nc blocks a side with
+// ncy a multiple of 8).
+struct CoarseGrid {
+    int B, ncx, ncy;
+};
+CoarseGrid coarse_grid(const Geom& g, int nc);
+// rc = P^T (r - A z) / B^2 (FP64, ncx * ncy; without with_z, of r alone) and
+// the network's input plane rc / s, s = rms(rc) left in *s_dev; r, z: the
+// chunk's F_R, F_Z.
+void op_coarse_restrict(Chunk& c, const CoarseGrid& cg, bool with_z, double* rc, float* plane, double* s_dev);
+// F_Z += scale * s * P ez (ez the network's output plane); F_Z = that without `add`.
+void op_coarse_prolong(Chunk& c, const CoarseGrid& cg, const float* ez, const double* s_dev, double scale, bool add);
+// The coarse operator's face coefficients (west kxc, south kyc; 0 on the
+// mesh boundary) and the block mean of the density feature.
+void coarse_fields(const Geom& g, const CoarseGrid& cg, const double* kx, const double* ky, const double* density,
+                   float* kxc, float* kyc, float* fc, hipStream_t s);
+// out = block mean of a - b.
+void coarse_mean_diff(const Geom& g, const CoarseGrid& cg, const double* a, const double* b, float* out,
+                      hipStream_t s);
 bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, int max_iters, int& iters,
                        double& error);
 // The flexible CG's preconditioner sweeps (weighted Jacobi on A z = r, into
@@ -356,6 +382,13 @@ void learn_destroy(Learner* L);
 long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop,
                     unsigned long long seed);
 long learn_pairs(const Learner& L);
+// Runs the first timestep of `deck` with FP64 PPCG and adds one pair per
+// snapshot for the network as the coarse-grid solve of the two-grid
+// preconditioner: the whole nc x nc coarse grid (blocks of n / nc cells a
+// side) with the snapshot's restricted error e_c, the consistent input
+// (I + K_c) e_c, the coarse operator and density feature.  Returns the
+// number of pairs added.
+long learn_add_deck_coarse(Learner& L, const std::string& deck, int snapshots, int nc);
 void learn_set_consistent(Learner& L, bool on);
 void learn_set_phases(Learner& L, int mask);
 void learn_train(Learner& L, int steps, int batch, double lr, double* losses);

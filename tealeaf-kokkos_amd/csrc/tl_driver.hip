@@ -66,7 +66,7 @@ struct Deck {
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
     FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_net_every=<m>, tl_fcg_net_iters=<m>,
-                     // tl_fcg_no_network
+                     // tl_fcg_no_network, tl_fcg_net_restart, tl_fcg_coarse
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -231,6 +231,8 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_fcg_net_every") d.fcg.net_every = I();
             else if (k == "tl_fcg_net_iters") d.fcg.net_iters = I();
             else if (k == "tl_fcg_net_restart") d.fcg.net_restart = true;
+            else if (k == "tl_fcg_coarse") d.fcg.coarse = true;
+            else if (k == "tl_fcg_coarse_post") d.fcg.coarse = true, d.fcg.coarse_pre = false;
             else if (k == "tl_use_chebyshev") { d.use_cg = d.use_ppcg = d.use_jacobi = d.ppcg_mixed = d.use_fcg = false; d.use_cheby = true; }
             else if (k == "tl_max_iters") d.max_iters = I();
             else if (k == "tl_eps") d.eps = R();

@@ -194,6 +194,47 @@ int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int
     return rc;
 }
 
+// The two-grid preconditioner's coarse pieces on a chunk (test hook):
+// nc coarse cells a side; kxc, kyc, fc (ncx * ncy, row-major) the coarse
+// operator's west and south faces and the density feature; rc the
+// restricted residual P^T (r - A z) / B^2 of the chunk's r and z (of r
+// alone without with_z).  Any output may be null.  0, or -1 on error.
+int tl_coarse_probe(const int* chunk, int nc, int with_z, float* kxc, float* kyc, float* fc, double* rc)
+{
+    int rc_ = -1;
+    guarded([&] {
+        Chunk* c = stencil_chunk(chunk);
+        const CoarseGrid cg = coarse_grid(c->g, nc);
+        if (cg.B == 0) throw Error("the mesh does not divide into " + std::to_string(nc) + " coarse cells a side");
+        const long n = (long)cg.ncx * cg.ncy;
+        float* d32 = nullptr;
+        double* d64 = nullptr;
+        float* plane = nullptr;
+        TL_HIP(hipMalloc(&d32, sizeof(float) * 3 * n));
+        TL_HIP(hipMalloc(&d64, sizeof(double) * (n + 1)));
+        TL_HIP(hipMalloc(&plane, sizeof(float) * n));
+        try {
+            coarse_fields(c->g, cg, c->f[F_KX], c->f[F_KY], c->f[F_DENSITY], d32, d32 + n, d32 + 2 * n, c->stream);
+            op_coarse_restrict(*c, cg, with_z != 0, d64, plane, d64 + n);
+            c->sync();
+            if (kxc) TL_HIP(hipMemcpy(kxc, d32, sizeof(float) * n, hipMemcpyDeviceToHost));
+            if (kyc) TL_HIP(hipMemcpy(kyc, d32 + n, sizeof(float) * n, hipMemcpyDeviceToHost));
+            if (fc) TL_HIP(hipMemcpy(fc, d32 + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+            if (rc) TL_HIP(hipMemcpy(rc, d64, sizeof(double) * n, hipMemcpyDeviceToHost));
+        } catch (...) {
+            (void)hipFree(d32);
+            (void)hipFree(d64);
+            (void)hipFree(plane);
+            throw;
+        }
+        (void)hipFree(d32);
+        (void)hipFree(d64);
+        (void)hipFree(plane);
+        rc_ = 0;
+    });
+    return rc_;
+}
+
 // The sweeps-only flexible CG's preconditioner pass with the previous
 // step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four
 // sweeps on that r from zero; sums[0..3) = z.w (w as given), ||r||^2, z.r.
@@ -254,6 +295,18 @@ long tl_learn_add_deck(void* h, const char* deck, int snapshots, int crops, int 
     guarded([&] {
         if (!h || !deck) throw Error("tl_learn_add_deck: null argument");
         n = learn_add_deck(*static_cast<Learner*>(h), deck, snapshots, crops, crop, seed);
+    });
+    return n;
+}
+
+// One whole-coarse-grid pair per snapshot of the deck's FP64 PPCG solve (the
+// network as the two-grid preconditioner's coarse solve); -1 on error.
+long tl_learn_add_deck_coarse(void* h, const char* deck, int snapshots, int nc)
+{
+    long n = -1;
+    guarded([&] {
+        if (!h || !deck) throw Error("tl_learn_add_deck_coarse: null argument");
+        n = learn_add_deck_coarse(*static_cast<Learner*>(h), deck, snapshots, nc);
     });
     return n;
 }

@@ -47,6 +47,13 @@ struct Learner {
     long br_cap = 0, bf_cap = 0, be_cap = 0, bkx_cap = 0, bky_cap = 0;
     float *pr = nullptr, *pf = nullptr, *pz = nullptr;  // FCG planes
     long pr_cap = 0, pf_cap = 0, pz_cap = 0;
+    // coarse pairs: whole coarse grids of crop x crop cells (the network is
+    // then the coarse-grid solve of the two-grid preconditioner)
+    bool coarse = false;
+    double* crc = nullptr;  // the coarse right-hand side (FP64) and its rms
+    long crc_cap = 0;
+    float *ckx = nullptr, *cky = nullptr;  // FCG: the coarse operator (unused at inference) 
+    long ckx_cap = 0, cky_cap = 0;
     double* scratch = nullptr;                          // per-crop sums
     long scratch_cap = 0;
 };
@@ -286,26 +293,30 @@ void learn_destroy(Learner* L)
     if (!L) return;
     if (g_active == L) g_active = nullptr;
     unet_destroy(L->net);
-    for (float* p : {L->dr, L->df, L->de, L->dkx, L->dky, L->br, L->bf, L->be, L->bkx, L->bky, L->pr, L->pf, L->pz})
+    for (float* p : {L->dr, L->df, L->de, L->dkx, L->dky, L->br, L->bf, L->be, L->bkx, L->bky, L->pr, L->pf, L->pz,
+                     L->ckx, L->cky})
         (void)hipFree(p);
+    (void)hipFree(L->crc);
     (void)hipFree(L->scratch);
     delete L;
 }
 
-long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop, unsigned long long seed)
-{
-    if (crop < 8 || crop % 8) throw Error("training crops must be a multiple of 8 pixels");
-    if (L.crop && L.crop != crop) throw Error("all training crops must have the same size");
-    L.crop = crop;
-    snapshots = std::max(1, std::min<int>(snapshots, (int)(sizeof kSchedule / sizeof kSchedule[0])));
-    struct Snap {
-        double *u = nullptr, *r = nullptr;
-    };
+namespace {
+
+struct Snap {
+    double *u = nullptr, *r = nullptr;
+};
+
+// The iterates of one FP64 PPCG solve (the deck's first timestep) that
+// training pairs are made from: u_k, r_k at the schedule's iterations of the
+// phases the learner samples, u* at convergence, and the density and face
+// coefficients of the chunk.
+struct SnapSet {
     std::vector<Snap> snaps;
     double *ustar = nullptr, *dens = nullptr, *kx = nullptr, *ky = nullptr;
     Geom geom{};
-    bool seen = false;
-    auto release = [&] {
+    ~SnapSet()
+    {
         for (Snap& s : snaps) {
             (void)hipFree(s.u);
             (void)hipFree(s.r);
@@ -314,7 +325,13 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         (void)hipFree(dens);
         (void)hipFree(kx);
         (void)hipFree(ky);
-    };
+    }
+};
+
+void collect_snapshots(Learner& L, const std::string& deck, int snapshots, SnapSet& set)
+{
+    snapshots = std::max(1, std::min<int>(snapshots, (int)(sizeof kSchedule / sizeof kSchedule[0])));
+    bool seen = false;
     // `snapshots` iterates of each phase, at the schedule's iterations: the
     // CG presteps (early, sharp residuals, like the flexible CG's own) and
     // the PPCG outer iterations (late, smooth errors)
@@ -326,40 +343,49 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         const size_t bytes = sizeof(double) * (size_t)c.g.alloc;
         Snap s;
         TL_HIP(hipMalloc(&s.u, bytes));
-        TL_HIP(hipMalloc(&s.r, bytes));
-        TL_HIP(hipMemcpyAsync(s.u, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
-        TL_HIP(hipMemcpyAsync(s.r, c.f[F_R], bytes, hipMemcpyDeviceToDevice, c.stream));
-        snaps.push_back(s);
+        set.snaps.push_back(s);
+        TL_HIP(hipMalloc(&set.snaps.back().r, bytes));
+        TL_HIP(hipMemcpyAsync(set.snaps.back().u, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(set.snaps.back().r, c.f[F_R], bytes, hipMemcpyDeviceToDevice, c.stream));
     };
     g_hooks.on_cg = [&](Chunk& c, int n) { snap(c, n, 0); };
     g_hooks.on_outer = [&](Chunk& c, int k) { snap(c, k, 1); };
     g_hooks.on_end = [&](Chunk& c) {
         if (seen) return;  // the first timestep only
         seen = true;
-        geom = c.g;
+        set.geom = c.g;
         const size_t bytes = sizeof(double) * (size_t)c.g.alloc;
-        TL_HIP(hipMalloc(&ustar, bytes));
-        TL_HIP(hipMalloc(&dens, bytes));
-        TL_HIP(hipMemcpyAsync(ustar, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
-        TL_HIP(hipMemcpyAsync(dens, c.f[F_DENSITY], bytes, hipMemcpyDeviceToDevice, c.stream));
-        TL_HIP(hipMalloc(&kx, bytes));
-        TL_HIP(hipMalloc(&ky, bytes));
-        TL_HIP(hipMemcpyAsync(kx, c.f[F_KX], bytes, hipMemcpyDeviceToDevice, c.stream));
-        TL_HIP(hipMemcpyAsync(ky, c.f[F_KY], bytes, hipMemcpyDeviceToDevice, c.stream));
+        for (double** q : {&set.ustar, &set.dens, &set.kx, &set.ky}) TL_HIP(hipMalloc(q, bytes));
+        TL_HIP(hipMemcpyAsync(set.ustar, c.f[F_U], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(set.dens, c.f[F_DENSITY], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(set.kx, c.f[F_KX], bytes, hipMemcpyDeviceToDevice, c.stream));
+        TL_HIP(hipMemcpyAsync(set.ky, c.f[F_KY], bytes, hipMemcpyDeviceToDevice, c.stream));
         TL_HIP(hipStreamSynchronize(c.stream));
     };
     try {
         (void)driver_run_deck(deck, 1);
     } catch (...) {
         g_hooks = SolveHooks{};
-        release();
         throw;
     }
     g_hooks = SolveHooks{};
-    if (!seen || snaps.empty() || geom.nx < crop || geom.ny < crop) {
-        release();
-        throw Error("the deck's solve gave no training pairs (2D PPCG deck larger than the crop needed)");
-    }
+    if (!seen || set.snaps.empty()) throw Error("the deck's solve gave no training snapshots (a 2D PPCG deck is needed)");
+}
+
+}  // namespace
+
+long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crops, int crop, unsigned long long seed)
+{
+    if (crop < 8 || crop % 8) throw Error("training crops must be a multiple of 8 pixels");
+    if (L.crop && L.crop != crop) throw Error("all training crops must have the same size");
+    if (L.coarse) throw Error("the learner holds coarse-grid pairs; crops cannot be added");
+    SnapSet set;
+    collect_snapshots(L, deck, snapshots, set);
+    const Geom geom = set.geom;
+    if (geom.nx < crop || geom.ny < crop) throw Error("the deck's mesh is smaller than the training crop");
+    L.crop = crop;
+    std::vector<Snap>& snaps = set.snaps;
+    double *ustar = set.ustar, *dens = set.dens, *kx = set.kx, *ky = set.ky;
     const long n_new = (long)snaps.size() * crops;
     const long px = (long)crop * crop;
     const long need = (L.pairs + n_new) * px, keep = L.pairs * px;
@@ -407,7 +433,52 @@ long learn_add_deck(Learner& L, const std::string& deck, int snapshots, int crop
         L.pairs += crops;
     }
     (void)hipFree(dxy);
-    release();
+    return n_new;
+}
+
+long learn_add_deck_coarse(Learner& L, const std::string& deck, int snapshots, int nc)
+{
+    if (L.crop && (!L.coarse || L.crop != nc)) throw Error("all training pairs must be coarse grids of one size");
+    SnapSet set;
+    collect_snapshots(L, deck, snapshots, set);
+    const CoarseGrid cg = coarse_grid(set.geom, nc);
+    if (cg.B == 0 || cg.ncy != nc)
+        throw Error("the deck's mesh does not divide into " + std::to_string(nc) + " x " + std::to_string(nc) +
+                    " square blocks");
+    L.crop = nc;
+    L.coarse = true;
+    const long px = (long)nc * nc;
+    const long n_new = (long)set.snaps.size();
+    const long need = (L.pairs + n_new) * px, keep = L.pairs * px;
+    regrow(L.dr, L.dr_cap, need, keep);
+    regrow(L.df, L.df_cap, need, keep);
+    regrow(L.de, L.de_cap, need, keep);
+    regrow(L.dkx, L.dkx_cap, need, keep);
+    regrow(L.dky, L.dky_cap, need, keep);
+    regrow(L.scratch, L.scratch_cap, 1, 0);
+    for (size_t k = 0; k < set.snaps.size(); ++k) {
+        const long off = L.pairs * px;
+        // the restricted error e_c of the snapshot and the coarse operator;
+        // the input is the consistent coarse residual (I + K_c) e_c
+        coarse_fields(set.geom, cg, set.kx, set.ky, set.dens, L.dkx + off, L.dky + off, L.df + off, nullptr);
+        coarse_mean_diff(set.geom, cg, set.ustar, set.snaps[k].u, L.de + off, nullptr);
+        hipLaunchKernelGGL(k_crop_residual, dim3(1), dim3(256), 0, nullptr, nc, L.de + off, L.dkx + off,
+                           L.dky + off, L.dr + off);
+        launched("coarse pair residual");
+        // scaled by the rms of its own input, as the preconditioner scales it
+        TL_HIP(hipMemset(L.scratch, 0, sizeof(double)));
+        hipLaunchKernelGGL(k_sumsq, dim3(blocks_for(px)), dim3(256), 0, nullptr, L.dr + off, px, L.scratch);
+        launched("coarse pair norm");
+        double ss = 0.0;
+        TL_HIP(hipMemcpy(&ss, L.scratch, sizeof(double), hipMemcpyDeviceToHost));
+        const double rms = std::sqrt(ss / (double)px);
+        const double inv_g = L.escale > 0.0 ? 1.0 / L.escale : 1.0;
+        hipLaunchKernelGGL(k_normalise, dim3(1), dim3(256), 0, nullptr, nc, rms > 0.0 ? 1.0 / rms : 0.0, inv_g,
+                           L.dr + off, L.de + off);
+        launched("coarse pair normalisation");
+        TL_HIP(hipDeviceSynchronize());
+        L.pairs += 1;
+    }
     return n_new;
 }
 
@@ -466,7 +537,28 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     const long n = (long)c.g.nx * c.g.ny;
     float *pf = nullptr, *pz = nullptr;
     hipStream_t keep = nullptr;
-    if (opt.network) {
+    CoarseGrid cg{};
+    if (opt.coarse) {
+        if (!opt.network) throw Error("tl_fcg_coarse needs the network (tl_fcg_no_network is set)");
+        if (!L->coarse) throw Error("tl_fcg_coarse: the loaded learned preconditioner was not trained on coarse grids");
+        cg = coarse_grid(c.g, L->crop);
+        if (cg.B == 0)
+            throw Error("tl_fcg_coarse: the mesh does not divide into the network's " + std::to_string(L->crop) +
+                        " coarse cells a side (with a multiple of 8 the other way)");
+        const long nc = (long)cg.ncx * cg.ncy;
+        regrow(L->pr, L->pr_cap, nc, 0);
+        regrow(L->pf, L->pf_cap, nc, 0);
+        regrow(L->pz, L->pz_cap, nc, 0);
+        regrow(L->crc, L->crc_cap, nc + 1, 0);
+        regrow(L->ckx, L->ckx_cap, nc, 0);
+        regrow(L->cky, L->cky_cap, nc, 0);
+        // the coarse density feature (the coarse operator is not needed here)
+        coarse_fields(c.g, cg, c.f[F_KX], c.f[F_KY], c.f[F_DENSITY], L->ckx, L->cky, L->pf, c.stream);
+        pf = L->pf;
+        pz = L->pz;
+        keep = unet_stream(*L->net);
+        unet_set_stream(*L->net, c.stream);
+    } else if (opt.network) {
         regrow(L->pr, L->pr_cap, n, 0);
         regrow(L->pf, L->pf_cap, n, 0);
         regrow(L->pz, L->pz_cap, n, 0);
@@ -533,6 +625,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     // one that applies it keeps r current.
     const bool sweeps_tb = TL_DIM == 2 && fcg_smooth_tb_supported(c, opt.smooth);
     auto uses_net = [&](int j) {
+        if (opt.coarse) return true;  // the two-grid preconditioner in every iteration
         return opt.network && (j - 1) % std::max(1, opt.net_every) == 0 && (opt.net_iters <= 0 || j <= opt.net_iters);
     };
     bool r_pending = false;  // r -= alpha w of the last step not yet applied
@@ -547,6 +640,16 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             std::swap(c.f[F_Z], c.f[F_SD]);
             std::swap(c.f[F_R], c.r_alt);
             r_pending = false;
+        } else if (opt.coarse) {
+            // the two-grid preconditioner: z = S r from zero, the coarse
+            // correction z += s G P N(rc / s) with rc = P^T (r - A z) / B^2,
+            // then S again from that z (S: the weighted Jacobi sweeps)
+            if (opt.coarse_pre) (void)fcg_smooth(c, opt.smooth, opt.omega, true, true, false);
+            const long nc = (long)cg.ncx * cg.ncy;
+            op_coarse_restrict(c, cg, opt.coarse_pre, L->crc, L->pr, L->crc + nc);
+            unet_apply(*L->net, L->pr, pf, pz, 1, cg.ncy, cg.ncx);
+            op_coarse_prolong(c, cg, pz, L->crc + nc, L->escale > 0.0 ? L->escale : 1.0, opt.coarse_pre);
+            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, false, true, !first);
         } else if (uses_net(it)) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
@@ -564,7 +667,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         // beta, p, w, alpha, u[, r] with the scalars on the device; one read
         // after the last network iteration a restart begins a new recurrence
         // from the improved iterate (p = z), which needs r current
-        const bool restart_next = opt.net_restart && uses_net(it) && !uses_net(it + 1);
+        const bool restart_next = !opt.coarse && opt.net_restart && uses_net(it) && !uses_net(it + 1);
         const bool defer_r = sweeps_tb && !uses_net(it + 1) && !restart_next && it < max_iters;
         op_fcg_iteration_dev(c, first, beta_blocks, u_pending, beta_rrn, defer_r, !pr_next);
         pr_next = false;

@@ -82,6 +82,7 @@ def lib(dim: int) -> ctypes.CDLL:
             "tl_learn_add_deck": (ctypes.c_long, [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_ulonglong]),
             "tl_learn_pairs": (ctypes.c_long, [vp]),
+            "tl_learn_add_deck_coarse": (ctypes.c_long, [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
             "tl_learn_consistent_pairs": (ctypes.c_int, [vp, ctypes.c_int]),
             "tl_learn_loss_smoothing": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double]),
             "tl_learn_sample_phases": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -96,6 +97,7 @@ def lib(dim: int) -> ctypes.CDLL:
             "tl_learn_forward_bytes": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "tl_reduce_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, dp, dp]),
             "tl_set_single_pass": (None, [ctypes.c_int]),
+            "tl_coarse_probe": (ctypes.c_int, [_P_INT, ctypes.c_int, ctypes.c_int, fp, fp, fp, dp]),
             "tl_learn_time_forward": (ctypes.c_double, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                         dp]),
         }
@@ -331,6 +333,12 @@ class Learner:
 
     def add_deck(self, deck: str, snapshots: int = 12, crops: int = 64, crop: int = 64, seed: int = 0) -> int:
         return self._check(self.L.tl_learn_add_deck(self.h, deck.encode(), snapshots, crops, crop, seed))
+
+    def add_deck_coarse(self, deck: str, snapshots: int = 12, nc: int = 256) -> int:
+        """One pair per snapshot of the deck's FP64 PPCG solve, each the whole
+        nc x nc coarse grid (the network as the two-grid preconditioner's
+        coarse solve, deck keyword tl_fcg_coarse)."""
+        return self._check(self.L.tl_learn_add_deck_coarse(self.h, deck.encode(), snapshots, nc))
 
     def consistent_pairs(self, on: bool = True) -> None:
         """Input residual of later pairs: A_c e of the crop (default) or the
