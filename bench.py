@@ -236,12 +236,18 @@ def kernel_probe(n: int, launches: int) -> dict:
     return {"probe": "ppcg_outer, ppcg_outer_mixed", "n": n, "launches": launches, "host_s_per_launch": dt}
 
 
+# keywords added to every deck run_solve runs (--no-lookahead)
+DECK_OPTS: list[str] = []
+
+
 def run_solve(deck: str, dim: int, warmup: int, timing: bool = True, only: str | None = None,
               max_steps: int = 0) -> dict:
     """Run a deck through the C++ driver; the timed steps follow `warmup`.
     With `timing`, HIP events bracket every launch in the timed steps, or
     only the launches of kernel `only` (two events per launch of every
     kernel add about 3% to a 4096^2 PPCG timestep)."""
+    if DECK_OPTS:
+        deck = deck.replace("*endtea", "\n".join(DECK_OPTS) + "\n*endtea")
     L = tealeaf_hip.lib(dim)
     L.tl_set_progress(1)
     L.tl_set_kernel_timing(warmup + 1 if timing else 0)
@@ -495,7 +501,8 @@ def line(value_s: float, args, workload: str, cells: int, **extra) -> dict:
         "warmup": args.warmup, "ms_per_step": round(1e3 * value_s, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": workload, "cells": cells, "parallelism": "single chunk, one GPU",
-                   "reductions": "single pass" if args.single_pass else "separate finish launch"},
+                   "reductions": "single pass" if args.single_pass else "separate finish launch",
+                   "lookahead": not args.no_lookahead},
     }
     out.update(extra)
     return out
@@ -591,6 +598,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (default: physical cores, "
                                                                  "capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lookahead", action="store_true",
+                    help="launch each CG / PPCG iteration only after reading the previous rrn")
     ap.add_argument("--single-pass", action="store_true",
                     help="close every grid reduction inside its sweep (default: a separate one-workgroup launch)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -608,6 +617,8 @@ def main() -> int:
                     help="only run the c4 deck capped at 100 iterations (counter runs)")
     args = ap.parse_args()
     n = args.n or {"c2": 4096, "c3": 256, "c4": 8192, "c5": 4096}[args.workload]
+    if args.no_lookahead:
+        DECK_OPTS.append("tl_no_lookahead")
     if args.single_pass:
         for d in (2, 3):
             tealeaf_hip.lib(d).tl_set_single_pass(1)

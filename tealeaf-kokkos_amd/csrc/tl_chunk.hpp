@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -46,7 +47,9 @@ struct Error : std::runtime_error {
     } while (0)
 
 // Result slots of the host-mapped reduction buffer (4 doubles each).
-enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_COUNT };
+// S_RRN2: the rrn of odd iterations when the driver runs one iteration
+// ahead (the host reads iteration n's while n + 1's is being computed).
+enum Slot { S_RRO = 0, S_PW, S_RRN, S_NORM, S_ERR, S_SUMMARY, S_RRN2, S_COUNT };
 
 // Kernels timed by the optional per-launch event timer.
 enum KernelId {
@@ -73,6 +76,15 @@ struct KernelTimer {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> pool;
+    long flushes = 0;
+    // Pairs recorded after a mark can be dropped (launches that turned out to
+    // do nothing) unless a flush has resolved them since.
+    struct Mark {
+        long flushes;
+        size_t n;
+    };
+    Mark mark() const { return Mark{flushes, pending.size()}; }
+    void discard_since(const Mark& m);
     hipEvent_t take();
     void begin(int k, hipStream_t s, Pending& p);
     void end(Pending& p, hipStream_t s);
@@ -183,8 +195,21 @@ struct Chunk {
     Finish pending{};  // the reduction close() launches k_finish for
     bool pending_open = false;
 
+    // Speculation: while `gate` is set, the reductions (and through them the
+    // kernels that take them) launched belong to an iteration launched
+    // before the previous rrn was read, and do nothing if that rrn stopped
+    // the solve; rrn epilogues set ds->stop by stop_eps, stop_le, stop_gt.
+    bool gate = false;
+    double stop_eps = -1.0, stop_le = -HUGE_VAL, stop_gt = HUGE_VAL;
+
     // Partials only: an explicit finish closes the reduction.
-    Reduce reducer() const { return Reduce{partials, Finish{}}; }
+    Reduce reducer() const
+    {
+        Finish f{};
+        f.ds = ds;
+        f.gate = gate ? 1 : 0;
+        return Reduce{partials, f};
+    }
     // The next reduction of `slot`, with epilogue `ep`: closed inside the
     // sweep that takes it (single_pass), else by close() after its launch.
     Reduce reducer(Slot slot, int ep, int hist_n = -1);
@@ -228,7 +253,7 @@ void op_cg_calc_p(Chunk& c, double beta);
 // next calc_w with fold_u or for calc_u.
 void op_cg_calc_w_dev(Chunk& c, bool fold, bool fold_u = false);
 void op_cg_calc_ur_dev(Chunk& c, int n);
-void op_cg_calc_r_dev(Chunk& c, int n);
+void op_cg_calc_r_dev(Chunk& c, int n, Slot slot = S_RRN);
 void op_cg_calc_u_dev(Chunk& c);
 void op_cg_calc_p_dev(Chunk& c);
 // Chebyshev switch: keep the boundary cells' p (with the pending CGCalcP
@@ -262,7 +287,7 @@ int tb_rows(int ny, int tiles_x, int L, int slots, int unroll = 6);
 // with the given inner steps (each a compiled pass size).
 std::vector<int> tb_passes(int inner_steps);
 bool ppcg_tb_supported(const Chunk& c, int inner_steps);
-void op_ppcg_outer_tb(Chunk& c, int inner_steps);
+void op_ppcg_outer_tb(Chunk& c, int inner_steps, Slot slot = S_RRN);
 // First row of the reflected-row table of the temporal-blocked kernels
 // (2D): entry y - TB_ROWTAB0 holds the byte offsets of the cell row and the
 // face row that row y reads.

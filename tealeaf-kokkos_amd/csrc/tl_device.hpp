@@ -137,6 +137,7 @@ struct DevScalars {
     double rro_prev;  // rro before the last rrn epilogue (a replaced residual recomputes beta from it)
     double bad;       // flexible CG: 1 when the last alpha had p.w <= 0 or a non-finite p.r
     double zr;        // flexible CG, sweeps only: z . r of the iteration, alpha's numerator
+    double stop;      // 1 after an rrn at which the driver stops or leaves the iteration it speculated
 };
 
 // The scalar recurrence of tea_solve.f90 that consumes a reduction's total.
@@ -180,12 +181,23 @@ struct Finish {
     int hist_n;
     unsigned seq;
     unsigned max_groups;  // groups the counters serve; a larger grid leaves the result undelivered
+    // A speculative iteration (the driver launched it before reading the
+    // previous rrn): its kernels do nothing and its finish delivers only the
+    // sequence number when ds->stop is set.  An rrn epilogue sets ds->stop
+    // when |rrn| < stop_eps, rrn <= stop_le, or (stop_gt finite) not
+    // rrn <= stop_gt: a NaN rrn takes the branch the host's tests take.
+    int gate;
+    double stop_eps, stop_le, stop_gt;
 };
 
 struct Reduce {
     double* partials;  // [nblocks * NV] (a reduction closed by k_finish)
     Finish fin;        // fin.counter set: the sweep closes the reduction itself
 };
+
+// A kernel of a speculative iteration whose predecessor stopped the solve:
+// it returns at once (every workgroup, before any barrier).
+__device__ inline bool gated_off(const Reduce& r) { return r.fin.gate && r.fin.ds->stop != 0.0; }
 
 template <int NV>
 __device__ inline void wave_fold(double (&v)[NV])
@@ -336,6 +348,12 @@ template <int NV, bool IN_SWEEP>
 __device__ inline void finish_scalars(const double (&v)[NV], const Finish& f, const DevScalars& cur)
 {
     DevScalars* ds = f.ds;
+    if (f.gate && cur.stop != 0.0) {
+        // a speculative iteration after the stop: nothing to deliver but the number
+        __threadfence_system();
+        *reinterpret_cast<volatile unsigned*>(f.hseq) = f.seq;
+        return;
+    }
     double out[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = v[k];
@@ -361,6 +379,9 @@ __device__ inline void finish_scalars(const double (&v)[NV], const Finish& f, co
         ds->rro_prev = cur.rro;
         ds->rro = v[0];
         ds->error = v[0];
+        ds->stop = (std::fabs(v[0]) < f.stop_eps || v[0] <= f.stop_le || (f.stop_gt < HUGE_VAL && !(v[0] <= f.stop_gt)))
+                       ? 1.0
+                       : 0.0;
         out[1] = beta;
         break;
     }

@@ -62,6 +62,10 @@ struct Deck {
     // residual (no counterpart in the reference): the FP64 solve at the
     // accuracy the mixed and learned forms are held to.
     bool ppcg_true_residual = false;
+    // tl_no_lookahead: launch each CG or PPCG iteration only after the
+    // previous one's rrn is read (no counterpart in the reference; the
+    // results are the same either way)
+    bool lookahead = true;
     // Flexible CG with the learned preconditioner (no counterpart in the
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
@@ -221,6 +225,7 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_use_ppcg_mixed") { d.use_cheby = d.use_cg = d.use_jacobi = d.use_fcg = false; d.use_ppcg = d.ppcg_mixed = true; }
             else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
             else if (k == "tl_ppcg_true_residual") d.ppcg_true_residual = true;
+            else if (k == "tl_no_lookahead") d.lookahead = false;
             else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
             else if (k == "tl_fcg_omega") {
                 d.fcg.omega = R();
@@ -532,6 +537,9 @@ struct StepInfo {
     // learned flexible CG: its iterations, and whether FP64 PPCG took over
     int fcg_iters = 0;
     bool fcg = false, fcg_fallback = false;
+    // iterations launched before the previous rrn was read, and how many of
+    // them the stop flag turned into no-ops
+    int ahead = 0, ahead_undone = 0;
     std::string tea_out;  // the step's tea.out lines
 };
 
@@ -582,6 +590,46 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
     std::vector<double> cg_alphas, cg_betas;
     std::vector<double> ch_alphas(max_iters + 3, 0.0), ch_betas(max_iters + 3, 0.0);
     const bool want_history = d.use_cheby || d.use_ppcg;
+
+    // One iteration ahead.  A CG iteration before any switch, or an FP64
+    // PPCG outer iteration of the temporal-blocked form, launches the next
+    // iteration of its kind before the host reads its rrn, so the GPU never
+    // waits for the host between them.  The rrn's epilogue sets the device
+    // stop flag exactly when the host's tests (convergence, the switch rule
+    // of tea_solve.f90:173-179) make the next iteration something else; the
+    // launched iteration then does nothing, and the host undoes its swaps of
+    // the buffers.  Iteration counts and results are those of the
+    // one-at-a-time loop.
+    const bool pipe = d.lookahead && !c.single_pass && !solve_hooks().on_cg && !solve_hooks().on_outer;
+    bool ahead = false;  // iteration n's kernels were launched during iteration n - 1
+    struct Bufs {
+        double *p, *p_alt, *r, *r_alt, *sd, *sd_alt;
+        KernelTimer::Mark mark;
+    };
+    auto save_bufs = [&] { return Bufs{c.f[F_P], c.p_alt, c.f[F_R], c.r_alt, c.f[F_SD], c.sd_alt, c.timer.mark()}; };
+    // a speculative iteration that did nothing: its swaps undone, its timer pairs dropped
+    auto restore_bufs = [&](const Bufs& b) {
+        c.f[F_P] = b.p;
+        c.p_alt = b.p_alt;
+        c.f[F_R] = b.r;
+        c.r_alt = b.r_alt;
+        c.f[F_SD] = b.sd;
+        c.sd_alt = b.sd_alt;
+        c.timer.discard_since(b.mark);
+    };
+    auto rrn_slot = [](int k) { return (k & 1) ? S_RRN2 : S_RRN; };
+    // the error at or below which iteration k is a switch to Chebyshev or PPCG
+    // rather than a CG iteration (while no switch has happened)
+    auto switch_le = [&](int k) {
+        if (!(d.use_cheby || d.use_ppcg)) return -HUGE_VAL;
+        if (d.errswitch) return k >= 20 ? d.epslim : -HUGE_VAL;
+        return k >= d.presteps ? 1.0 : -HUGE_VAL;
+    };
+    // after a PPCG outer iteration the next is a CG iteration when not
+    // error <= 1 (the switch test of tea_solve.f90:176-177 fails again);
+    // under errswitch it is always PPCG
+    const double ppcg_gt = d.errswitch ? HUGE_VAL : 1.0;
+    auto ppcg_stops = [&](double e) { return std::fabs(e) < d.eps || (ppcg_gt < HUGE_VAL && !(e <= ppcg_gt)); };
 
     halo(c, {FIELD_ENERGY1, FIELD_DENSITY}, 2);
     if (d.use_cg || d.use_cheby || d.use_ppcg) {
@@ -697,14 +745,24 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     if (mixed_form) rrn_replaced = error;
                 }
                 // rro on the device is the last rrn (= error), as at tea_solve.f90:187,304
-                op_cg_calc_w_dev(c, p_pending);
-                if (mixed) {
+                const bool tb = !mixed && ppcg_tb_supported(c, d.ppcg_inner_steps);
+                const bool spec = pipe && tb && !mixed_form;
+                if (spec && ahead) {
+                    // launched during the previous iteration
+                } else if (mixed) {
+                    op_cg_calc_w_dev(c, p_pending);
                     TL_TIMED(c, K_PPCG_MIXED, op_ppcg_outer_mixed(c, d.ppcg_inner_steps));
                     ++si.mixed_steps;
-                } else if (ppcg_tb_supported(c, d.ppcg_inner_steps)) {
+                } else if (tb) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
-                    TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps));
+                    op_cg_calc_w_dev(c, p_pending);
+                    if (spec) {
+                        c.stop_eps = d.eps;
+                        c.stop_gt = ppcg_gt;
+                    }
+                    TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n)));
                 } else {
+                    op_cg_calc_w_dev(c, p_pending);
                     op_ppcg_calc_ur_dev(c);
                     // tea_leaf_run_ppcg_inner_steps (tea_solve.f90:607-656)
                     op_ppcg_init_sd(c);
@@ -719,7 +777,36 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     op_2norm_rrn_dev(c);
                 }
                 p_pending = true;
-                error = op_error(c);
+                if (spec) {
+                    ahead = false;
+                    Bufs bufs{};
+                    const bool next = n + 1 <= max_iters;
+                    if (next) {
+                        bufs = save_bufs();
+                        c.gate = true;
+                        c.stop_eps = d.eps;
+                        c.stop_gt = ppcg_gt;
+                        op_cg_calc_w_dev(c, true);
+                        TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n + 1)));
+                        c.gate = false;
+                    }
+                    error = c.result(rrn_slot(n));
+                    if (next) {
+                        ++si.ahead;
+                        if (ppcg_stops(error)) {
+                            restore_bufs(bufs);
+                            ++si.ahead_undone;
+                        } else {
+                            ahead = true;
+                        }
+                    }
+                    c.stop_eps = -1.0;
+                    c.stop_gt = HUGE_VAL;
+                } else if (tb) {
+                    error = c.result(rrn_slot(n));
+                } else {
+                    error = op_error(c);
+                }
                 if (solve_hooks().on_outer) solve_hooks().on_outer(c, cheby_calc_steps);
                 if (mixed_form) {
                     const bool converged = std::fabs(error) < d.eps;
@@ -749,6 +836,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             ++cheby_calc_steps;
         } else if (d.use_cg || d.use_cheby || d.use_ppcg) {
             ++cg_calc_steps;
+            bool via_strip = false;  // the p, w strip and calc_r path (rrn in rrn_slot(n))
             if (stale_p_halo) {
                 // p and u are up to date here: Chebyshev steps ran since the
                 // last CG iteration
@@ -758,15 +846,48 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
             } else if (defer_u) {
                 // CGCalcUr's u += alpha p rides on the next iteration's p, w
                 // strip, which reads p anyway (bitwise the same update)
-                op_cg_calc_w_dev(c, p_pending, u_pending);
-                op_cg_calc_r_dev(c, want_history ? n : 0);
+                via_strip = true;
+                if (!ahead) {
+                    c.stop_eps = d.eps;
+                    c.stop_le = switch_le(n + 1);
+                    op_cg_calc_w_dev(c, p_pending, u_pending);
+                    op_cg_calc_r_dev(c, want_history ? n : 0, rrn_slot(n));
+                }
                 u_pending = true;
             } else {
                 op_cg_calc_w_dev(c, p_pending);
                 op_cg_calc_ur_dev(c, want_history ? n : 0);
             }
             p_pending = true;
-            error = op_error(c);
+            if (via_strip) {
+                ahead = false;
+                Bufs bufs{};
+                const bool next = pipe && cheby_calc_steps == 0 && n + 1 <= max_iters;
+                if (next) {
+                    bufs = save_bufs();
+                    c.gate = true;
+                    c.stop_eps = d.eps;
+                    c.stop_le = switch_le(n + 2);
+                    op_cg_calc_w_dev(c, true, true);
+                    op_cg_calc_r_dev(c, want_history ? n + 1 : 0, rrn_slot(n + 1));
+                    c.gate = false;
+                }
+                error = c.result(rrn_slot(n));
+                if (next) {
+                    // the test the rrn's epilogue applied to set the stop flag
+                    ++si.ahead;
+                    if (std::fabs(error) < d.eps || error <= switch_le(n + 1)) {
+                        restore_bufs(bufs);
+                        ++si.ahead_undone;
+                    } else {
+                        ahead = true;
+                    }
+                }
+                c.stop_eps = -1.0;
+                c.stop_le = -HUGE_VAL;
+            } else {
+                error = op_error(c);
+            }
             if (solve_hooks().on_cg) solve_hooks().on_cg(c, n);
         } else if (d.use_jacobi && jacobi_tb_supported(c)) {
             // iterations n .. n + K - 1 in one temporal-blocked pass, ending
@@ -917,10 +1038,10 @@ std::string run_deck(const std::string& text, int max_steps)
         o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
             "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
             "\"dt\": %.17g, \"mixed_steps\": %d, \"replacements\": %d, \"mixed_fallback\": %d, "
-            "\"fcg\": %d, \"fcg_iters\": %d, \"fcg_fallback\": %d, ",
+            "\"fcg\": %d, \"fcg_iters\": %d, \"fcg_fallback\": %d, \"ahead\": %d, \"ahead_undone\": %d, ",
             si.iters, si.error, si.exact_error, si.switch_step, si.est_itc, si.eigmin, si.eigmax, si.cg_steps,
             si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0,
-            si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0);
+            si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0, si.ahead, si.ahead_undone);
         if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
             summary(o, tea_out, *c, step, time);
         else

@@ -362,6 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_w(Geom g, KPtr k, const doubl
                                                      double* __restrict__ p_out, double* __restrict__ w, Reduce red,
                                                      double* __restrict__ u)
 {
+    if (gated_off(red)) return;
     Strip s = make_strip(g);
     double v[1] = {0.0};
     const double beta = FOLD ? ds->beta : 0.0;
@@ -526,6 +527,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_z(Geom g, KPtr k, const double*
                                                    double* __restrict__ p_out, double* __restrict__ w,
                                                    double* __restrict__ u, Reduce red)
 {
+    if (gated_off(red)) return;
     ZWalk z = make_zwalk(g);
     const double beta = ds->beta;
     const double alpha = U ? ds->alpha : 0.0;
@@ -842,6 +844,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_pw_strip(Geom g, PWStrip t, KPtr k
                                                        double* __restrict__ w, double* __restrict__ u, Reduce red,
                                                        const double* __restrict__ rin = nullptr)
 {
+    if (gated_off(red)) return;
     const int lane = threadIdx.x % WAVE;
     // the XCD-aware order is taken per workgroup (workgroups, not waves, go
     // round-robin over the XCDs); a workgroup's waves take consecutive
@@ -1004,6 +1007,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_calc_ur(Geom g, double alpha_v, co
                                                       const double* __restrict__ p, const double* __restrict__ w,
                                                       const double* __restrict__ mi, double* __restrict__ z, Reduce red)
 {
+    if (gated_off(red)) return;
     Strip s = make_strip(g);
     double v[1] = {0.0};
     const double alpha = ds ? ds->alpha : alpha_v;
@@ -1586,6 +1590,17 @@ void KernelTimer::flush()
         pool.push_back(p.b);
     }
     pending.clear();
+    ++flushes;
+}
+
+void KernelTimer::discard_since(const Mark& m)
+{
+    if (m.flushes != flushes) return;
+    for (size_t i = m.n; i < pending.size(); ++i) {
+        pool.push_back(pending[i].a);
+        pool.push_back(pending[i].b);
+    }
+    if (m.n < pending.size()) pending.resize(m.n);
 }
 
 KernelTimer::~KernelTimer()
@@ -1758,7 +1773,8 @@ Finish Chunk::finish_args(Slot slot, int ep, int hist_n)
 {
     const unsigned seq = ++fin_seq[slot];
     return Finish{d_counter, d_slots, ds, d_result + 4 * slot, h_result_dev + 4 * slot, hist_a, hist_b,
-                  h_seq_dev + slot, ep, hist_n, seq, (unsigned)((partials_cap + REDUCE_GROUP - 1) / REDUCE_GROUP)};
+                  h_seq_dev + slot, ep, hist_n, seq, (unsigned)((partials_cap + REDUCE_GROUP - 1) / REDUCE_GROUP),
+                  gate ? 1 : 0, stop_eps, stop_le, stop_gt};
 }
 
 Reduce Chunk::reducer(Slot slot, int ep, int hist_n)
@@ -2111,11 +2127,11 @@ void op_cg_calc_ur_dev(Chunk& c, int n)
     c.close(c.g.nblocks, 1);
 }
 
-void op_cg_calc_r_dev(Chunk& c, int n)
+void op_cg_calc_r_dev(Chunk& c, int n, Slot slot)
 {
     if (n > 0) c.ensure_history(n);
     TIMED(c, K_CG_R, STRIP_LAUNCH(k_cg_calc_ur<false>, c, 0.0, c.ds, (int)c.preconditioner, c.f[F_U], c.f[F_R],
-                                  c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_RRN, n > 0 ? n : -1)));
+                                  c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(slot, EP_RRN, n > 0 ? n : -1)));
     c.close(c.g.nblocks, 1);
 }
 

@@ -105,7 +105,7 @@ const uint2* tb_rowtab(Chunk& c)
 // (beta, error and rro on the device, rrn in S_RRN).  The inner steps run as
 // passes of 10, 5, 2 and 1 levels (largest first), handing sd over through
 // memory between passes; every cell's arithmetic is the same in any split.
-void op_ppcg_outer_tb(Chunk& c, int inner_steps)
+void op_ppcg_outer_tb(Chunk& c, int inner_steps, Slot slot)
 {
     if (!ppcg_tb_supported(c, inner_steps)) throw Error("temporal-blocked PPCG is 2D, no preconditioner");
     if ((int)c.alphas.size() < inner_steps) throw Error("PPCG coefficients shorter than the inner steps");
@@ -130,7 +130,7 @@ void op_ppcg_outer_tb(Chunk& c, int inner_steps)
         a.coef = c.d_coef + 2 * done;
         a.rowtab = tb_rowtab(c);
         a.row0 = TB_ROWTAB0;
-        a.red = last ? c.reducer(S_RRN, EP_RRN) : c.reducer();
+        a.red = last ? c.reducer(slot, EP_RRN) : c.reducer();
         const int ow = 2 * WAVE - 2 * tb::tb_margin(L);
         a.tiles_x = (c.g.nx + ow - 1) / ow;
         a.rows = tb_rows(c.g.ny, a.tiles_x, L, c.simds, tb::TB_UNROLL);
@@ -154,7 +154,7 @@ void op_ppcg_outer_tb(Chunk& c, int inner_steps)
 
 bool ppcg_tb_supported(const Chunk&, int) { return false; }
 std::vector<int> tb_passes(int) { return {}; }
-void op_ppcg_outer_tb(Chunk&, int) { throw Error("temporal-blocked PPCG is implemented for 2D"); }
+void op_ppcg_outer_tb(Chunk&, int, Slot) { throw Error("temporal-blocked PPCG is implemented for 2D"); }
 
 #endif
 

@@ -410,6 +410,7 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
 template <int L, bool FIRST, bool LAST>
 __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
 {
+    if (gated_off(A.red)) return;
     using R = TBRing<L>;
     __shared__ double2 u_ring[R::N * WAVE];
     // one more element: the last lane's east face reads past the last row

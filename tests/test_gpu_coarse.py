@@ -95,6 +95,8 @@ def test_coarse_probe_refuses_bad_grids(hip):
     g = hip.ExtChunk(hip.lib(2), 2, 96, 40)
     for nc in (7, 10, 96, 48):  # not a multiple of 8 / not dividing / B < 2 / y blocks not a multiple of 8
         assert g.L.tl_coarse_probe(g.c, nc, 1, None, None, None, None) == -1
+        assert g.L.tl_last_error()
+        g.L.tl_clear_error()
     g.close()
 
 
