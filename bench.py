@@ -1,29 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark: TeaLeaf's linear solve on one MI355X.
 
-Default workload c4 (the 2D 8192^2 configuration the target is stated on):
-32 seeded random rectangles (splitmix64 seed 0, density log-uniform in
-[0.1, 100], energy uniform in [1e-4, 25]) on a 10x10 domain, eps 1e-10,
-dt 0.004.  A step is one timestep, i.e. one ``tea_leaf()`` solve from its
-first halo update to the finalised energy (the reference's "Tea Solve" time
-per step, tealeaf_2d/diffuse.f90:149).  --warmup W timesteps run untimed
-first, then --steps K are timed; the driver times each solve with a device
+A step is one timestep, i.e. one ``tea_leaf()`` solve from its first halo
+update to the finalised energy (the reference's "Tea Solve" time per step,
+tealeaf_2d/diffuse.f90:149).  --warmup W timesteps run untimed first, then
+--steps K are timed; the driver times each solve with a device
 synchronisation at its end.
 
-`value` is the FP64 PPCG solve (30 CG presteps, 10 inner Chebyshev steps:
-the reference's solver, rebuilt) in seconds per timestep.  Under `forms`
-the same deck and timesteps are solved by the forms the target compares
-against it: mixed-precision PPCG (FP32 inner steps), learned flexible CG
+Default workload c2: the 2D 4096^2 two-rectangle deck (ambient density 100,
+energy 1e-4; rectangles (0,0)-(1,2) and (1,1)-(6,6) of density 0.1) on a
+10x10 domain, eps 1e-10, dt 0.004.  `value` is its FP64 PPCG solve (30 CG
+presteps, 10 inner Chebyshev steps: the reference's solver, rebuilt) in
+seconds per timestep over the K timed steps; `forms` holds the same deck
+and timesteps solved by mixed-precision PPCG and by the flexible CG with
+Jacobi sweeps.  The line also carries the roofline of the FP64 solve's
+dominant kernel (algorithmic bytes per launch over its mean HIP-event
+launch time) and a CPU baseline from the oracle (see cpu_baseline()).
+
+Attached to it, over a fixed --c4-warmup + --c4-steps timesteps (1 + 2) so
+that the run's length does not grow with K: the comparison the target is
+stated on, c4, 8192^2 with 32 seeded random rectangles (splitmix64 seed 0,
+density log-uniform in [0.1, 100], energy uniform in [1e-4, 25]).  FP64
+PPCG against mixed-precision PPCG (FP32 inner steps), learned flexible CG
 (BF16 U-Net + 4 weighted Jacobi sweeps, trained first), the same flexible
 CG without the network (the ablation), and FP64 PPCG accepted on the true
-residual ||u0 - A u||^2 (the equal-accuracy baseline: the reference's
-PPCG stops on its recurrence residual).  `north_star` sums up the speedups
-against the 1.5x target.  The line also carries the roofline of the FP64
-solve's dominant kernel (algorithmic bytes per launch over its mean
-HIP-event launch time) and a CPU baseline from the oracle (see
-cpu_baseline()).  c2 is the 4096^2 two-rectangle deck, c3 the 3D 256^3
-cube, c5 the 4096^2 learned-FCG configuration; --solver picks CG,
-Chebyshev or Jacobi instead of PPCG for c2/c3.
+residual ||u0 - A u||^2 (the equal-accuracy baseline: the reference's PPCG
+stops on its recurrence residual), under `forms_c4`; `north_star` sums up
+the speedups against the 1.5x target.  --workload c4 runs that comparison
+over W + K timesteps as a line of its own (about 15 s per FP64 timestep),
+c3 the 3D 256^3 cube, c5 the 4096^2 learned-FCG configuration; --solver
+picks CG, Chebyshev or Jacobi instead of PPCG for c2/c3.
 """
 from __future__ import annotations
 
@@ -454,7 +460,7 @@ def true_residual_form(ref: dict, tr: dict) -> dict:
     }
 
 
-def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None) -> dict:
+def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None, warmup: int) -> dict:
     """Learned flexible CG on the benchmark deck: the network is trained first
     (train_learner, meshes of --train-n cells per side with dt scaled to
     the benchmark's face coefficients), then applied every --net-every
@@ -465,7 +471,7 @@ def learned_form(args, deck: str, n: int, ref: dict, sweeps: dict | None) -> dic
     print("bench: learned flexible CG on the same deck", file=sys.stderr, flush=True)
     got = run_solve(deck.replace("tl_use_ppcg", f"tl_use_fcg_learned\ntl_fcg_net_every={args.net_every}\ntl_fcg_net_iters={args.net_iters}"
                                         + ("\ntl_fcg_net_restart" if args.net_restart else "")), 2,
-                    args.warmup, timing=False)
+                    warmup, timing=False)
     learner.use(False)
     roof = unet_roofline(learner, n)
     learner.close()
@@ -515,36 +521,37 @@ def copy_rate() -> float:
     return round(L.tl_copy_probe(1 << 27, 10), 1)
 
 
-def north_star(args, n: int) -> dict:
+def c4_comparison(args, n: int, warmup: int, steps: int, kernel_table_run: bool) -> dict:
     """C4: FP64 PPCG against mixed PPCG, learned FCG, the flexible CG without
     the network and FP64 PPCG accepted on the true residual, on the same
-    deck and timesteps."""
-    total = args.warmup + args.steps
-    deck = decks.random_rects_2d(n, args.seed, steps=total, solver="ppcg", presteps=30, inner=10,
+    deck and timesteps (`warmup` untimed, then `steps` timed)."""
+    deck = decks.random_rects_2d(n, args.seed, steps=warmup + steps, solver="ppcg", presteps=30, inner=10,
                                  max_iters=10000, extra=("tl_check_result",))
     what = (f"2D {n}x{n}, 32 random rectangles (splitmix64 seed {args.seed}), eps 1e-10, dt 0.004; "
             f"FP64 PPCG (30 presteps, 10 inner steps) vs mixed PPCG vs learned FCG")
-    print(f"bench: {what}, {args.warmup} + {args.steps} timesteps", file=sys.stderr, flush=True)
-    first = run_solve(deck, 2, 0, max_steps=1)
-    top = max(first["kernels"].items(), key=lambda kv: kv[1]["ms"])[0]
-    print(f"bench: FP64 PPCG, kernel with the most time: {top}", file=sys.stderr, flush=True)
-    ref = run_solve(deck, 2, args.warmup, only=top)
+    print(f"bench: {what}, {warmup} + {steps} timesteps", file=sys.stderr, flush=True)
+    first, top = None, "ppcg_outer"
+    if kernel_table_run:
+        first = run_solve(deck, 2, 0, max_steps=1)
+        top = max(first["kernels"].items(), key=lambda kv: kv[1]["ms"])[0]
+        print(f"bench: FP64 PPCG, kernel with the most time: {top}", file=sys.stderr, flush=True)
+    ref = run_solve(deck, 2, warmup, only=top)
     forms = {}
     print("bench: mixed-precision PPCG", file=sys.stderr, flush=True)
-    mix = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg_mixed"), 2, args.warmup, only="ppcg_outer_mixed")
+    mix = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg_mixed"), 2, warmup, only="ppcg_outer_mixed")
     forms["mixed_ppcg"] = mixed_form(ref, mix)
     print("bench: FP64 PPCG accepted on the true residual", file=sys.stderr, flush=True)
-    tr = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg\ntl_ppcg_true_residual"), 2, args.warmup, timing=False)
+    tr = run_solve(deck.replace("tl_use_ppcg", "tl_use_ppcg\ntl_ppcg_true_residual"), 2, warmup, timing=False)
     forms["fp64_ppcg_true_residual"] = true_residual_form(ref, tr)
     print("bench: flexible CG, Jacobi sweeps only", file=sys.stderr, flush=True)
-    sw = run_solve(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network"), 2, args.warmup,
-                   timing=False)
+    sw = run_solve(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network"), 2, warmup, timing=False)
     forms["fcg_sweeps_only"] = fcg_form(ref, sw)
     if not args.no_learned:
-        forms["learned_fcg"] = learned_form(args, deck, n, ref, forms["fcg_sweeps_only"])
+        forms["learned_fcg"] = learned_form(args, deck, n, ref, forms["fcg_sweeps_only"], warmup)
     t_ref, t_true = ref["s_per_step"], tr["s_per_step"]
-    ns = {"target_speedup": 1.5, "fp64_ppcg_s": round(t_ref, 6), "fp64_ppcg_true_residual_s": round(t_true, 6)}
-    for key in ("mixed_ppcg", "learned_fcg"):
+    ns = {"target_speedup": 1.5, "cells": n * n, "warmup": warmup, "steps": steps, "fp64_ppcg_s": round(t_ref, 6),
+          "fp64_ppcg_true_residual_s": round(t_true, 6)}
+    for key in ("mixed_ppcg", "learned_fcg", "fcg_sweeps_only"):
         if key in forms:
             t = forms[key]["s_per_step"]
             ns[key] = {"s": t, "vs_fp64_ppcg": round(t_ref / t, 3), "vs_fp64_ppcg_true_residual": round(t_true / t, 3)}
@@ -553,6 +560,14 @@ def north_star(args, n: int) -> dict:
         # what the trained network itself adds: the learned solve against the
         # same flexible CG with the network switched off
         ns["learned_fcg"]["network_gain_vs_sweeps_only"] = forms["learned_fcg"].get("gain_vs_sweeps_only")
+    return {"deck": deck, "what": what, "first": first, "ref": ref, "forms": forms, "north_star": ns}
+
+
+def north_star(args, n: int) -> dict:
+    """The c4 workload's line: the C4 comparison over --warmup + --steps
+    timesteps, FP64 PPCG as `value`."""
+    r = c4_comparison(args, n, args.warmup, args.steps, True)
+    deck, what, first, ref, forms, ns = r["deck"], r["what"], r["first"], r["ref"], r["forms"], r["north_star"]
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle sample)", file=sys.stderr, flush=True)
@@ -573,9 +588,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed timesteps (default 3 for c4, 9 otherwise)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed timesteps before them")
-    ap.add_argument("--workload", default="c4", choices=["c2", "c3", "c4", "c5"],
-                    help="c4: 2D 8192^2, 32 random rectangles (the target's configuration); c2: 2D 4096^2 two "
-                         "rectangles; c3: 3D 256^3 cube; c5: 2D 4096^2 learned FCG on random rectangles and circles")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default): 2D 4096^2 two rectangles, with the c4 comparison attached; c4: 2D 8192^2, "
+                         "32 random rectangles (the target's configuration) over --warmup + --steps timesteps; "
+                         "c3: 3D 256^3 cube; c5: 2D 4096^2 learned FCG on random rectangles and circles")
+    ap.add_argument("--c4-warmup", type=int, default=1, help="c2: untimed 8192^2 timesteps of the attached comparison")
+    ap.add_argument("--c4-steps", type=int, default=2, help="c2: timed 8192^2 timesteps of the attached comparison")
+    ap.add_argument("--c4-n", type=int, default=8192, help="c2: cells per side of the attached comparison")
+    ap.add_argument("--no-c4", action="store_true", help="c2: skip the attached 8192^2 comparison")
     ap.add_argument("--train-decks", type=int, default=8, help="learned form: layouts that give training pairs")
     ap.add_argument("--train-n", type=int, default=None, help="learned form: training mesh (default min(n, 2048))")
     ap.add_argument("--snapshots", type=int, default=16, help="learned form: iterates sampled per solve phase")
@@ -754,6 +774,15 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline and args.workload == "c2" and args.solver == "ppcg":
         cpu = cpu_baseline(deck, n, args.cpu_threads, ref["timed"])
+    c4 = {}
+    if args.workload == "c2" and args.solver == "ppcg" and not args.no_c4 and args.c4_steps > 0:
+        # the target's comparison at 8192^2 over a fixed number of timesteps,
+        # so that the run's length does not grow with --steps
+        r = c4_comparison(args, args.c4_n, args.c4_warmup, args.c4_steps, False)
+        c4 = {"north_star": r["north_star"], "forms_c4": r["forms"],
+              "c4_workload": r["what"], "c4_iterations": [st["iters"] for st in r["ref"]["timed"]],
+              "c4_true_rrn": [st["exact_error"] for st in r["ref"]["timed"]],
+              "c4_roofline": roofline_of(r["ref"], args.c4_n)}
 
     timed = ref["timed"]
     out = line(ref["s_per_step"], args, f"{what}, FP64, eps 1e-10, dt 0.004, presteps 30, inner steps 10",
@@ -764,7 +793,7 @@ def main() -> int:
                kernels=kernel_table(first),
                kernels_note="every kernel, HIP events over the first timestep (a run of its own); roofline: "
                             "events around the roofline kernel only, over the timed steps",
-               copy_gbs=copy_rate(), roofline=roofline_of(ref, n), forms=forms, cpu_baseline=cpu)
+               copy_gbs=copy_rate(), roofline=roofline_of(ref, n), forms=forms, cpu_baseline=cpu, **c4)
     print(json.dumps(out), flush=True)
     return 0
 
