@@ -751,7 +751,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                     // launched during the previous iteration
                 } else if (mixed) {
                     op_cg_calc_w_dev(c, p_pending);
-                    TL_TIMED(c, K_PPCG_MIXED, op_ppcg_outer_mixed(c, d.ppcg_inner_steps));
+                    op_ppcg_outer_mixed(c, d.ppcg_inner_steps);
                     ++si.mixed_steps;
                 } else if (tb) {
                     // calc_ur, init_sd, the inner steps and ||r||^2 in one sweep
@@ -760,7 +760,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                         c.stop_eps = d.eps;
                         c.stop_gt = ppcg_gt;
                     }
-                    TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n)));
+                    op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n));
                 } else {
                     op_cg_calc_w_dev(c, p_pending);
                     op_ppcg_calc_ur_dev(c);
@@ -787,7 +787,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
                         c.stop_eps = d.eps;
                         c.stop_gt = ppcg_gt;
                         op_cg_calc_w_dev(c, true);
-                        TL_TIMED(c, K_PPCG_OUTER, op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n + 1)));
+                        op_ppcg_outer_tb(c, d.ppcg_inner_steps, rrn_slot(n + 1));
                         c.gate = false;
                     }
                     error = c.result(rrn_slot(n));

@@ -443,13 +443,15 @@ void op_ppcg_outer_mixed(Chunk& c, int inner_steps)
     a.rows = tb_rows(c.g.ny, a.tiles_x, L, MX_WAVES_PER_SIMD * c.simds);
     a.nblocks = (long)a.tiles_x * ((c.g.ny + a.rows - 1) / a.rows);
     if (a.nblocks > c.partials_cap) throw Error("mixed-precision PPCG: more waves than reduction partials");
-    switch (L) {
-#define TL_MX_CASE(n) \
-    case n: launch<n>(c, a); break;
-        TL_MX_CASE(1) TL_MX_CASE(3) TL_MX_CASE(10)
-#undef TL_MX_CASE
-    default: throw Error("mixed-precision PPCG pass needs 1, 3 or 10 levels");
-    }
+    if (L != 1 && L != 3 && L != 10) throw Error("mixed-precision PPCG pass needs 1, 3 or 10 levels");
+    // the timer brackets the pass alone, not the finish of its reduction
+    TL_TIMED(c, K_PPCG_MIXED, {
+        switch (L) {
+        case 1: launch<1>(c, a); break;
+        case 3: launch<3>(c, a); break;
+        default: launch<10>(c, a); break;
+        }
+    });
     c.close(a.nblocks, 1);
     std::swap(c.f[F_R], c.r_alt);
 }

@@ -137,12 +137,15 @@ void op_ppcg_outer_tb(Chunk& c, int inner_steps, Slot slot)
         const long tiles_y = (c.g.ny + a.rows - 1) / a.rows;
         a.nblocks = (long)a.tiles_x * tiles_y;
         if (a.nblocks > c.partials_cap) throw Error("temporal-blocked PPCG: more waves than reduction partials");
-        switch (L) {
-        case 10: tb::tb_launch_10(c, a, first, last); break;
-        case 5: tb::tb_launch_5(c, a, first, last); break;
-        case 2: tb::tb_launch_2(c, a, first, last); break;
-        default: tb::tb_launch_1(c, a, first, last); break;
-        }
+        // the timer brackets each pass alone, not the finish of its reduction
+        TL_TIMED(c, K_PPCG_OUTER, {
+            switch (L) {
+            case 10: tb::tb_launch_10(c, a, first, last); break;
+            case 5: tb::tb_launch_5(c, a, first, last); break;
+            case 2: tb::tb_launch_2(c, a, first, last); break;
+            default: tb::tb_launch_1(c, a, first, last); break;
+            }
+        });
         if (last) c.close(a.nblocks, 1);
         std::swap(c.f[F_R], c.r_alt);
         if (!last) std::swap(c.f[F_SD], c.sd_alt);
