@@ -628,6 +628,9 @@ def main() -> int:
                     help="only run fused PPCG outer iterations on an n^2 chunk (counter runs)")
     ap.add_argument("--inner-probe", action="store_true",
                     help="only run 3D two-step PPCG inner sweeps on an n^3 chunk (counter runs)")
+    ap.add_argument("--unet-probe", action="store_true",
+                    help="only time the U-Net forward pass (untrained weights) at 4096^2 and 8192^2, with its 3x3 "
+                         "convolutions staged in LDS and reading through L2")
     ap.add_argument("--stream-probe", action="store_true",
                     help="only run the streaming probes of the attainable HBM rate")
     ap.add_argument("--cpu-check", action="store_true",
@@ -657,6 +660,16 @@ def main() -> int:
         return 0
     if args.inner_probe:
         print(json.dumps(inner_probe(args.n or 256, 9)), flush=True)
+        return 0
+    if args.unet_probe:
+        learner = tealeaf_hip.Learner(seed=3)
+        for staged in (1, 0):
+            tealeaf_hip.lib(2).tl_learn_conv_staging(staged)
+            for m in (4096, 8192):
+                roof = unet_roofline(learner, m)
+                print(json.dumps({"probe": "unet forward", "n": m, "conv_staging": bool(staged), **roof}), flush=True)
+        tealeaf_hip.lib(2).tl_learn_conv_staging(1)
+        learner.close()
         return 0
     if args.stream_probe:
         import ctypes

@@ -311,6 +311,10 @@ double tl_learn_time_forward(void* learner, int n, int h, int w, int reps, doubl
  * written once as stored (BF16 activations with padded channels, FP32 planes
  * in and out): the HBM side of the network's roofline. */
 double tl_learn_forward_bytes(int n, int h, int w);
+/* The network's 3x3 convolutions: 1 (default) stages each workgroup's input
+ * tile in LDS, 0 reads it through L2 once per tap; outputs and gradients are
+ * bitwise the same either way.  No counterpart in the reference. */
+void tl_learn_conv_staging(int on);
 
 /* Error handling of the ext_*_ calls. */
 void tl_set_abort_on_error(int on);

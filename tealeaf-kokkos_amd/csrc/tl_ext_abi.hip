@@ -462,6 +462,7 @@ double tl_learn_time_forward(void* h, int n, int hh, int ww, int reps, double* m
             TL_HIP(hipMalloc(&p.d[k], sizeof(float) * px));
             TL_HIP(hipMemset(p.d[k], 0, sizeof(float) * px));
         }
+        TL_HIP(hipDeviceSynchronize());  // the fills, before the network's own stream reads the planes
         unet_apply(u, p.d[0], p.d[1], p.d[2], n, hh, ww);
         hipEvent_t e0, e1;
         TL_HIP(hipEventCreate(&e0));
@@ -479,6 +480,11 @@ double tl_learn_time_forward(void* h, int n, int hh, int ww, int reps, double* m
     });
     return ms;
 }
+
+// The network's 3x3 convolutions: 1 stages each workgroup's input tile in LDS
+// (the default), 0 reads it through L2 once per tap; the outputs and
+// gradients are bitwise the same.  Applies to every network of the library.
+void tl_learn_conv_staging(int on) { set_conv_staging(on != 0); }
 
 // Bytes one network forward pass on n x h x w planes moves, each tensor read
 // or written once as stored (BF16 activations, FP32 planes in and out).

@@ -143,6 +143,132 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
     }
 }
 
+// The same convolution with the input staged in LDS: a workgroup copies the
+// (TR + 2) input rows x 66 pixels x cin channels that its TR output rows x
+// 64 pixels read (zero outside the image, ReLU' applied for MASK) into LDS
+// once.  Its 4 waves split the TR rows and, 4 / TR ways, the cotiles: a
+// wave computes a whole 64-pixel row for its cotiles, so each weight it
+// loads serves 64 pixels, with B read from LDS and the next K chunk's
+// weights loaded while the current one's MFMAs run.  Every accumulator sees
+// the same MFMA sequence as in k_conv3x3, so the output is bitwise the same;
+// the input is read from L2 (TR + 2) / TR times instead of 9 times per
+// cotile.  A staged pixel holds cin + 8 channels, so that the 16-byte reads
+// of 8 consecutive lanes (8 consecutive pixels) fall in distinct banks.
+template <int NT, int TR, bool RELU, bool MASK>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvArgs a)
+{
+    extern __shared__ uint4 stage[];
+    const int pu = a.cin / 8 + 1;  // 16-byte units per staged pixel
+    const int xt = (int)(blockIdx.x % a.xtiles);
+    const long rt = blockIdx.x / a.xtiles;
+    const int rtiles = (a.h + TR - 1) / TR;
+    const int n = (int)(rt / rtiles), y0 = (int)(rt % rtiles) * TR;
+    const int px0 = xt * 64;
+    const int c8 = a.cin / 8;
+    const int units = (TR + 2) * 66 * c8;
+    // SB loads of 16 bytes in flight per thread before any is stored
+    constexpr int SB = 8;
+    for (int u0 = 0; u0 < units; u0 += 256 * SB) {
+        uint4 v[SB], mv[SB];
+        int at[SB];
+#pragma unroll
+        for (int q = 0; q < SB; ++q) {
+            const int u = u0 + (int)threadIdx.x + 256 * q;
+            v[q] = mv[q] = make_uint4(0u, 0u, 0u, 0u);
+            at[q] = -1;
+            if (u < units) {
+                const int c = u % c8, pq = u / c8;
+                const int sx = pq % 66, sr = pq / 66;
+                const int yy = y0 - 1 + sr, xx = px0 - 1 + sx;
+                at[q] = (sr * 66 + sx) * pu + c;
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+                    const long off = (((long)n * a.h + yy) * a.w + xx) * a.cin + 8 * c;
+                    v[q] = ld16(a.x + off);
+                    if (MASK) mv[q] = ld16(a.m + off);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SB; ++q)
+            if (at[q] >= 0) stage[at[q]] = MASK ? relu_mask(v[q], mv[q]) : v[q];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int j = lane % 16, kq = lane / 16;
+    constexpr int G = 4 / TR;  // waves per output row, each taking every G-th cotile
+    const int r = wave / G;
+    const int y = y0 + r;
+    if (y >= a.h) return;  // no barrier follows
+    for (int cot = wave % G; cot < a.cotiles; cot += G) {
+        const int co0 = cot * 16 * NT;
+        const u16* wrow = a.wt + (long)(co0 + j) * a.kp + kq * 8;
+        f32x4 acc[4][NT];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        uint4 wv[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) wv[nt] = ld16(wrow + (long)16 * nt * a.kp);
+        // this lane's k = kc + 8 kq as (tap, ci), advanced by 32 per chunk
+        int ci = kq * 8, tap = 0;
+        while (ci >= a.cin) {
+            ci -= a.cin;
+            ++tap;
+        }
+        int tdy = tap / 3 - 1, tdx = tap % 3 - 1;
+        for (int kc = 0; kc < a.kp; kc += 32) {
+            bf16x8 av[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) av[nt] = as_bf8(wv[nt]);
+            if (kc + 32 < a.kp) {
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) wv[nt] = ld16(wrow + (long)16 * nt * a.kp + kc + 32);
+            }
+            const bool kin = tap < 9;  // k < 9 cin: the padded k carry zero weights and inputs
+            const uint4* srow = stage + ((r + 1 + (kin ? tdy : 0)) * 66 + 1 + tdx + j) * pu + (ci >> 3);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const uint4 v = kin ? srow[16 * mt * pu] : make_uint4(0u, 0u, 0u, 0u);
+                const bf16x8 bv = as_bf8(v);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[nt], bv, acc[mt][nt], 0, 0, 0);
+            }
+            ci += 32;
+            while (ci >= a.cin) {
+                ci -= a.cin;
+                ++tap;
+                if (++tdx > 1) {
+                    tdx = -1;
+                    ++tdy;
+                }
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int px = px0 + 16 * mt + j;
+            if (px >= a.w) continue;
+            u16* out = a.y + (((long)n * a.h + y) * a.w + px) * a.cout;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = co0 + 16 * nt + 4 * kq;
+                u16 h4[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float sv = acc[mt][nt][v] + (a.bias ? a.bias[co + v] : 0.0f);
+                    if (RELU) sv = sv > 0.0f ? sv : 0.0f;
+                    h4[v] = f2bf(sv);
+                }
+                uint2 pk;
+                pk.x = (uint32_t)h4[0] | ((uint32_t)h4[1] << 16);
+                pk.y = (uint32_t)h4[2] | ((uint32_t)h4[3] << 16);
+                *reinterpret_cast<uint2*>(out + co) = pk;
+            }
+        }
+    }
+}
+
 struct WGradArgs {
     const u16* x;   // layer input [n][h][w][cin]
     const u16* dy;  // gradient at the layer output [n][h][w][cout]
@@ -581,7 +707,75 @@ void check_launch(const char* what)
     if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+bool g_conv_staging = true;
+
+// LDS bytes of a staged tile of TR output rows
+long staged_bytes(int tr, int cin) { return (long)(tr + 2) * 66 * (cin + 8) * 2; }
+
+// LDS a workgroup may use on the current device
+long lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) return 65536;
+    return v;
+}
+
+template <int NT, int TR, bool RELU, bool MASK>
+void launch_staged(const ConvArgs& a, hipStream_t s)
+{
+    const long bytes = staged_bytes(TR, a.cin);
+    static bool raised = false;  // per instantiation: allow more than the default dynamic LDS
+    if (!raised && bytes > 65536) {
+        TL_HIP(hipFuncSetAttribute((const void*)k_conv3x3_lds<NT, TR, RELU, MASK>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit()));
+        raised = true;
+    }
+    const long blocks = (long)a.n * ((a.h + TR - 1) / TR) * a.xtiles;
+    hipLaunchKernelGGL((k_conv3x3_lds<NT, TR, RELU, MASK>), dim3((unsigned)blocks), dim3(256), (size_t)bytes, s, a);
+}
+
+template <int NT, bool RELU, bool MASK>
+void launch_conv_nt(ConvArgs& a, int tr, hipStream_t s)
+{
+    switch (tr) {
+    case 4: launch_staged<NT, 4, RELU, MASK>(a, s); return;
+    case 2: launch_staged<NT, 2, RELU, MASK>(a, s); return;
+    case 1: launch_staged<NT, 1, RELU, MASK>(a, s); return;
+    default: break;
+    }
+    a.waves = (long)a.n * a.h * a.xtiles * a.cotiles;
+    hipLaunchKernelGGL((k_conv3x3<NT, RELU, MASK>), dim3((unsigned)((a.waves + 3) / 4)), dim3(256), 0, s, a);
+}
+
+// One 3x3 convolution of ConvArgs a (xtiles set; cout a multiple of 16 * nt):
+// the staged kernel where its tile fits the LDS, with its column tiles
+// narrowed until the cotiles split evenly over the waves of a row, else the
+// one that reads through L2.
+template <bool RELU, bool MASK>
+void launch_conv(ConvArgs& a, int nt, hipStream_t s)
+{
+    static const long limit = lds_limit();
+    int tr = 0;
+    if (g_conv_staging && (long)a.n * a.h * a.xtiles < (1L << 31) && a.cin % 8 == 0) {
+        if (staged_bytes(4, a.cin) <= std::min(limit, 65536L)) tr = 4;
+        else if (staged_bytes(2, a.cin) <= std::min(limit, 81920L)) tr = 2;
+        else if (staged_bytes(1, a.cin) <= limit) tr = 1;
+    }
+    if (tr) {
+        const int g = 4 / tr;
+        while (nt > 1 && (a.cout / (16 * nt)) % g) nt /= 2;
+        if ((a.cout / (16 * nt)) % g) tr = 0;
+    }
+    a.cotiles = a.cout / (16 * nt);
+    if (nt == 4) launch_conv_nt<4, RELU, MASK>(a, tr, s);
+    else if (nt == 2) launch_conv_nt<2, RELU, MASK>(a, tr, s);
+    else launch_conv_nt<1, RELU, MASK>(a, tr, s);
+}
+
 }  // namespace
+
+void set_conv_staging(bool on) { g_conv_staging = on; }
 
 // ---------------------------------------------------------------------------
 // Network
@@ -671,13 +865,24 @@ T* dalloc(long n)
     T* p = nullptr;
     TL_HIP(hipMalloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)));
     TL_HIP(hipMemset(p, 0, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+    // the fill runs on the null stream, which is not ordered with the
+    // network's non-blocking stream: finish it before a kernel writes p
+    TL_HIP(hipDeviceSynchronize());
     return p;
+}
+
+// Frees a buffer that kernels still queued on the network's stream may use.
+void dfree(void* p)
+{
+    if (!p) return;
+    TL_HIP(hipDeviceSynchronize());
+    (void)hipFree(p);
 }
 
 void grow(UBuf& b, long n)
 {
     if (n <= b.cap) return;
-    (void)hipFree(b.d);
+    dfree(b.d);
     b.d = dalloc<u16>(n);
     b.cap = n;
 }
@@ -821,15 +1026,8 @@ void UNet::conv_fwd(int l, const u16* x, u16* y, int n, int h, int w)
     const int nt = col_tiles(k.cout);
     a.cotiles = k.cout / (16 * nt);
     a.xtiles = (w + 63) / 64;
-    a.waves = (long)n * h * a.xtiles * a.cotiles;
-    const dim3 grid((unsigned)((a.waves + 3) / 4)), block(256);
-    if (k.relu) {
-        if (nt == 4) hipLaunchKernelGGL((k_conv3x3<4, true, false>), grid, block, 0, stream, a);
-        else if (nt == 2) hipLaunchKernelGGL((k_conv3x3<2, true, false>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((k_conv3x3<1, true, false>), grid, block, 0, stream, a);
-    } else {
-        hipLaunchKernelGGL((k_conv3x3<1, false, false>), grid, block, 0, stream, a);
-    }
+    if (k.relu) launch_conv<true, false>(a, nt, stream);
+    else launch_conv<false, false>(a, nt, stream);
     check_launch("conv forward");
 }
 
@@ -868,17 +1066,8 @@ void UNet::conv_bwd(int l, const u16* x, const u16* y, const u16* dy, u16* dx, i
     const int nt = col_tiles(k.cin);
     a.cotiles = k.cin / (16 * nt);
     a.xtiles = (w + 63) / 64;
-    a.waves = (long)n * h * a.xtiles * a.cotiles;
-    const dim3 g2((unsigned)((a.waves + 3) / 4));
-    if (k.relu) {
-        if (nt == 4) hipLaunchKernelGGL((k_conv3x3<4, false, true>), g2, block, 0, stream, a);
-        else if (nt == 2) hipLaunchKernelGGL((k_conv3x3<2, false, true>), g2, block, 0, stream, a);
-        else hipLaunchKernelGGL((k_conv3x3<1, false, true>), g2, block, 0, stream, a);
-    } else {
-        if (nt == 4) hipLaunchKernelGGL((k_conv3x3<4, false, false>), g2, block, 0, stream, a);
-        else if (nt == 2) hipLaunchKernelGGL((k_conv3x3<2, false, false>), g2, block, 0, stream, a);
-        else hipLaunchKernelGGL((k_conv3x3<1, false, false>), g2, block, 0, stream, a);
-    }
+    if (k.relu) launch_conv<false, true>(a, nt, stream);
+    else launch_conv<false, false>(a, nt, stream);
     check_launch("conv backward data");
 }
 
@@ -926,17 +1115,17 @@ float UNet::loss_and_backward(int n, int h, int w, const float* kx, const float*
     TL_HIP(hipMemsetAsync(loss, 0, sizeof(float), stream));
     if (kx && ky) {
         if (4 * p0 > dbuf_cap) {
-            (void)hipFree(dbuf);
+            dfree(dbuf);
             dbuf = dalloc<float>(4 * p0);
             dbuf_cap = 4 * p0;
         }
+        if (2L * n > sums_cap) {
+            dfree(sums);
+            sums = dalloc<float>(2L * n);
+            sums_cap = 2L * n;
+        }
         TL_HIP(hipMemsetAsync(sums, 0, sizeof(float) * 2, stream));
         if (smooth > 0) {
-            if (2L * n > sums_cap) {
-                (void)hipFree(sums);
-                sums = dalloc<float>(2L * n);
-                sums_cap = 2L * n;
-            }
             hipLaunchKernelGGL(k_anorm_smooth_sums, dim3(n), dim3(256), 0, stream, act[OUT].d, 16, target, kx, ky,
                                dbuf, p0, h, w, smooth, omega, sums);
             hipLaunchKernelGGL(k_anorm_smooth_grad, dim3(n), dim3(256), 0, stream, kx, ky, dbuf, p0, grad[OUT].d, 16,
@@ -1029,7 +1218,7 @@ float unet_loss_grad(UNet& u, const float* r, const float* f, const float* targe
     u.ensure(n, h, w);
     const long p = (long)n * h * w;
     if (p > u.target_cap) {
-        (void)hipFree(u.target);
+        dfree(u.target);
         u.target = dalloc<float>(p);
         u.target_cap = p;
     }

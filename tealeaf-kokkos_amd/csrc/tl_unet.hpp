@@ -35,6 +35,9 @@ long unet_param_count(const UNet& u);
 // Multiply-accumulates of one forward pass over n x h x w pixels.
 double unet_forward_macs(int n, int h, int w);
 double unet_forward_bytes(int n, int h, int w);  // HBM bytes of one forward pass, each tensor once
+// Whether the 3x3 convolutions stage their input tile in LDS (the default)
+// or read it through L2 per tap; both give bitwise the same output.
+void set_conv_staging(bool on);
 // Parameters and gradients in layer order, each layer W[cout][9][cin] then b[cout].
 void unet_get_params(const UNet& u, float* host);
 void unet_set_params(UNet& u, const float* host);

@@ -95,6 +95,7 @@ def lib(dim: int) -> ctypes.CDLL:
                                                      fp]),
             "tl_learn_apply": (ctypes.c_int, [vp, fp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "tl_learn_forward_bytes": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "tl_learn_conv_staging": (None, [ctypes.c_int]),
             "tl_reduce_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, dp, dp]),
             "tl_set_single_pass": (None, [ctypes.c_int]),
             "tl_coarse_probe": (ctypes.c_int, [_P_INT, ctypes.c_int, ctypes.c_int, fp, fp, fp, dp]),
