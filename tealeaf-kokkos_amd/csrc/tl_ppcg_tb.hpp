@@ -243,7 +243,12 @@ struct TBRing {
 // LDS rings of level s + 1 are read before level s computes, so their
 // latency overlaps level s's arithmetic (the rows differ, so no write of
 // level s is missed).  `tab` is the row table entry of row y + TB_PF - PH.
-template <int L, bool FIRST, bool LAST, int PH>
+// WARM: the first block of a wave (y = ylo + PH = Y0 - L + PH).  The strip's
+// output rows Y0.. depend on level s only at rows >= Y0 - (L - s), i.e. from
+// round PH = 2 s on, so the arithmetic of the levels s > PH / 2 is left out;
+// every load and store stays, so that the memory operations in flight at the
+// loop head are those of every other block.
+template <int L, bool FIRST, bool LAST, int PH, bool WARM>
 __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y, ConstU* tab)
 {
     using R = TBRing<L>;
@@ -299,14 +304,15 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         const double kxe1 = nkxe;
         double2 u = nu;
         const double a = st.ca[s - 1], b = st.cb[s - 1];
-        if (s < L) {
+        const bool skip = WARM && 2 * s > PH;  // a constant once the loop is unrolled
+        if (s < L && !(WARM && 2 * (s + 1) > PH)) {
             nkx = kxr[R::row(PH, s + 1) * WAVE];
             nky = kyr[R::row(PH, s + 1) * WAVE];
             nu = ur[R::row(PH, s + 1) * WAVE];
             nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
         }
-        double rn0, rn1;
-        {
+        double rn0 = 0.0, rn1 = 0.0;
+        if (!skip) {
             const double c0 = st.sdw[s - 1][S1][0], c1 = st.sdw[s - 1][S1][1];
             const double s0 = st.sdw[s - 1][S2][0], s1 = st.sdw[s - 1][S2][1];
             const double n0 = st.sdw[s - 1][S0][0], n1 = st.sdw[s - 1][S0][1];
@@ -351,16 +357,16 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     }
 }
 
-template <int L, bool FIRST, bool LAST, int PH>
+template <int L, bool FIRST, bool LAST, int PH, bool WARM>
 __device__ __forceinline__ void tb_rounds(const TBArgs& A, const TBCtx& c, TBState<L>& st, int y, ConstU* tab)
 {
     if constexpr (PH < TB_UNROLL) {
-        tb_round<L, FIRST, LAST, PH>(A, c, st, y + PH, tab);
+        tb_round<L, FIRST, LAST, PH, WARM>(A, c, st, y + PH, tab);
         // the scheduler may not move work between rounds (interleaving them
         // raises the register count, and the copies it then makes of
         // registers with loads in flight wait for those loads)
         __builtin_amdgcn_sched_barrier(0);
-        tb_rounds<L, FIRST, LAST, PH + 1>(A, c, st, y, tab);
+        tb_rounds<L, FIRST, LAST, PH + 1, WARM>(A, c, st, y, tab);
     }
 }
 
@@ -398,8 +404,9 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
     // rows ylo + 12 k ... start each block, so that the LDS ring rows and
     // register window slots of a round depend only on its place in the block
     const int yend = c.yhi + L;
-    for (int y = c.ylo; y <= yend; y += TB_UNROLL) {
-        tb_rounds<L, FIRST, LAST, 0>(A, c, st, y, tab + 2 * (y - c.ylo + TB_PF));
+    tb_rounds<L, FIRST, LAST, 0, true>(A, c, st, c.ylo, tab + 2 * TB_PF);
+    for (int y = c.ylo + TB_UNROLL; y <= yend; y += TB_UNROLL) {
+        tb_rounds<L, FIRST, LAST, 0, false>(A, c, st, y, tab + 2 * (y - c.ylo + TB_PF));
     }
     if (LAST) {
         double v[1] = {st.acc};
