@@ -183,6 +183,17 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
  * (fused = 1; 2D, 4 sweeps) or one launch per sweep (fused = 0).  Returns 0,
  * or -1 with tl_last_error() set. */
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
+/* The same with a weight per sweep, weights[0 .. sweeps) (1 to 16 sweeps,
+ * each weight positive): sweep s applies z += (r - A z) ((w_0 / D) f_s),
+ * f_s = w_s / w_0, the form the Chebyshev-weighted sweeps
+ * (tl_fcg_cheby) take.  No counterpart in the reference. */
+int tl_fcg_smooth_weights(const int* chunk, int sweeps, const double* weights, int from_zero, int fused);
+/* The weights the flexible CG's sweeps use, weights[0 .. sweeps): omega
+ * each, or with cheby_ratio > 1 (tl_fcg_cheby_ratio; tl_fcg_cheby is 30)
+ * 1 / theta_k, theta_k the roots of the degree-`sweeps` Chebyshev polynomial
+ * on [2 / cheby_ratio, 2] in decreasing order.  No GPU work.  No
+ * counterpart in the reference.  Returns 0, or -1 (tl_last_error()). */
+int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* weights);
 /* The sweeps-only flexible CG's preconditioner pass with the previous
  * step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four
  * weighted Jacobi sweeps on that r from zero, in one pass; sums[0..3) =

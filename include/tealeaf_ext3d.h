@@ -182,6 +182,8 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
  * are 2D only (see tealeaf_ext2d.h): the 3D library exports the symbol, and a
  * call returns -1 with tl_last_error() set. */
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
+int tl_fcg_smooth_weights(const int* chunk, int sweeps, const double* weights, int from_zero, int fused);
+int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* weights);
 /* The sweeps-only flexible CG's preconditioner pass with the previous
  * step's r update folded in (2D, 4 sweeps): r -= alpha w, then z = the four
  * weighted Jacobi sweeps on that r from zero, in one pass; sums[0..3) =

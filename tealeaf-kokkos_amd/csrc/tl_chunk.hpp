@@ -346,7 +346,9 @@ Learner* active_learner();
 // the device scalars): true when it reached eps on the true residual, false
 // when it stalled (the caller then solves with FP64 PPCG).
 // z = s N(r / s) (network on, every net_every-th iteration) or 0, then
-// `smooth` weighted-Jacobi sweeps z += omega D^-1 (r - A z).
+// `smooth` weighted-Jacobi sweeps z += w_s D^-1 (r - A z): the Chebyshev
+// step weights of [2 / cheby_ratio, 2] (fcg_sweep_weights), or w_s = omega
+// with cheby_ratio = 0.
 struct FcgOptions {
     bool network = true;
     int net_every = 1;
@@ -354,6 +356,7 @@ struct FcgOptions {
     bool net_restart = false;  // start a new recurrence (p = z) after the last network iteration
     int smooth = 4;
     double omega = 0.8;
+    double cheby_ratio = 100.0;
     // the network as the coarse-grid solve of a two-grid preconditioner
     // (tl_coarse.hip), in every iteration: sweeps, restriction, network,
     // prolongation, sweeps
@@ -383,13 +386,24 @@ void coarse_mean_diff(const Geom& g, const CoarseGrid& cg, const double* a, cons
                       hipStream_t s);
 bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, int max_iters, int& iters,
                        double& error);
-// The flexible CG's preconditioner sweeps (weighted Jacobi on A z = r, into
-// F_Z): one temporal-blocked pass where supported and allowed.
-// With dot_w the pass also leaves the first phase of z . w in the reduction
-// partials and returns their count (else 0).
-long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused = true, bool dot_w = false);
+// The sweep weights of the options, w[0 .. opt.smooth - 1].  Chebyshev:
+// w_k = 1 / theta_k, theta_k the roots of the degree-`smooth` Chebyshev
+// polynomial on [2 / cheby_ratio, 2] in decreasing order
+// ((hi + lo) / 2 + (hi - lo) / 2 cos((2 k + 1) pi / (2 smooth))).  The
+// residual polynomial prod_k (1 - x / theta_k) then stays below 1 in
+// magnitude on that interval, and D^-1 A's spectrum lies in (0, 2).
+void fcg_sweep_weights(const FcgOptions& opt, double* w);
+// Sweep q's weight over the first one, the factor the sweep kernels apply to
+// w_1 / D (exactly 1 for q = 0 and for equal weights).
+inline double sweep_ratio(const double* w, int q) { return w[q] / w[0]; }
+// The flexible CG's preconditioner sweeps (weighted Jacobi on A z = r with
+// weights[0 .. sweeps - 1], into F_Z): one temporal-blocked pass where
+// supported and allowed.  With dot_w the pass also leaves the first phase
+// of z . w in the reduction partials and returns their count (else 0).
+long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, bool allow_fused = true,
+                bool dot_w = false);
 bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);
-long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps,
+long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double* weights, int sweeps,
                       const double* w = nullptr, double* r_out = nullptr);
 
 // The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps

@@ -69,8 +69,9 @@ struct Deck {
     // Flexible CG with the learned preconditioner (no counterpart in the
     // reference), FP64 PPCG as its fallback solver.
     bool use_fcg = false;
-    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_net_every=<m>, tl_fcg_net_iters=<m>,
-                     // tl_fcg_no_network, tl_fcg_net_restart, tl_fcg_coarse
+    FcgOptions fcg;  // tl_fcg_smooth=<sweeps>, tl_fcg_omega=<w>, tl_fcg_cheby, tl_fcg_cheby_ratio=<a>,
+                     // tl_fcg_net_every=<m>, tl_fcg_net_iters=<m>, tl_fcg_no_network, tl_fcg_net_restart,
+                     // tl_fcg_coarse
     int test_problem = 0;
     std::vector<State> states;
 };
@@ -228,9 +229,15 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_no_lookahead") d.lookahead = false;
             else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
             else if (k == "tl_fcg_omega") {
-                d.fcg.omega = R();
+                d.fcg.omega = R();  // a fixed weight instead of the Chebyshev weights
+                d.fcg.cheby_ratio = 0.0;
                 // beyond 1 the sweeps stop being an SPD preconditioner (and can diverge)
                 if (!(d.fcg.omega > 0.0 && d.fcg.omega <= 1.0)) throw Error("tl_fcg_omega must lie in (0, 1]");
+            }
+            else if (k == "tl_fcg_cheby") d.fcg.cheby_ratio = 100.0;
+            else if (k == "tl_fcg_cheby_ratio") {
+                d.fcg.cheby_ratio = R();
+                if (!(d.fcg.cheby_ratio > 1.0)) throw Error("tl_fcg_cheby_ratio must exceed 1");
             }
             else if (k == "tl_fcg_no_network") d.fcg.network = false;
             else if (k == "tl_fcg_net_every") d.fcg.net_every = I();

@@ -13,6 +13,7 @@
 
 #include <sys/time.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <memory>
@@ -181,17 +182,47 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors)
 // z (the u0 field's neighbour in the chunk, F_Z) = `sweeps` weighted Jacobi
 // sweeps on A z = r from z, or from zero with from_zero; fused = 1 runs them
 // as one temporal-blocked pass, 0 as one launch per sweep.
-int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused)
+int tl_fcg_smooth_weights(const int* chunk, int sweeps, const double* weights, int from_zero, int fused)
 {
     int rc = -1;
     guarded([&] {
         Chunk* c = stencil_chunk(chunk);
+        if (sweeps < 1 || sweeps > 16 || !weights) throw Error("tl_fcg_smooth_weights: 1 to 16 sweeps and their weights");
+        for (int k = 0; k < sweeps; ++k)
+            if (!(weights[k] > 0.0 && std::isfinite(weights[k]))) throw Error("sweep weights must be positive");
         if (fused && !fcg_smooth_tb_supported(*c, sweeps)) throw Error("no temporal-blocked pass for these sweeps");
-        fcg_smooth(*c, sweeps, omega, from_zero != 0, fused != 0);
+        fcg_smooth(*c, sweeps, weights, from_zero != 0, fused != 0);
         c->sync();
         rc = 0;
     });
     return rc;
+}
+
+// The sweep weights the flexible CG uses for `sweeps` sweeps: omega each, or
+// with cheby_ratio > 1 the Chebyshev step weights of [2 / cheby_ratio, 2]
+// (tl_fcg_cheby, tl_fcg_cheby_ratio).  No GPU work.
+int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* weights)
+{
+    int rc = -1;
+    guarded([&] {
+        if (sweeps < 1 || sweeps > 16 || !weights) throw Error("tl_fcg_sweep_weights: 1 to 16 sweeps");
+        if (cheby_ratio != 0.0 && !(cheby_ratio > 1.0)) throw Error("the Chebyshev ratio must exceed 1");
+        FcgOptions o;
+        o.smooth = sweeps;
+        o.omega = omega;
+        o.cheby_ratio = cheby_ratio;
+        fcg_sweep_weights(o, weights);
+        rc = 0;
+    });
+    return rc;
+}
+
+// The same with one weight for every sweep.
+int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused)
+{
+    double w[16];
+    for (int k = 0; k < 16; ++k) w[k] = omega;
+    return tl_fcg_smooth_weights(chunk, sweeps, w, from_zero, fused);
 }
 
 // The two-grid preconditioner's coarse pieces on a chunk (test hook):
@@ -247,7 +278,8 @@ int tl_fcg_smooth_update(const int* chunk, double omega, double alpha, double* s
         DevScalars s{};
         s.alpha = alpha;
         TL_HIP(hipMemcpyAsync(c->ds, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
-        const long nb = op_fcg_smooth_tb(*c, nullptr, c->f[F_SD], omega, 4, c->f[F_W], c->r_alt);
+        const double w[4] = {omega, omega, omega, omega};
+        const long nb = op_fcg_smooth_tb(*c, nullptr, c->f[F_SD], w, 4, c->f[F_W], c->r_alt);
         std::swap(c->f[F_Z], c->f[F_SD]);
         std::swap(c->f[F_R], c->r_alt);
         op_finish_sums3(*c, nb, sums);

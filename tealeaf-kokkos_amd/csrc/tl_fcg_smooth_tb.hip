@@ -1,9 +1,11 @@
 // The flexible CG's preconditioner sweeps as one temporal-blocked pass (2D).
 //
 // The preconditioner applies K weighted Jacobi sweeps on A z = r,
-// z_(s) = z_(s-1) + (r - A z_(s-1)) (omega / D), D the diagonal of A (1 plus
+// z_(s) = z_(s-1) + (r - A z_(s-1)) ((w_1 / D) f_s), D the diagonal of A (1 plus
 // the coefficients of the cell's interior faces; a face on the mesh boundary
-// carries no flux), starting from z = 0 or from the network's output.  Sweep
+// carries no flux), starting from z = 0 or from the network's output; the
+// sweep weights are w_s = w_1 f_s (f_1 = 1; all f_s = 1 for a fixed weight,
+// whose sweeps are then the plain (r - A z) (omega / D)).  Sweep
 // s needs sweep s - 1 on the neighbouring rows only, so the wavefront of the
 // temporal-blocked Jacobi kernel (tl_jacobi_tb.hpp) runs all K sweeps in one
 // pass: sweep s at row y - s from a three-row register window of sweep s - 1,
@@ -48,7 +50,8 @@ struct FArgs {
     const uint2* rowtab;
     int row0;
     int ny;
-    double omega;
+    double w1;    // the first sweep's weight
+    double f[4];  // sweep s's weight over w1 (f[0] = 1)
     int tiles_x;
     int rows;
     long nblocks;
@@ -79,7 +82,8 @@ struct FCtx {
     bool hi0, hi1;
     bool kw0, kw1;  // the lane's x faces (west of c0, west of c1) are inside the mesh
     bool out0, out1;
-    double omega, alpha;
+    double w1, alpha;
+    double f[4];
     double* rr;  // LDS rings [ring rows][64][2]
     double* kxr;
     double* kyr;
@@ -154,7 +158,7 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         d1 += kyn1;
         d0 += nky.x;
         d1 += nky.y;
-        const double2 wd = make_double2(c.omega / d0, c.omega / d1);
+        const double2 wd = make_double2(c.w1 / d0, c.w1 / d1);
         wdr[R::row(PH, 1) * WAVE] = wd;
         st.wd1 = wd;
     }
@@ -189,8 +193,9 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         az1 += kyn1 * (c1 - n1);
         az0 += ky.x * (c0 - s0);
         az1 += ky.y * (c1 - s1);
-        const double zn0 = c0 + (r.x - az0) * wd.x;
-        const double zn1 = c1 + (r.y - az1) * wd.y;
+        const double fs = c.f[s - 1];
+        const double zn0 = c0 + (r.x - az0) * (wd.x * fs);
+        const double zn1 = c1 + (r.y - az1) * (wd.y * fs);
         kyn0 = ky.x;
         kyn1 = ky.y;
         if (s < K) {
@@ -269,7 +274,9 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     }
     c.ylo = c.Y0 - K;
     c.yhi = min(c.Y0 + A.rows - 1, g.ny + 1);
-    c.omega = A.omega;
+    c.w1 = A.w1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c.f[q] = A.f[q];
     c.alpha = RUPD ? A.ds->alpha : 0.0;
     c.rr = reinterpret_cast<double*>(r_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
@@ -324,8 +331,8 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 // previous step's r -= alpha w (alpha from the device scalars) into r_out,
 // sweeps on that r, and leaves z_out . w, ||r_out||^2 and z_out . r_out as
 // three partials per block.
-long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega, int sweeps, const double* w,
-                      double* r_out)
+long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double* weights, int sweeps,
+                      const double* w, double* r_out)
 {
     if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
     if (r_out && (!w || z_in)) throw Error("temporal-blocked smoothing: the r update needs w and a zero start");
@@ -342,7 +349,8 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
     a.rowtab = tb_rowtab(c);
     a.row0 = TB_ROWTAB0;
     a.ny = c.g.ny;
-    a.omega = omega;
+    a.w1 = weights[0];
+    for (int q = 0; q < 4; ++q) a.f[q] = sweep_ratio(weights, q);
     constexpr int K = 4;
     const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
     a.tiles_x = (c.g.nx + ow - 1) / ow;
@@ -370,7 +378,7 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, double omega,
 #else  // 3D keeps the one-sweep kernel (the flexible CG is 2D)
 
 bool fcg_smooth_tb_supported(const Chunk&, int) { return false; }
-long op_fcg_smooth_tb(Chunk&, const double*, double*, double, int, const double*, double*)
+long op_fcg_smooth_tb(Chunk&, const double*, double*, const double*, int, const double*, double*)
 {
     throw Error("temporal-blocked smoothing is implemented for 2D");
 }

@@ -199,11 +199,12 @@ __global__ void k_plane_out(Geom g, const float* __restrict__ pz, double s, doub
     }
 }
 
-// One weighted Jacobi sweep on A z = r: z_out = z + (r - A z) (omega / D),
-// D the diagonal of A (1 + the coefficients of the cell's interior faces;
-// a face on the mesh boundary carries no flux).  Interior cells.
+// One weighted Jacobi sweep on A z = r: z_out = z + (r - A z) ((w1 / D) f),
+// the sweep's weight w1 f, D the diagonal of A (1 + the coefficients of the
+// cell's interior faces; a face on the mesh boundary carries no flux).
+// Interior cells.
 __global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double* __restrict__ ky,
-                             const double* __restrict__ r, const double* __restrict__ z, double omega,
+                             const double* __restrict__ r, const double* __restrict__ z, double w1, double f,
                              double* __restrict__ zo)
 {
     const long n = (long)g.nx * g.ny;
@@ -216,7 +217,7 @@ __global__ void k_fcg_smooth(Geom g, const double* __restrict__ kx, const double
         if (x > HALO) { az += kx[c] * (zc - z[c - 1]); d += kx[c]; }
         if (y < g.ny + 1) { az += ky[c + g.sx] * (zc - z[c + g.sx]); d += ky[c + g.sx]; }
         if (y > HALO) { az += ky[c] * (zc - z[c - g.sx]); d += ky[c]; }
-        zo[c] = zc + (r[c] - az) * (omega / d);
+        zo[c] = zc + (r[c] - az) * ((w1 / d) * f);
     }
 }
 
@@ -227,7 +228,18 @@ unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, std::min<long>(
 // z = `sweeps` weighted Jacobi sweeps on A z = r from z (from zero with
 // from_zero): one temporal-blocked pass where it applies, else one launch
 // per sweep.  The result is left in F_Z.
-long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_fused, bool dot_w)
+void fcg_sweep_weights(const FcgOptions& opt, double* w)
+{
+    const int m = opt.smooth;
+    if (opt.cheby_ratio > 0.0) {
+        const double hi = 2.0, lo = 2.0 / opt.cheby_ratio, pi = 3.14159265358979323846;
+        for (int k = 0; k < m; ++k) w[k] = 1.0 / (0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos((2 * k + 1) * pi / (2 * m)));
+    } else {
+        for (int k = 0; k < m; ++k) w[k] = opt.omega;
+    }
+}
+
+long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, bool allow_fused, bool dot_w)
 {
     // the sweep kernels are the 5-point operator on one plane
     if (TL_DIM != 2) throw Error("the flexible CG's preconditioner sweeps are 2D only");
@@ -236,7 +248,7 @@ long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_f
         return 0;
     }
     if (allow_fused && fcg_smooth_tb_supported(c, sweeps)) {
-        const long nb = op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], omega, sweeps,
+        const long nb = op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], weights, sweeps,
                                          dot_w ? c.f[F_W] : nullptr);
         std::swap(c.f[F_Z], c.f[F_SD]);
         return nb;
@@ -245,7 +257,7 @@ long fcg_smooth(Chunk& c, int sweeps, double omega, bool from_zero, bool allow_f
     const long n = (long)c.g.nx * c.g.ny * c.g.nz;
     for (int k = 0; k < sweeps; ++k) {
         hipLaunchKernelGGL(k_fcg_smooth, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_KX], c.f[F_KY],
-                           c.f[F_R], c.f[F_Z], omega, c.f[F_SD]);
+                           c.f[F_R], c.f[F_Z], weights[0], sweep_ratio(weights, k), c.f[F_SD]);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) throw Error(std::string("FCG smoothing: ") + hipGetErrorString(e));
         std::swap(c.f[F_Z], c.f[F_SD]);
@@ -534,6 +546,9 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     iters = 0;
     error = 0.0;
     if (TL_DIM != 2 || c.g.nx % 8 || c.g.ny % 8 || c.preconditioner || (!L && opt.network)) return false;
+    if (opt.smooth > 16) throw Error("tl_fcg_smooth: at most 16 sweeps");
+    double sw[16] = {};
+    if (opt.smooth > 0) fcg_sweep_weights(opt, sw);
     const long n = (long)c.g.nx * c.g.ny;
     float *pf = nullptr, *pz = nullptr;
     hipStream_t keep = nullptr;
@@ -636,7 +651,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         long beta_blocks = 0;  // z . w left by the preconditioner pass
         const bool beta_rrn = r_pending;
         if (r_pending) {
-            beta_blocks = op_fcg_smooth_tb(c, nullptr, c.f[F_SD], opt.omega, opt.smooth, c.f[F_W], c.r_alt);
+            beta_blocks = op_fcg_smooth_tb(c, nullptr, c.f[F_SD], sw, opt.smooth, c.f[F_W], c.r_alt);
             std::swap(c.f[F_Z], c.f[F_SD]);
             std::swap(c.f[F_R], c.r_alt);
             r_pending = false;
@@ -644,12 +659,12 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             // the two-grid preconditioner: z = S r from zero, the coarse
             // correction z += s G P N(rc / s) with rc = P^T (r - A z) / B^2,
             // then S again from that z (S: the weighted Jacobi sweeps)
-            if (opt.coarse_pre) (void)fcg_smooth(c, opt.smooth, opt.omega, true, true, false);
+            if (opt.coarse_pre) (void)fcg_smooth(c, opt.smooth, sw, true, true, false);
             const long nc = (long)cg.ncx * cg.ncy;
             op_coarse_restrict(c, cg, opt.coarse_pre, L->crc, L->pr, L->crc + nc);
             unet_apply(*L->net, L->pr, pf, pz, 1, cg.ncy, cg.ncx);
             op_coarse_prolong(c, cg, pz, L->crc + nc, L->escale > 0.0 ? L->escale : 1.0, opt.coarse_pre);
-            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, false, true, !first);
+            beta_blocks = fcg_smooth(c, opt.smooth, sw, false, true, !first);
         } else if (uses_net(it)) {
             hipLaunchKernelGGL(k_plane_in, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, c.f[F_R],
                                c.f[F_DENSITY], s > 0.0 ? 1.0 / s : 0.0, it == 1 ? 1 : 0, L->pr, pf);
@@ -660,9 +675,9 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
             hipLaunchKernelGGL(k_plane_out, dim3(blocks_for(n)), dim3(256), 0, c.stream, c.g, pz, s * g_scale,
                                c.f[F_Z]);
             launched("FCG output");
-            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, false, true, !first);
+            beta_blocks = fcg_smooth(c, opt.smooth, sw, false, true, !first);
         } else {
-            beta_blocks = fcg_smooth(c, opt.smooth, opt.omega, true, true, !first);
+            beta_blocks = fcg_smooth(c, opt.smooth, sw, true, true, !first);
         }
         // beta, p, w, alpha, u[, r] with the scalars on the device; one read
         // after the last network iteration a restart begins a new recurrence

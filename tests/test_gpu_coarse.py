@@ -122,7 +122,8 @@ def test_learned_two_grid_gains(hip):
         gains = []
         for seed in (0, 1):
             deck = decks.random_shapes_2d(512, seed, steps=1, extra=("tl_check_result",))
-            fdeck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned")
+            # the fixed sweep weight the network's loss is measured after
+            fdeck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_omega=0.8")
             ref = hip.run_deck(deck, dim=2)["steps"][0]
             sweeps = hip.run_deck(fdeck.replace("tl_use_fcg_learned", "tl_use_fcg_learned\ntl_fcg_no_network"),
                                   dim=2)["steps"][0]

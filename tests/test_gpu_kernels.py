@@ -486,7 +486,8 @@ def test_stencil_refused_with_neighbour_chunks(hip, oracle):
 
 
 def jacobi_sweeps_numpy(r, z, kx, ky, omega, sweeps):
-    """Weighted Jacobi sweeps z <- z + omega (r - A z) / D on the mesh cells,
+    """Weighted Jacobi sweeps z <- z + omega (r - A z) / D on the mesh cells
+    (omega a number, or one weight per sweep),
     A = I + sum over interior faces of k (e_i - e_j)(e_i - e_j)^T (a face on
     the mesh boundary carries no flux) and D = diag(A); arrays in the
     reference's flat layout, (1, ny + 4, nx + 4), kx the west and ky the south
@@ -501,10 +502,11 @@ def jacobi_sweeps_numpy(r, z, kx, ky, omega, sweeps):
     ks[0, :] = 0.0
     kn[-1, :] = 0.0
     D = 1.0 + kw + ke + ks + kn
-    for _ in range(sweeps):
+    ws = np.broadcast_to(np.asarray(omega, dtype=float), (sweeps,))
+    for s in range(sweeps):
         P = np.pad(Z, 1, mode="edge")
         AZ = Z + kw * (Z - P[1:-1, :-2]) + ke * (Z - P[1:-1, 2:]) + ks * (Z - P[:-2, 1:-1]) + kn * (Z - P[2:, 1:-1])
-        Z = Z + omega * (R - AZ) / D
+        Z = Z + ws[s] * (R - AZ) / D
     return Z
 
 
@@ -535,6 +537,39 @@ def test_fcg_smooth_fused(hip, shape, from_zero):
     want = jacobi_sweeps_numpy(vals["r"], z0, vals["kx"], vals["ky"], 0.8, 4)
     got = out[1].reshape(want.shape)
     assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+
+
+@pytest.mark.parametrize("shape", [(37, 23), (5, 4), (130, 41), (64, 64)])
+@pytest.mark.parametrize("from_zero", [True, False])
+def test_fcg_smooth_chebyshev_weights(hip, shape, from_zero):
+    """The sweeps with a weight per sweep (the Chebyshev step weights of
+    tl_fcg_cheby, up to 7.1, from tl_fcg_sweep_weights): the temporal-blocked
+    pass equal to one launch per sweep on every mesh cell, and both against
+    numpy's sweeps with those weights."""
+    import ctypes
+    nx, ny = shape
+    w = (ctypes.c_double * 4)()
+    L = hip.lib(2)
+    assert L.tl_fcg_sweep_weights(ctypes.c_int(4), ctypes.c_double(0.8), ctypes.c_double(30.0), w) == 0
+    assert max(w[:]) > 7.0
+    rng = np.random.default_rng(nx * 53 + ny + from_zero)
+    full = (1, ny + 4, nx + 4)
+    vals = {"r": rng.uniform(-1, 1, size=full), "z": rng.uniform(-1, 1, size=full),
+            "kx": rng.uniform(0.1, 40.0, size=full), "ky": rng.uniform(0.1, 40.0, size=full)}
+    print(f"fcg chebyshev sweeps {shape} from_zero={from_zero}", flush=True)
+    out = []
+    for fused in (0, 1):
+        g = hip.ExtChunk(L, 2, nx, ny)
+        for k, v in vals.items():
+            g.set(k, v)
+        rc = g.L.tl_fcg_smooth_weights(g.c, ctypes.c_int(4), w, ctypes.c_int(int(from_zero)), ctypes.c_int(fused))
+        assert rc == 0, g.L.tl_last_error()
+        out.append(interior(g, g.get("z")))
+        g.close()
+    np.testing.assert_array_equal(out[0], out[1])
+    z0 = np.zeros_like(vals["z"]) if from_zero else vals["z"]
+    want = jacobi_sweeps_numpy(vals["r"], z0, vals["kx"], vals["ky"], list(w), 4)
+    assert np.max(np.abs(out[1].reshape(want.shape) - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
 
 
 @pytest.mark.parametrize("shape", [(37, 23), (5, 4), (131, 6), (130, 41), (2, 17), (64, 64)])

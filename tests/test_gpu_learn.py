@@ -139,7 +139,8 @@ def test_train_fcg_and_ablation(hip, oracle):
 
         for s in (0, 1):
             deck = decks.random_shapes_2d(n, s, steps=1, extra=("tl_check_result",))
-            fcg_deck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned")
+            # the fixed sweep weight the network's loss is measured after
+            fcg_deck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_omega=0.8")
             ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
             cg = oracle.run_deck(deck.replace("tl_use_ppcg", "tl_use_cg"), dim=2, reduce_parts=8, threads=8)
             trained.use()
@@ -196,3 +197,29 @@ def test_fcg_sweeps_only(hip, oracle, kind, n, kw):
             assert sg["exact_error"] < 1e-10
         for key in ("vol", "mass", "ie", "temp"):
             assert abs(sg[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sg[key], so[key])
+
+
+@pytest.mark.parametrize("kind,n", [("two_rect", 256), ("shapes", 192)])
+def test_fcg_chebyshev_sweeps(hip, oracle, kind, n):
+    """The sweeps with Chebyshev step weights on [2/100, 2] (the default;
+    tl_fcg_cheby) instead of the fixed 0.8 (tl_fcg_omega=0.8): the same four
+    sweeps per iteration, the same
+    bar as the fixed weight (eps on the true residual, FP64 PPCG's totals
+    to 1e-7), and fewer iterations than the fixed weight on every timestep
+    (the preconditioned operator's small eigenvalues are raised about 4x)."""
+    import decks
+    if kind == "shapes":
+        deck = decks.random_shapes_2d(n, 7, steps=2, extra=("tl_check_result",))
+    else:
+        deck = decks.two_rect_2d(n, "ppcg", steps=2, extra=("tl_check_result",))
+    ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
+    jac = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network\ntl_fcg_omega=0.8"), dim=2)
+    che = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network\ntl_fcg_cheby"), dim=2)
+    for sc, sj, so in zip(che["steps"], jac["steps"], ref["steps"]):
+        print(f"  step {sc['step']}: Chebyshev {sc['fcg_iters']} it, fixed weight {sj['fcg_iters']} it, "
+              f"exact {sc['exact_error']:.3e}", flush=True)
+        assert sc["fcg"] == 1 and sc["fcg_fallback"] == 0
+        assert sc["exact_error"] < 1e-10
+        for key in ("vol", "mass", "ie", "temp"):
+            assert abs(sc[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sc[key], so[key])
+        assert sc["fcg_iters"] < sj["fcg_iters"]
