@@ -348,15 +348,16 @@ Learner* active_learner();
 // z = s N(r / s) (network on, every net_every-th iteration) or 0, then
 // `smooth` weighted-Jacobi sweeps z += w_s D^-1 (r - A z): the Chebyshev
 // step weights of [2 / cheby_ratio, 2] (fcg_sweep_weights), or w_s = omega
-// with cheby_ratio = 0.
+// with cheby_ratio = 0; a negative cheby_ratio is set per solve by
+// fcg_auto_ratio.
 struct FcgOptions {
     bool network = true;
     int net_every = 1;
     int net_iters = 0;  // > 0: the network only in the first net_iters iterations, sweeps alone after
     bool net_restart = false;  // start a new recurrence (p = z) after the last network iteration
-    int smooth = 4;
+    int smooth = 16;  // four temporal-blocked passes of 4
     double omega = 0.8;
-    double cheby_ratio = 100.0;
+    double cheby_ratio = -1.0;
     // the network as the coarse-grid solve of a two-grid preconditioner
     // (tl_coarse.hip), in every iteration: sweeps, restriction, network,
     // prolongation, sweeps
@@ -388,11 +389,19 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
                        double& error);
 // The sweep weights of the options, w[0 .. opt.smooth - 1].  Chebyshev:
 // w_k = 1 / theta_k, theta_k the roots of the degree-`smooth` Chebyshev
-// polynomial on [2 / cheby_ratio, 2] in decreasing order
-// ((hi + lo) / 2 + (hi - lo) / 2 cos((2 k + 1) pi / (2 smooth))).  The
-// residual polynomial prod_k (1 - x / theta_k) then stays below 1 in
-// magnitude on that interval, and D^-1 A's spectrum lies in (0, 2).
+// polynomial on [2 / cheby_ratio, 2]
+// ((hi + lo) / 2 + (hi - lo) / 2 cos((2 k + 1) pi / (2 smooth))), in the
+// Lebedev-Finogenov order for a power-of-two count, else in decreasing
+// order.  The residual polynomial prod_k (1 - x / theta_k) then stays below
+// 1 in magnitude on that interval, and D^-1 A's spectrum lies in (0, 2).
 void fcg_sweep_weights(const FcgOptions& opt, double* w);
+// The Chebyshev interval ratio for the chunk's operator: 30 sqrt(mean diag A),
+// within [30, 10000].  D^-1 A's bulk spectrum widens with the face
+// coefficients (dt / dx^2 times the conductivity); this fits the best ratio
+// measured for 16 sweeps from 192^2 to 8192^2 (about 100 at 192^2, 300 at
+// 512^2, 3000 at 4096^2 and 8192^2, with a flat optimum from 1000 to 10000
+// at the largest).
+double fcg_auto_ratio(Chunk& c);
 // Sweep q's weight over the first one, the factor the sweep kernels apply to
 // w_1 / D (exactly 1 for q = 0 and for equal weights).
 inline double sweep_ratio(const double* w, int q) { return w[q] / w[0]; }
@@ -402,9 +411,13 @@ inline double sweep_ratio(const double* w, int q) { return w[q] / w[0]; }
 // of z . w in the reduction partials and returns their count (else 0).
 long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, bool allow_fused = true,
                 bool dot_w = false);
-bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);
+bool fcg_smooth_tb_supported(const Chunk& c, int sweeps);  // 2D, a multiple of 4 sweeps
+// One temporal-blocked pass of 4 sweeps (tl_fcg_smooth_tb.hip), with the
+// weights weights[first_sweep ..] of a set that starts at weights[0].
 long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double* weights, int sweeps,
-                      const double* w = nullptr, double* r_out = nullptr);
+                      const double* w = nullptr, double* r_out = nullptr, bool rsums = false, int first_sweep = 0);
+// `sweeps` (a multiple of 4) sweeps as chained passes, into F_Z.
+long op_fcg_sweep_passes(Chunk& c, const double* weights, int sweeps, bool from_zero, bool dot_w, bool r_update);
 
 // The C++ driver (tl_driver.hip): a deck run to its end, or to max_steps
 // timesteps when that is positive; returns the JSON report.
@@ -469,6 +482,9 @@ double op_jacobi_solve(Chunk& c, bool fused);
 void op_copy_u(Chunk& c);
 void op_calc_residual(Chunk& c);
 double op_calc_2norm(Chunk& c, bool norm_r);
+// Mean of A's diagonal over the mesh cells (1 + the cell's interior face
+// coefficients), deterministic.
+double op_mean_diag(Chunk& c);
 void op_finalise(Chunk& c);
 
 }  // namespace tl

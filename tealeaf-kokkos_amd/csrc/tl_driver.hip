@@ -193,6 +193,7 @@ Deck parse_deck(const std::string& text)
     if (state_max < 1) throw Error("No states defined.");
     d.states.resize(state_max);
     Words w{lines};
+    bool smooth_set = false;  // tl_fcg_smooth given
     while (w.next_line()) {
         for (std::string k = w.word(false); !k.empty(); k = w.word(false)) {
             auto R = [&] { return parse_real(w.word(true)); };
@@ -227,14 +228,20 @@ Deck parse_deck(const std::string& text)
             else if (k == "tl_ppcg_mixed_replace") d.mixed_replace = R();
             else if (k == "tl_ppcg_true_residual") d.ppcg_true_residual = true;
             else if (k == "tl_no_lookahead") d.lookahead = false;
-            else if (k == "tl_fcg_smooth") d.fcg.smooth = I();
+            else if (k == "tl_fcg_smooth") {
+                d.fcg.smooth = I();
+                smooth_set = true;
+            }
             else if (k == "tl_fcg_omega") {
-                d.fcg.omega = R();  // a fixed weight instead of the Chebyshev weights
+                // a fixed weight instead of the Chebyshev weights, with the
+                // 4 sweeps the fixed weight was tuned for unless set
+                d.fcg.omega = R();
                 d.fcg.cheby_ratio = 0.0;
+                if (!smooth_set) d.fcg.smooth = 4;
                 // beyond 1 the sweeps stop being an SPD preconditioner (and can diverge)
                 if (!(d.fcg.omega > 0.0 && d.fcg.omega <= 1.0)) throw Error("tl_fcg_omega must lie in (0, 1]");
             }
-            else if (k == "tl_fcg_cheby") d.fcg.cheby_ratio = 100.0;
+            else if (k == "tl_fcg_cheby") d.fcg.cheby_ratio = -1.0;  // the ratio from the operator (default)
             else if (k == "tl_fcg_cheby_ratio") {
                 d.fcg.cheby_ratio = R();
                 if (!(d.fcg.cheby_ratio > 1.0)) throw Error("tl_fcg_cheby_ratio must exceed 1");
@@ -544,6 +551,7 @@ struct StepInfo {
     // learned flexible CG: its iterations, and whether FP64 PPCG took over
     int fcg_iters = 0;
     bool fcg = false, fcg_fallback = false;
+    double fcg_ratio = 0.0;  // the sweeps' Chebyshev interval ratio (0: a fixed weight)
     // iterations launched before the previous rrn was read, and how many of
     // them the stop flag turned into no-ops
     int ahead = 0, ahead_undone = 0;
@@ -655,7 +663,10 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
         int it = 0;
         double err = 0.0;
         si.fcg = true;
-        fcg_done = fcg_learned_solve(c, learner, d.fcg, d.eps, max_iters, it, err);
+        FcgOptions o = d.fcg;
+        if (TL_DIM == 2 && o.smooth > 0 && o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c);
+        si.fcg_ratio = o.cheby_ratio;
+        fcg_done = fcg_learned_solve(c, learner, o, d.eps, max_iters, it, err);
         si.fcg_iters = it;
         if (fcg_done) {
             error = err;
@@ -1045,10 +1056,11 @@ std::string run_deck(const std::string& text, int max_steps)
         o.f("\"iters\": %d, \"error\": %.17g, \"exact_error\": %.17g, \"switch_step\": %d, \"est_itc\": %d, "
             "\"eigmin\": %.17g, \"eigmax\": %.17g, \"cg_steps\": %d, \"cheby_steps\": %d, \"solve_s\": %.9g, "
             "\"dt\": %.17g, \"mixed_steps\": %d, \"replacements\": %d, \"mixed_fallback\": %d, "
-            "\"fcg\": %d, \"fcg_iters\": %d, \"fcg_fallback\": %d, \"ahead\": %d, \"ahead_undone\": %d, ",
+            "\"fcg\": %d, \"fcg_iters\": %d, \"fcg_fallback\": %d, \"fcg_ratio\": %.17g, \"ahead\": %d, "
+            "\"ahead_undone\": %d, ",
             si.iters, si.error, si.exact_error, si.switch_step, si.est_itc, si.eigmin, si.eigmax, si.cg_steps,
             si.cheby_steps, si.solve_s, dt, si.mixed_steps, si.replacements, si.mixed_fallback ? 1 : 0,
-            si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0, si.ahead, si.ahead_undone);
+            si.fcg ? 1 : 0, si.fcg_iters, si.fcg_fallback ? 1 : 0, si.fcg_ratio, si.ahead, si.ahead_undone);
         if (d.summary_frequency != 0 && step % d.summary_frequency == 0)
             summary(o, tea_out, *c, step, time);
         else

@@ -206,7 +206,8 @@ int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* w
     int rc = -1;
     guarded([&] {
         if (sweeps < 1 || sweeps > 16 || !weights) throw Error("tl_fcg_sweep_weights: 1 to 16 sweeps");
-        if (cheby_ratio != 0.0 && !(cheby_ratio > 1.0)) throw Error("the Chebyshev ratio must exceed 1");
+        if (cheby_ratio != 0.0 && !(cheby_ratio > 1.0))
+            throw Error("the Chebyshev ratio must exceed 1 (the ratio from the operator needs a chunk)");
         FcgOptions o;
         o.smooth = sweeps;
         o.omega = omega;

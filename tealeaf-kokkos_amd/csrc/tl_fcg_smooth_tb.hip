@@ -42,7 +42,7 @@ struct FArgs {
     double* z_out;
     const double* w;     // with DOTW: the previous direction's A p, for z . w
     double* partials;
-    double* r_out;       // with RUPD: r - alpha w (!= r), the previous step's residual update
+    double* r_out;       // with RM 1: r - alpha w (!= r), the previous step's residual update
     const DevScalars* ds;
     const double* r;
     const double* kx;
@@ -60,7 +60,7 @@ struct FArgs {
 struct FRow {
     double2 z, r, ky;
     double2 wo;  // w of the output row two rounds on (DOTW)
-    double2 wi;  // w of the row itself (RUPD)
+    double2 wi;  // w of the row itself (RM 1)
     double kx0, kx1;
     bool ky_in;  // the row's south face is inside the mesh
 };
@@ -70,8 +70,8 @@ struct FState {
     double zw[K][3][2];  // z after sweep s, slot (round) % 3, column
     FRow pf[2];
     double acc;          // z . w over the wave's own cells (DOTW)
-    double acc_rr;       // ||r - alpha w||^2 over the wave's own cells (RUPD)
-    double acc_zr;       // z . (r - alpha w) over the wave's own cells (RUPD)
+    double acc_rr;       // ||r||^2 over the wave's own cells, r as updated (RM 1) or read (RM 2)
+    double acc_zr;       // z . r over the wave's own cells (RM 1, 2)
     double2 wd1;         // omega / D of level 1's row, formed this round
 };
 
@@ -90,12 +90,12 @@ struct FCtx {
     double* wdr;  // omega / D of the row's cells
 };
 
-template <int K, bool ZERO, bool DOTW, bool RUPD>
+template <int K, bool ZERO, bool DOTW, int RM>
 __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t, int PHk, int y, FRow& v)
 {
     const unsigned oy = t[2 * PHk], of = t[2 * PHk + 1];
     if (!ZERO) v.z = ldp(A.z_in, oy + c.bcp);
-    if (RUPD) v.wi = ldp(A.w, oy + c.bcp);
+    if (RM == 1) v.wi = ldp(A.w, oy + c.bcp);
     // the row that leaves the pass when row y enters it (slack rows hold
     // those past the mesh); lanes without output read a pad pair
     if (DOTW) v.wo = ldp(A.w, 8u * (unsigned)(c.g.base + (long)(y - K) * c.g.sx) + c.bsp);
@@ -108,7 +108,7 @@ __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t,
     v.ky_in = y > HALO && y <= A.ny + 1;
 }
 
-template <int K, bool ZERO, bool DOTW, bool RUPD, int PH>
+template <int K, bool ZERO, bool DOTW, int RM, int PH>
 __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     using R = tb::TBRing<K>;
@@ -129,7 +129,14 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     st.zw[0][S0][0] = ZERO ? 0.0 : pick(v.z, c.hi0);
     st.zw[0][S0][1] = ZERO ? 0.0 : pick(v.z, c.hi1);
     double2 rin = v.r;
-    if (RUPD) {
+    if (RM == 2) {
+        // r as read: the wave's own cells sum its square
+        const bool yin = y >= c.Y0 && y <= c.yhi;
+        const double r0 = pick(rin, c.hi0), r1 = pick(rin, c.hi1);
+        st.acc_rr += yin && c.out0 ? r0 * r0 : 0.0;
+        st.acc_rr += yin && c.out1 ? r1 * r1 : 0.0;
+    }
+    if (RM == 1) {
         // CGCalcUr's r -= alpha w of the previous step, on the pair as loaded;
         // the wave's own cells store it and sum its square
         rin.x = v.r.x - c.alpha * v.wi.x;
@@ -163,7 +170,7 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         st.wd1 = wd;
     }
     const double2 wo = v.wo;  // w of this round's output row
-    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, PH, y + 2, v);
+    f_load<K, ZERO, DOTW, RM>(A, c, tab, PH, y + 2, v);
 
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
@@ -211,7 +218,7 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
                 st.acc += zin && c.out0 ? zn0 * wo.x : 0.0;
                 st.acc += zin && c.out1 ? zn1 * wo.y : 0.0;
             }
-            if (RUPD) {  // r of the output row, as updated on entry
+            if (RM != 0) {  // r of the output row (as updated on entry with RM 1)
                 st.acc_zr += zin && c.out0 ? zn0 * r.x : 0.0;
                 st.acc_zr += zin && c.out1 ? zn1 * r.y : 0.0;
             }
@@ -219,18 +226,20 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     }
 }
 
-template <int K, bool ZERO, bool DOTW, bool RUPD, int PH>
+template <int K, bool ZERO, bool DOTW, int RM, int PH>
 __device__ __forceinline__ void f_rounds(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     if constexpr (PH < tb::TB_UNROLL) {
-        f_round<K, ZERO, DOTW, RUPD, PH>(A, c, st, y + PH, tab);
+        f_round<K, ZERO, DOTW, RM, PH>(A, c, st, y + PH, tab);
         __builtin_amdgcn_sched_barrier(0);
-        f_rounds<K, ZERO, DOTW, RUPD, PH + 1>(A, c, st, y, tab);
+        f_rounds<K, ZERO, DOTW, RM, PH + 1>(A, c, st, y, tab);
     }
 }
 
-// RUPD (the previous step's r update folded in) comes with DOTW.
-template <int K, bool ZERO, bool DOTW, bool RUPD>
+// RM (r mode): 0 r only read; 1 the previous step's r update folded in
+// (from zero, with DOTW), ||r||^2 and z.r summed; 2 ||r||^2 and z.r of r as
+// read summed (with DOTW).
+template <int K, bool ZERO, bool DOTW, int RM>
 __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
 {
     using R = tb::TBRing<K>;
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     c.w1 = A.w1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) c.f[q] = A.f[q];
-    c.alpha = RUPD ? A.ds->alpha : 0.0;
+    c.alpha = RM == 1 ? A.ds->alpha : 0.0;
     c.rr = reinterpret_cast<double*>(r_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
@@ -300,13 +309,13 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     st.acc_zr = 0.0;
     st.wd1 = make_double2(0.0, 0.0);
     ConstU* tab = (ConstU*)(const unsigned*)A.rowtab + 2 * (c.ylo - A.row0);
-    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, 0, c.ylo, st.pf[0]);
-    f_load<K, ZERO, DOTW, RUPD>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
+    f_load<K, ZERO, DOTW, RM>(A, c, tab, 0, c.ylo, st.pf[0]);
+    f_load<K, ZERO, DOTW, RM>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
     __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
     const int yend = c.yhi + K;
     for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
-        f_rounds<K, ZERO, DOTW, RUPD, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
-    if (RUPD) {
+        f_rounds<K, ZERO, DOTW, RM, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+    if (RM != 0) {
         double v[3] = {st.acc, st.acc_rr, st.acc_zr};
         block_partial<3, 1>(v, Reduce{A.partials});
     } else if (DOTW) {
@@ -321,7 +330,33 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 {
     // the kernel addresses a field with 32-bit byte offsets
     const bool fits = (double)c.g.alloc * sizeof(double) < 4294967296.0;
-    return sweeps == 4 && c.g.nx >= 2 && c.g.ny >= 2 && fits;
+    return sweeps >= 4 && sweeps % 4 == 0 && c.g.nx >= 2 && c.g.ny >= 2 && fits;
+}
+
+// `sweeps` (a multiple of 4) sweeps as sweeps / 4 passes, each from the
+// previous one's z, with weights[0 ..) in order; the result in F_Z.
+// r_update (from zero): the first pass completes the previous step's
+// r -= alpha w, the updated r in F_R.  The last pass leaves z . w (dot_w, or
+// with r_update) and, with r_update, ||r||^2 and z . r in the reduction
+// partials and returns their count (else 0).
+long op_fcg_sweep_passes(Chunk& c, const double* weights, int sweeps, bool from_zero, bool dot_w, bool r_update)
+{
+    if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, a multiple of 4 sweeps");
+    if (r_update && !from_zero) throw Error("temporal-blocked smoothing: the r update starts from zero");
+    const int passes = sweeps / 4;
+    long nb = 0;
+    for (int q = 0; q < passes; ++q) {
+        const bool first = q == 0, last = q + 1 == passes;
+        const double* z_in = first && from_zero ? nullptr : c.f[F_Z];
+        const double* w = (last && (dot_w || r_update)) || (first && r_update) ? c.f[F_W] : nullptr;
+        double* r_out = first && r_update ? c.r_alt : nullptr;
+        const bool rsums = last && r_update && !first;
+        const long b = op_fcg_smooth_tb(c, z_in, c.f[F_SD], weights, 4, w, r_out, rsums, 4 * q);
+        std::swap(c.f[F_Z], c.f[F_SD]);
+        if (r_out) std::swap(c.f[F_R], c.r_alt);
+        if (last) nb = b;
+    }
+    return nb;
 }
 
 // z_out = `sweeps` weighted Jacobi sweeps on A z = r from z_in (zero when
@@ -332,10 +367,11 @@ bool fcg_smooth_tb_supported(const Chunk& c, int sweeps)
 // sweeps on that r, and leaves z_out . w, ||r_out||^2 and z_out . r_out as
 // three partials per block.
 long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double* weights, int sweeps,
-                      const double* w, double* r_out)
+                      const double* w, double* r_out, bool rsums, int first_sweep)
 {
-    if (!fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
+    if (sweeps != 4 || !fcg_smooth_tb_supported(c, sweeps)) throw Error("temporal-blocked smoothing: 2D, 4 sweeps");
     if (r_out && (!w || z_in)) throw Error("temporal-blocked smoothing: the r update needs w and a zero start");
+    if (rsums && (!w || !z_in || r_out)) throw Error("temporal-blocked smoothing: r sums need w and a start z");
     ftb::FArgs a{};
     a.z_in = z_in;
     a.z_out = z_out;
@@ -349,8 +385,10 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double*
     a.rowtab = tb_rowtab(c);
     a.row0 = TB_ROWTAB0;
     a.ny = c.g.ny;
+    // every pass factors its weights over the set's first one, as the
+    // one-sweep kernel does, so that any split into passes gives the same bits
     a.w1 = weights[0];
-    for (int q = 0; q < 4; ++q) a.f[q] = sweep_ratio(weights, q);
+    for (int q = 0; q < 4; ++q) a.f[q] = sweep_ratio(weights, first_sweep + q);
     constexpr int K = 4;
     const int ow = 2 * WAVE - 2 * tb::tb_margin(K);
     a.tiles_x = (c.g.nx + ow - 1) / ow;
@@ -360,15 +398,17 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double*
     const dim3 grid((unsigned)a.nblocks), block(WAVE);
     TL_TIMED(c, r_out ? K_FCG_SMOOTH_R : K_FCG_SMOOTH, {
         if (r_out)
-            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, true>), grid, block, 0, c.stream, c.g, a);
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, 1>), grid, block, 0, c.stream, c.g, a);
+        else if (rsums)
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true, 2>), grid, block, 0, c.stream, c.g, a);
         else if (z_in && w)
-            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true, false>), grid, block, 0, c.stream, c.g, a);
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, true, 0>), grid, block, 0, c.stream, c.g, a);
         else if (z_in)
-            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false, false>), grid, block, 0, c.stream, c.g, a);
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, false, false, 0>), grid, block, 0, c.stream, c.g, a);
         else if (w)
-            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, false>), grid, block, 0, c.stream, c.g, a);
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, true, 0>), grid, block, 0, c.stream, c.g, a);
         else
-            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false, false>), grid, block, 0, c.stream, c.g, a);
+            hipLaunchKernelGGL((ftb::k_fcg_smooth_tb<K, true, false, 0>), grid, block, 0, c.stream, c.g, a);
     });
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(std::string("kernel launch: ") + hipGetErrorString(e));
@@ -378,7 +418,11 @@ long op_fcg_smooth_tb(Chunk& c, const double* z_in, double* z_out, const double*
 #else  // 3D keeps the one-sweep kernel (the flexible CG is 2D)
 
 bool fcg_smooth_tb_supported(const Chunk&, int) { return false; }
-long op_fcg_smooth_tb(Chunk&, const double*, double*, const double*, int, const double*, double*)
+long op_fcg_smooth_tb(Chunk&, const double*, double*, const double*, int, const double*, double*, bool, int)
+{
+    throw Error("temporal-blocked smoothing is implemented for 2D");
+}
+long op_fcg_sweep_passes(Chunk&, const double*, int, bool, bool, bool)
 {
     throw Error("temporal-blocked smoothing is implemented for 2D");
 }

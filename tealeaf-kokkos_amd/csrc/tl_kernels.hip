@@ -1353,6 +1353,24 @@ __global__ __launch_bounds__(BLOCK) void k_2norm(Geom g, const double* __restric
     block_partial<1>(v, red);
 }
 
+// Sum over the interior faces (each once; a face on the mesh boundary
+// carries no flux) of their coefficients: sum of diag(A) = cells + 2 x this.
+__global__ __launch_bounds__(BLOCK) void k_face_sum(Geom g, const double* __restrict__ kx,
+                                                    const double* __restrict__ ky, const double* __restrict__ kz,
+                                                    Reduce red)
+{
+    Strip s = make_strip(g);
+    double v[1] = {0.0};
+    for (int n = 0; n < s.count; ++n, strip_advance(s)) {
+        if (s.kk > HALO) v[0] += kx[s.i];
+        if (s.jj > HALO) v[0] += ky[s.i];
+#if TL_DIM == 3
+        if (s.ii > HALO) v[0] += kz[s.i];
+#endif
+    }
+    block_partial<1>(v, red);
+}
+
 // a . b over the interior (flexible CG)
 __global__ __launch_bounds__(BLOCK) void k_dot(Geom g, const double* __restrict__ a, const double* __restrict__ b,
                                                Reduce red)
@@ -2626,6 +2644,14 @@ void op_fcg_iteration_dev(Chunk& c, bool first, long beta_blocks, bool& u_pendin
                                        c.f[F_R], c.f[F_P], c.f[F_W], c.f[F_MI], c.f[F_Z], c.reducer(S_RRN, EP_FCG_RRN)));
     }
     c.close(c.g.nblocks, 1);
+}
+
+double op_mean_diag(Chunk& c)
+{
+    STRIP_LAUNCH(k_face_sum, c, c.f[F_KX], c.f[F_KY], TL_DIM == 3 ? c.f[F_KZ] : nullptr,
+                 c.reducer(S_NORM, EP_STORE));
+    c.close(c.g.nblocks, 1);
+    return 1.0 + 2.0 * c.result(S_NORM) / ((double)c.g.nx * c.g.ny * c.g.nz);
 }
 
 double op_calc_2norm(Chunk& c, bool norm_r)

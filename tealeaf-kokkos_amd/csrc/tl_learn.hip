@@ -233,10 +233,40 @@ void fcg_sweep_weights(const FcgOptions& opt, double* w)
     const int m = opt.smooth;
     if (opt.cheby_ratio > 0.0) {
         const double hi = 2.0, lo = 2.0 / opt.cheby_ratio, pi = 3.14159265358979323846;
-        for (int k = 0; k < m; ++k) w[k] = 1.0 / (0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos((2 * k + 1) * pi / (2 * m)));
+        // roots in decreasing order: the iterate never grows (every partial
+        // product of (1 - x / theta) stays within 1 on [0, 2]), but a
+        // rounding error of an early sweep is multiplied by the large
+        // late weights (up to 5.5e7 for 16 sweeps on [2/3000, 2])
+        std::vector<double> th(m);
+        for (int k = 0; k < m; ++k) th[k] = 0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos((2 * k + 1) * pi / (2 * m));
+        // for m a power of two, the Lebedev-Finogenov order (root i paired
+        // with root n + 1 - i at each doubling) keeps both the partial
+        // products and those rounding factors small (90 and 353 for 16)
+        std::vector<int> idx(1, 1);
+        if ((m & (m - 1)) == 0) {
+            while ((int)idx.size() < m) {
+                const int n2 = 2 * (int)idx.size();
+                std::vector<int> next;
+                for (int v : idx) {
+                    next.push_back(v);
+                    next.push_back(n2 + 1 - v);
+                }
+                idx.swap(next);
+            }
+        } else {
+            idx.resize(m);
+            for (int k = 0; k < m; ++k) idx[k] = k + 1;
+        }
+        for (int k = 0; k < m; ++k) w[k] = 1.0 / th[idx[k] - 1];
     } else {
         for (int k = 0; k < m; ++k) w[k] = opt.omega;
     }
+}
+
+double fcg_auto_ratio(Chunk& c)
+{
+    const double a = 30.0 * std::sqrt(op_mean_diag(c));
+    return std::min(10000.0, std::max(30.0, a));
 }
 
 long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, bool allow_fused, bool dot_w)
@@ -247,12 +277,8 @@ long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, boo
         if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
         return 0;
     }
-    if (allow_fused && fcg_smooth_tb_supported(c, sweeps)) {
-        const long nb = op_fcg_smooth_tb(c, from_zero ? nullptr : c.f[F_Z], c.f[F_SD], weights, sweeps,
-                                         dot_w ? c.f[F_W] : nullptr);
-        std::swap(c.f[F_Z], c.f[F_SD]);
-        return nb;
-    }
+    if (allow_fused && fcg_smooth_tb_supported(c, sweeps))
+        return op_fcg_sweep_passes(c, weights, sweeps, from_zero, dot_w, false);
     if (from_zero) TL_HIP(hipMemsetAsync(c.f[F_Z], 0, sizeof(double) * (size_t)c.g.alloc, c.stream));
     const long n = (long)c.g.nx * c.g.ny * c.g.nz;
     for (int k = 0; k < sweeps; ++k) {
@@ -548,7 +574,11 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     if (TL_DIM != 2 || c.g.nx % 8 || c.g.ny % 8 || c.preconditioner || (!L && opt.network)) return false;
     if (opt.smooth > 16) throw Error("tl_fcg_smooth: at most 16 sweeps");
     double sw[16] = {};
-    if (opt.smooth > 0) fcg_sweep_weights(opt, sw);
+    if (opt.smooth > 0) {
+        FcgOptions o = opt;
+        if (o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c);
+        fcg_sweep_weights(o, sw);
+    }
     const long n = (long)c.g.nx * c.g.ny;
     float *pf = nullptr, *pz = nullptr;
     hipStream_t keep = nullptr;
@@ -651,9 +681,7 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
         long beta_blocks = 0;  // z . w left by the preconditioner pass
         const bool beta_rrn = r_pending;
         if (r_pending) {
-            beta_blocks = op_fcg_smooth_tb(c, nullptr, c.f[F_SD], sw, opt.smooth, c.f[F_W], c.r_alt);
-            std::swap(c.f[F_Z], c.f[F_SD]);
-            std::swap(c.f[F_R], c.r_alt);
+            beta_blocks = op_fcg_sweep_passes(c, sw, opt.smooth, true, true, true);
             r_pending = false;
         } else if (opt.coarse) {
             // the two-grid preconditioner: z = S r from zero, the coarse

@@ -28,16 +28,43 @@ def test_fixed_weight(hip):
     assert np.array_equal(weights(hip, 7, 0.65, 0.0), [0.65] * 7)
 
 
-@pytest.mark.parametrize("m,ratio", [(4, 30.0), (4, 4.0), (3, 100.0), (8, 30.0), (1, 2.0)])
+def partial_growth(w, x):
+    """Largest |prod_{j <= k} (1 - w_j x)| over k and x (the iterate's
+    growth) and over the suffixes (what a rounding error made before the
+    last sweeps is multiplied by)."""
+    pre = suf = 0.0
+    q = np.ones_like(x)
+    for wk in w:
+        q = q * (1.0 - wk * x)
+        pre = max(pre, np.abs(q).max())
+    q = np.ones_like(x)
+    for wk in w[::-1]:
+        q = q * (1.0 - wk * x)
+        suf = max(suf, np.abs(q).max())
+    return pre, suf
+
+
+@pytest.mark.parametrize("m,ratio", [(4, 30.0), (4, 4.0), (3, 100.0), (8, 30.0), (1, 2.0), (16, 3000.0), (12, 1000.0)])
 def test_chebyshev_weights(hip, m, ratio):
     w = weights(hip, m, 0.8, ratio)
     lo, hi = 2.0 / ratio, 2.0
     sigma = lambda x: (hi + lo - 2.0 * x) / (hi - lo)
-    # 1 / w_k are the zeros of T_m(sigma(x)), in decreasing order
-    assert np.all(np.diff(1.0 / w) < 0)
+    # 1 / w_k are the zeros of T_m(sigma(x)), each once
     assert np.all((1.0 / w > lo) & (1.0 / w < hi))
+    assert np.all(np.diff(np.sort(1.0 / w)) > 0)
     tm = np.polynomial.chebyshev.Chebyshev.basis(m)
     assert np.max(np.abs(tm(sigma(1.0 / w)))) < 1e-12
+    pre, suf = partial_growth(w, np.linspace(0.0, hi, 20001))
+    if (m & (m - 1)) == 0:
+        # the pairing order: both growths stay small (decreasing order: the
+        # iterate within 1, but 5.5e7 on the suffixes for m = 16)
+        assert pre < 200 and suf < 1000, (pre, suf)
+        if m >= 4:
+            # each sweep k < m/2 of the first half pairs with the mirrored root
+            th = 1.0 / w
+            assert th[0] == th.max() and th[1] == th.min()
+    else:
+        assert np.all(np.diff(1.0 / w) < 0) and pre <= 1.0 + 1e-12
     x = np.linspace(lo, hi, 20001)
     q = np.prod(1.0 - np.outer(x, w), axis=1)
     bound = 1.0 / np.cosh(m * np.arccosh(sigma(0.0)))

@@ -186,6 +186,10 @@ def test_fcg_sweeps_only(hip, oracle, kind, n, kw):
         deck = decks.two_rect_2d(n, "ppcg", steps=2, extra=("tl_check_result",), **kw)
     ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
     fdeck = deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network")
+    if cap is not None:
+        # 4 sweeps of a fixed weight, which need more than the cap (16
+        # Chebyshev-weighted sweeps meet eps within it)
+        fdeck = fdeck.replace("tl_fcg_no_network", "tl_fcg_no_network\ntl_fcg_omega=0.8")
     print(f"sweeps-only FCG {kind} {n} {kw} cap={cap}", flush=True)
     got = hip.run_deck(fdeck, dim=2)
     for sg, so in zip(got["steps"], ref["steps"]):
@@ -201,12 +205,12 @@ def test_fcg_sweeps_only(hip, oracle, kind, n, kw):
 
 @pytest.mark.parametrize("kind,n", [("two_rect", 256), ("shapes", 192)])
 def test_fcg_chebyshev_sweeps(hip, oracle, kind, n):
-    """The sweeps with Chebyshev step weights on [2/100, 2] (the default;
-    tl_fcg_cheby) instead of the fixed 0.8 (tl_fcg_omega=0.8): the same four
-    sweeps per iteration, the same
+    """The 16 sweeps with Chebyshev step weights on [2/a, 2] (the default, a
+    from the operator: 5 sqrt(mean diag A)) instead of 4 of the fixed 0.8
+    (tl_fcg_omega=0.8): the same
     bar as the fixed weight (eps on the true residual, FP64 PPCG's totals
     to 1e-7), and fewer iterations than the fixed weight on every timestep
-    (the preconditioned operator's small eigenvalues are raised about 4x)."""
+    (the preconditioned operator's bulk spectrum is squeezed around 1)."""
     import decks
     if kind == "shapes":
         deck = decks.random_shapes_2d(n, 7, steps=2, extra=("tl_check_result",))
@@ -214,11 +218,12 @@ def test_fcg_chebyshev_sweeps(hip, oracle, kind, n):
         deck = decks.two_rect_2d(n, "ppcg", steps=2, extra=("tl_check_result",))
     ref = oracle.run_deck(deck, dim=2, reduce_parts=8, threads=8)
     jac = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network\ntl_fcg_omega=0.8"), dim=2)
-    che = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network\ntl_fcg_cheby"), dim=2)
+    che = hip.run_deck(deck.replace("tl_use_ppcg", "tl_use_fcg_learned\ntl_fcg_no_network"), dim=2)
     for sc, sj, so in zip(che["steps"], jac["steps"], ref["steps"]):
         print(f"  step {sc['step']}: Chebyshev {sc['fcg_iters']} it, fixed weight {sj['fcg_iters']} it, "
               f"exact {sc['exact_error']:.3e}", flush=True)
         assert sc["fcg"] == 1 and sc["fcg_fallback"] == 0
+        assert 30.0 <= sc["fcg_ratio"] <= 10000.0 and sj["fcg_ratio"] == 0.0
         assert sc["exact_error"] < 1e-10
         for key in ("vol", "mass", "ie", "temp"):
             assert abs(sc[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sc[key], so[key])
