@@ -183,7 +183,7 @@ int tl_jacobi_iterate_fused(const int* chunk, int count, double* errors);
  * (fused = 1; 2D, 4 sweeps) or one launch per sweep (fused = 0).  Returns 0,
  * or -1 with tl_last_error() set. */
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused);
-/* The same with a weight per sweep, weights[0 .. sweeps) (1 to 16 sweeps,
+/* The same with a weight per sweep, weights[0 .. sweeps) (1 to 32 sweeps,
  * each weight positive): sweep s applies z += (r - A z) ((w_0 / D) f_s),
  * f_s = w_s / w_0, the form the Chebyshev-weighted sweeps
  * (tl_fcg_cheby) take.  No counterpart in the reference. */

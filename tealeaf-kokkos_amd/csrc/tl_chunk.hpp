@@ -355,7 +355,7 @@ struct FcgOptions {
     int net_every = 1;
     int net_iters = 0;  // > 0: the network only in the first net_iters iterations, sweeps alone after
     bool net_restart = false;  // start a new recurrence (p = z) after the last network iteration
-    int smooth = 16;  // four temporal-blocked passes of 4
+    int smooth = 32;  // eight temporal-blocked passes of 4
     double omega = 0.8;
     double cheby_ratio = -1.0;
     // the network as the coarse-grid solve of a two-grid preconditioner
@@ -395,13 +395,14 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
 // order.  The residual polynomial prod_k (1 - x / theta_k) then stays below
 // 1 in magnitude on that interval, and D^-1 A's spectrum lies in (0, 2).
 void fcg_sweep_weights(const FcgOptions& opt, double* w);
-// The Chebyshev interval ratio for the chunk's operator: 30 sqrt(mean diag A),
-// within [30, 10000].  D^-1 A's bulk spectrum widens with the face
-// coefficients (dt / dx^2 times the conductivity); this fits the best ratio
-// measured for 16 sweeps from 192^2 to 8192^2 (about 100 at 192^2, 300 at
-// 512^2, 3000 at 4096^2 and 8192^2, with a flat optimum from 1000 to 10000
-// at the largest).
-double fcg_auto_ratio(Chunk& c);
+// The Chebyshev interval ratio for the chunk's operator and m sweeps:
+// 30 (m / 16)^2 sqrt(mean diag A), within [30, 30000].  D^-1 A's bulk
+// spectrum widens with the face coefficients (dt / dx^2 times the
+// conductivity), and a higher degree covers a wider interval; this fits the
+// best ratios measured from 192^2 to 8192^2 (for 16 sweeps about 100 at
+// 192^2, 300 at 512^2, 3000 at 4096^2 and 8192^2, a flat optimum from 1000
+// to 10000 at the largest; for 32 sweeps 3x that a little better than 1x).
+double fcg_auto_ratio(Chunk& c, int sweeps);
 // Sweep q's weight over the first one, the factor the sweep kernels apply to
 // w_1 / D (exactly 1 for q = 0 and for equal weights).
 inline double sweep_ratio(const double* w, int q) { return w[q] / w[0]; }

@@ -664,7 +664,7 @@ StepInfo tea_leaf(Chunk& c, const Deck& d, double dt)
         double err = 0.0;
         si.fcg = true;
         FcgOptions o = d.fcg;
-        if (TL_DIM == 2 && o.smooth > 0 && o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c);
+        if (TL_DIM == 2 && o.smooth > 0 && o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c, o.smooth);
         si.fcg_ratio = o.cheby_ratio;
         fcg_done = fcg_learned_solve(c, learner, o, d.eps, max_iters, it, err);
         si.fcg_iters = it;

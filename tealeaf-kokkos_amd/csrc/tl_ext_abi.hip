@@ -187,7 +187,7 @@ int tl_fcg_smooth_weights(const int* chunk, int sweeps, const double* weights, i
     int rc = -1;
     guarded([&] {
         Chunk* c = stencil_chunk(chunk);
-        if (sweeps < 1 || sweeps > 16 || !weights) throw Error("tl_fcg_smooth_weights: 1 to 16 sweeps and their weights");
+        if (sweeps < 1 || sweeps > 32 || !weights) throw Error("tl_fcg_smooth_weights: 1 to 32 sweeps and their weights");
         for (int k = 0; k < sweeps; ++k)
             if (!(weights[k] > 0.0 && std::isfinite(weights[k]))) throw Error("sweep weights must be positive");
         if (fused && !fcg_smooth_tb_supported(*c, sweeps)) throw Error("no temporal-blocked pass for these sweeps");
@@ -205,7 +205,7 @@ int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* w
 {
     int rc = -1;
     guarded([&] {
-        if (sweeps < 1 || sweeps > 16 || !weights) throw Error("tl_fcg_sweep_weights: 1 to 16 sweeps");
+        if (sweeps < 1 || sweeps > 32 || !weights) throw Error("tl_fcg_sweep_weights: 1 to 32 sweeps");
         if (cheby_ratio != 0.0 && !(cheby_ratio > 1.0))
             throw Error("the Chebyshev ratio must exceed 1 (the ratio from the operator needs a chunk)");
         FcgOptions o;
@@ -221,8 +221,8 @@ int tl_fcg_sweep_weights(int sweeps, double omega, double cheby_ratio, double* w
 // The same with one weight for every sweep.
 int tl_fcg_smooth(const int* chunk, int sweeps, double omega, int from_zero, int fused)
 {
-    double w[16];
-    for (int k = 0; k < 16; ++k) w[k] = omega;
+    double w[32];
+    for (int k = 0; k < 32; ++k) w[k] = omega;
     return tl_fcg_smooth_weights(chunk, sweeps, w, from_zero, fused);
 }
 

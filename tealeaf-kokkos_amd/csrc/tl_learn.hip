@@ -263,10 +263,11 @@ void fcg_sweep_weights(const FcgOptions& opt, double* w)
     }
 }
 
-double fcg_auto_ratio(Chunk& c)
+double fcg_auto_ratio(Chunk& c, int sweeps)
 {
-    const double a = 30.0 * std::sqrt(op_mean_diag(c));
-    return std::min(10000.0, std::max(30.0, a));
+    const double m = sweeps / 16.0;
+    const double a = 30.0 * m * m * std::sqrt(op_mean_diag(c));
+    return std::min(30000.0, std::max(30.0, a));
 }
 
 long fcg_smooth(Chunk& c, int sweeps, const double* weights, bool from_zero, bool allow_fused, bool dot_w)
@@ -572,11 +573,11 @@ bool fcg_learned_solve(Chunk& c, Learner* L, const FcgOptions& opt, double eps, 
     iters = 0;
     error = 0.0;
     if (TL_DIM != 2 || c.g.nx % 8 || c.g.ny % 8 || c.preconditioner || (!L && opt.network)) return false;
-    if (opt.smooth > 16) throw Error("tl_fcg_smooth: at most 16 sweeps");
-    double sw[16] = {};
+    if (opt.smooth > 32) throw Error("tl_fcg_smooth: at most 32 sweeps");
+    double sw[32] = {};
     if (opt.smooth > 0) {
         FcgOptions o = opt;
-        if (o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c);
+        if (o.cheby_ratio < 0.0) o.cheby_ratio = fcg_auto_ratio(c, o.smooth);
         fcg_sweep_weights(o, sw);
     }
     const long n = (long)c.g.nx * c.g.ny;

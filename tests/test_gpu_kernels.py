@@ -541,7 +541,7 @@ def test_fcg_smooth_fused(hip, shape, from_zero):
 
 @pytest.mark.parametrize("shape", [(37, 23), (5, 4), (130, 41), (64, 64)])
 @pytest.mark.parametrize("from_zero", [True, False])
-@pytest.mark.parametrize("m,ratio", [(4, 30.0), (8, 300.0), (12, 1000.0), (16, 3000.0)])
+@pytest.mark.parametrize("m,ratio", [(4, 30.0), (8, 300.0), (12, 1000.0), (16, 3000.0), (32, 13000.0)])
 def test_fcg_smooth_chebyshev_weights(hip, shape, from_zero, m, ratio):
     """The sweeps with a weight per sweep (the Chebyshev step weights of
     [2 / ratio, 2] from tl_fcg_sweep_weights, up to 182 for 16 sweeps): the

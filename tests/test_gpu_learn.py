@@ -205,8 +205,8 @@ def test_fcg_sweeps_only(hip, oracle, kind, n, kw):
 
 @pytest.mark.parametrize("kind,n", [("two_rect", 256), ("shapes", 192)])
 def test_fcg_chebyshev_sweeps(hip, oracle, kind, n):
-    """The 16 sweeps with Chebyshev step weights on [2/a, 2] (the default, a
-    from the operator: 5 sqrt(mean diag A)) instead of 4 of the fixed 0.8
+    """The 32 sweeps with Chebyshev step weights on [2/a, 2] (the default, a
+    from the operator: 120 sqrt(mean diag A)) instead of 4 of the fixed 0.8
     (tl_fcg_omega=0.8): the same
     bar as the fixed weight (eps on the true residual, FP64 PPCG's totals
     to 1e-7), and fewer iterations than the fixed weight on every timestep
@@ -223,7 +223,7 @@ def test_fcg_chebyshev_sweeps(hip, oracle, kind, n):
         print(f"  step {sc['step']}: Chebyshev {sc['fcg_iters']} it, fixed weight {sj['fcg_iters']} it, "
               f"exact {sc['exact_error']:.3e}", flush=True)
         assert sc["fcg"] == 1 and sc["fcg_fallback"] == 0
-        assert 30.0 <= sc["fcg_ratio"] <= 10000.0 and sj["fcg_ratio"] == 0.0
+        assert 30.0 <= sc["fcg_ratio"] <= 30000.0 and sj["fcg_ratio"] == 0.0
         assert sc["exact_error"] < 1e-10
         for key in ("vol", "mass", "ie", "temp"):
             assert abs(sc[key] - so[key]) <= FCG_RTOL * abs(so[key]), (key, sc[key], so[key])
