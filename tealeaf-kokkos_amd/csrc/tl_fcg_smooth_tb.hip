@@ -108,7 +108,12 @@ __device__ __forceinline__ void f_load(const FArgs& A, const FCtx& c, ConstU* t,
     v.ky_in = y > HALO && y <= A.ny + 1;
 }
 
-template <int K, bool ZERO, bool DOTW, int RM, int PH>
+// WARM: the first block of a wave (y = ylo + PH = Y0 - K + PH).  The strip's
+// output rows depend on sweep s only at rows >= Y0 - (K - s), i.e. from
+// round PH = 2 s on, so the arithmetic of the sweeps s > PH / 2 is left out;
+// every load and store stays, so that the memory operations in flight at
+// the loop head are those of every other block.
+template <int K, bool ZERO, bool DOTW, int RM, int PH, bool WARM>
 __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     using R = tb::TBRing<K>;
@@ -178,13 +183,17 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         const double2 kx = nkx, ky = nky, r = nr;
         const double kxe1 = nkxe;
         const double2 wd = s == 1 ? st.wd1 : nwd;
-        if (s < K) {
+        const bool skip = WARM && 2 * s > PH;  // a constant once the loop is unrolled
+        if (s < K && !(WARM && 2 * (s + 1) > PH)) {
             nkx = kxr[R::row(PH, s + 1) * WAVE];
             nky = kyr[R::row(PH, s + 1) * WAVE];
             nr = rr[R::row(PH, s + 1) * WAVE];
             nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
             if (s >= 2) nwd = wdr[R::row(PH, s + 1) * WAVE];
         }
+        if (skip && s < K) continue;
+        double zn0 = 0.0, zn1 = 0.0;
+        if (!skip) {
         const double c0 = st.zw[s - 1][S1][0], c1 = st.zw[s - 1][S1][1];
         const double s0 = st.zw[s - 1][S2][0], s1 = st.zw[s - 1][S2][1];
         const double n0 = st.zw[s - 1][S0][0], n1 = st.zw[s - 1][S0][1];
@@ -201,10 +210,11 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
         az0 += ky.x * (c0 - s0);
         az1 += ky.y * (c1 - s1);
         const double fs = c.f[s - 1];
-        const double zn0 = c0 + (r.x - az0) * (wd.x * fs);
-        const double zn1 = c1 + (r.y - az1) * (wd.y * fs);
+        zn0 = c0 + (r.x - az0) * (wd.x * fs);
+        zn1 = c1 + (r.y - az1) * (wd.y * fs);
         kyn0 = ky.x;
         kyn1 = ky.y;
+        }
         if (s < K) {
             st.zw[s][S0][0] = zn0;
             st.zw[s][S0][1] = zn1;
@@ -226,13 +236,13 @@ __device__ __forceinline__ void f_round(const FArgs& A, const FCtx& c, FState<K>
     }
 }
 
-template <int K, bool ZERO, bool DOTW, int RM, int PH>
+template <int K, bool ZERO, bool DOTW, int RM, int PH, bool WARM>
 __device__ __forceinline__ void f_rounds(const FArgs& A, const FCtx& c, FState<K>& st, int y, ConstU* tab)
 {
     if constexpr (PH < tb::TB_UNROLL) {
-        f_round<K, ZERO, DOTW, RM, PH>(A, c, st, y + PH, tab);
+        f_round<K, ZERO, DOTW, RM, PH, WARM>(A, c, st, y + PH, tab);
         __builtin_amdgcn_sched_barrier(0);
-        f_rounds<K, ZERO, DOTW, RM, PH + 1>(A, c, st, y, tab);
+        f_rounds<K, ZERO, DOTW, RM, PH + 1, WARM>(A, c, st, y, tab);
     }
 }
 
@@ -313,8 +323,9 @@ __global__ __launch_bounds__(WAVE) void k_fcg_smooth_tb(Geom g, FArgs A)
     f_load<K, ZERO, DOTW, RM>(A, c, tab, 1, c.ylo + 1, st.pf[1]);
     __builtin_amdgcn_s_waitcnt(tb::VMCNT0);
     const int yend = c.yhi + K;
-    for (int y = c.ylo; y <= yend; y += tb::TB_UNROLL)
-        f_rounds<K, ZERO, DOTW, RM, 0>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
+    f_rounds<K, ZERO, DOTW, RM, 0, true>(A, c, st, c.ylo, tab + 4);
+    for (int y = c.ylo + tb::TB_UNROLL; y <= yend; y += tb::TB_UNROLL)
+        f_rounds<K, ZERO, DOTW, RM, 0, false>(A, c, st, y, tab + 2 * (y - c.ylo + 2));
     if (RM != 0) {
         double v[3] = {st.acc, st.acc_rr, st.acc_zr};
         block_partial<3, 1>(v, Reduce{A.partials});
