@@ -87,8 +87,11 @@ __device__ __forceinline__ double from_east(double v)
 // columns (c0 = X0 - L' + 2 lane, c1 = c0 + 1), so a wave spans 128 columns
 // and writes the 128 - 2L' in its middle.  Register windows rotate by round
 // number instead of moving data: within an unrolled block of rounds the
-// slot of every window is a compile-time constant.  The windows that span
-// L+1 rows (u being accumulated, kx, ky) live in LDS rings indexed by row.
+// slot of every window is a compile-time constant.  u being accumulated
+// passes from level to level in a two-row register window per level, as r
+// does; the windows that span L+1 rows (kx, ky and the operator's
+// diagonal, formed once per row by level 1) live in LDS rings indexed by
+// row.
 // One prefetched row: r, w (first pass) or sd, u, p and ky as the aligned
 // pair that holds the lane's cells, kx at the lane's two faces.
 struct TBRow {
@@ -103,6 +106,7 @@ template <int L>
 struct TBState {
     double sdw[L][3][2];  // sd of level s, slot (round - ylo) % 3, column
     double rw[L][2][2];   // r of level s, slot (round - ylo) % 2, column
+    double2 uw[L][2];     // u after level s, slot (round - ylo) % 2
     TBRow pf[TB_PF];      // prefetched rows
     double acc;
     double ca[L], cb[L];  // step coefficients (wave-uniform)
@@ -121,8 +125,8 @@ struct TBCtx {
     bool hi0, hi1;
     bool out0, out1;
     double alpha, theta;
-    double* ur;  // LDS rings [ring rows][64][2]
-    double* kxr;
+    double* kxr;  // LDS rings [ring rows][64][2]
+    double* dr;
     double* kyr;
 };
 
@@ -256,15 +260,16 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
     constexpr int R0 = PH % 2, R1 = (PH + 1) % 2;                      // rounds y, y-1
     const int lane = threadIdx.x;
     TBRow& v = st.pf[PH % TB_PF];
-    double2* ur = reinterpret_cast<double2*>(c.ur) + lane;
     double2* kxr = reinterpret_cast<double2*>(c.kxr) + lane;
     double2* kyr = reinterpret_cast<double2*>(c.kyr) + lane;
+    double2* dr = reinterpret_cast<double2*>(c.dr) + lane;
     // kx at the east lane's first face: the next element of the ring row
     const double* kxe = c.kxr + 2 * lane + 2;
 
     // inputs of level 1 (row y - 1)
-    double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE], nu = ur[R::row(PH, 1) * WAVE];
+    double2 nkx = kxr[R::row(PH, 1) * WAVE], nky = kyr[R::row(PH, 1) * WAVE];
     double nkxe = kxe[2 * WAVE * R::row(PH, 1)];
+    double2 nd = make_double2(0.0, 0.0);
 
     // level 0 at row y: CGCalcUr (r -= alpha w, u += alpha p) and PPCGInitSd,
     // evaluated on the loaded pair (elementwise), then assigned to the cells
@@ -288,7 +293,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         }
         // rows outside the strip accumulate nothing that is stored
         const bool yin = y >= c.Y0 && y <= c.yhi;
-        ur[R::row(PH, 0) * WAVE] = make_double2(yin ? pick(uu, c.hi0) : 0.0, yin ? pick(uu, c.hi1) : 0.0);
+        st.uw[0][R0] = make_double2(yin ? pick(uu, c.hi0) : 0.0, yin ? pick(uu, c.hi1) : 0.0);
         kxr[R::row(PH, 0) * WAVE] = make_double2(v.kx0, v.kx1);
         kyn0 = pick(v.ky, c.hi0);
         kyn1 = pick(v.ky, c.hi1);
@@ -302,14 +307,15 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
         const int z = y - s;
         const double2 kx = nkx, ky = nky;
         const double kxe1 = nkxe;
-        double2 u = nu;
+        const double2 dp = nd;
+        double2 u = st.uw[s - 1][R1];
         const double a = st.ca[s - 1], b = st.cb[s - 1];
         const bool skip = WARM && 2 * s > PH;  // a constant once the loop is unrolled
         if (s < L && !(WARM && 2 * (s + 1) > PH)) {
             nkx = kxr[R::row(PH, s + 1) * WAVE];
             nky = kyr[R::row(PH, s + 1) * WAVE];
-            nu = ur[R::row(PH, s + 1) * WAVE];
             nkxe = kxe[2 * WAVE * R::row(PH, s + 1)];
+            nd = dr[R::row(PH, s + 1) * WAVE];
         }
         double rn0 = 0.0, rn1 = 0.0;
         if (!skip) {
@@ -319,9 +325,16 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
             // x-neighbours: c0's west is the previous lane's c1, c1's east the next lane's c0
             const double w0 = from_west(c1), e1 = from_east(c0);
             const double e0 = c1, w1 = c0;
-            // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z)
-            const double m0 = (1.0 + (kx.y + kx.x) + (kyn0 + ky.x)) * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
-            const double m1 = (1.0 + (kxe1 + kx.y) + (kyn1 + ky.y)) * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
+            // SMVP (ext_shared.hpp:37-41) at (c0, z) and (c1, z); the
+            // diagonal is the same at every level, so level 1 forms it for
+            // row z and the levels after it read it back from the ring
+            double2 d = dp;
+            if (s == 1) {
+                d = make_double2(1.0 + (kx.y + kx.x) + (kyn0 + ky.x), 1.0 + (kxe1 + kx.y) + (kyn1 + ky.y));
+                dr[R::row(PH, 1) * WAVE] = d;
+            }
+            const double m0 = d.x * c0 - (kx.y * e0 + kx.x * w0) - (kyn0 * n0 + ky.x * s0);
+            const double m1 = d.y * c1 - (kxe1 * e1 + kx.y * w1) - (kyn1 * n1 + ky.y * s1);
             kyn0 = ky.x;
             kyn1 = ky.y;
             // PPCGCalcU: r -= A sd; u += sd
@@ -335,7 +348,7 @@ __device__ __forceinline__ void tb_round(const TBArgs& A, const TBCtx& c, TBStat
                 st.sdw[s][S0][1] = a * c1 + b * rn1;
                 st.rw[s][R0][0] = rn0;
                 st.rw[s][R0][1] = rn1;
-                ur[R::row(PH, s) * WAVE] = u;
+                st.uw[s][R0] = u;
             }
         }
         if (s == L) {
@@ -381,6 +394,7 @@ __device__ __forceinline__ void tb_wave(const TBArgs& A, const TBCtx& c)
             st.sdw[s][0][q] = st.sdw[s][1][q] = st.sdw[s][2][q] = 0.0;
             st.rw[s][0][q] = st.rw[s][1][q] = 0.0;
         }
+        st.uw[s][0] = st.uw[s][1] = make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int k = 0; k < TB_PF; ++k) {
@@ -419,11 +433,11 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
 {
     if (gated_off(A.red)) return;
     using R = TBRing<L>;
-    __shared__ double2 u_ring[R::N * WAVE];
     // one more element: the last lane's east face reads past the last row
     // (that lane owns no cell, so the value is never used)
     __shared__ double2 kx_ring[R::N * WAVE + 1];
     __shared__ double2 ky_ring[R::N * WAVE];
+    __shared__ double2 d_ring[R::N * WAVE];
     const int lane = threadIdx.x;
     const long t = xcd_tile(blockIdx.x, A.nblocks);
     const int tx = (int)(t % A.tiles_x), ty = (int)(t / A.tiles_x);
@@ -463,9 +477,9 @@ __global__ __launch_bounds__(WAVE) void k_ppcg_tb(Geom g, TBArgs A)
     c.yhi = min(c.Y0 + A.rows - 1, c.ny + 1);
     c.alpha = FIRST ? A.ds->alpha : 0.0;
     c.theta = A.theta;
-    c.ur = reinterpret_cast<double*>(u_ring);
     c.kxr = reinterpret_cast<double*>(kx_ring);
     c.kyr = reinterpret_cast<double*>(ky_ring);
+    c.dr = reinterpret_cast<double*>(d_ring);
     tb_wave<L, FIRST, LAST>(A, c);
 }
 
